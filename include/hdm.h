@@ -1,0 +1,220 @@
+/*
+ * hdm.h - C ABI of the MI355X Hausdorff-distance engine (libhdm.so).
+ *
+ * Two 3-D masks plus their dims go in; the exact Hausdorff distance (HD) and
+ * the exact average HD (64-bit integer sum and count, divided in FP64 on the
+ * host) come out.  6-connectivity morphological dilation, iterated until each
+ * mask covers the other, as in the reference engine.
+ *
+ * Conventions (all entry points):
+ *   - volumes are dense, x fastest: index = x + nx*(y + ny*z)
+ *     (reference: hello.cu:453, 615; forBench/Volume.h:51-54);
+ *   - a voxel is "on" iff mask[v] == label (hello.cu:453-454, 615-616);
+ *   - FP = segm & ~gold, FN = gold & ~segm (hello.cu:624-625);
+ *   - first-hit d(v) of an FP voxel = least t >= 1 with v in D^t(gold),
+ *     of an FN voxel = least t with v in D^t(segm) (hello.cu:1590-1626, 1618);
+ *   - HD = max d(v); average HD = sum d(v) / |FP u FN|.
+ *   - return value 0 = HDM_OK, negative = HDM_E_* (the library never exits
+ *     the process and never resets the device).
+ *   - device pointers are plain HIP device addresses; streams are hipStream_t
+ *     passed as void*.  NULL stream = the library's own per-device stream.
+ *
+ * Differences from the reference that are deliberate (its quirks are not
+ * reproduced; results follow the mathematical definitions above):
+ *   identical masks give HD = 0 (reference: 1, hello.cu:1893);
+ *   an empty mask with a non-empty partner returns HDM_E_EMPTY_MASK
+ *   (reference: hangs, hello.cu:1887); dims need not be tile multiples
+ *   (reference floors them, hello.cu:921-923); robust termination compares
+ *   integers, cut = ceil(r * count) (reference: float product, hello.cu:1751).
+ */
+#ifndef HDM_H
+#define HDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HDM_ABI_VERSION 1
+
+/* ---- error codes ------------------------------------------------------- */
+#define HDM_OK 0
+#define HDM_E_INVALID (-1)    /* bad argument (null pointer, dims, label type) */
+#define HDM_E_EMPTY_MASK (-2) /* one mask empty while the other has voxels     */
+#define HDM_E_HIP (-3)        /* a HIP runtime call failed                      */
+#define HDM_E_NOMEM (-4)      /* device allocation failed                       */
+#define HDM_E_TIMEOUT (-5)    /* device grid barrier exceeded its spin limit    */
+#define HDM_E_BUDGET (-6)     /* request larger than the server's HBM budget    */
+#define HDM_E_NODEVICE (-7)   /* no HIP device visible                          */
+#define HDM_E_STUCK (-8)      /* work queue drained with targets uncovered      */
+#define HDM_E_CAPACITY (-9)   /* a fixed-size table is full                     */
+
+/* ---- option enums ------------------------------------------------------ */
+#define HDM_PTR_DEVICE 0 /* gold/segm/map are device pointers           */
+#define HDM_PTR_HOST 1   /* gold/segm/map are host pointers (copied)    */
+
+#define HDM_DTYPE_U8 0
+#define HDM_DTYPE_I32 1
+
+/* Tile (block) shapes, x * y * z voxels.  One wavefront lane owns one x-row
+ * of the tile (64 or 32 bits).  Packed layout is tile-major. */
+#define HDM_TILE_AUTO 0
+#define HDM_TILE_64x8x8 1
+#define HDM_TILE_64x4x4 2
+#define HDM_TILE_64x8x4 3
+#define HDM_TILE_32x8x8 4
+#define HDM_TILE_COUNT 5
+
+typedef struct hdm_opts {
+  double robustness;  /* (0,1]; 1 = exact HD.  cut = ceil(r * |FP|), ceil(r * |FN|) */
+  int32_t ptr_kind;   /* HDM_PTR_DEVICE or HDM_PTR_HOST                               */
+  int32_t tile;       /* HDM_TILE_*                                                    */
+  int32_t grid_ctas;  /* persistent-kernel workgroups; 0 = occupancy-derived           */
+  int32_t chunk;      /* queue entries a wave takes per claim; 0 = default             */
+  int32_t crop;       /* 1 (default) = restrict dilation to the tile bounding box      */
+  int32_t reserved0;
+  void *stream;       /* hipStream_t or NULL                                           */
+  uint16_t *map_out;  /* optional nx*ny*nz first-hit map: d(v) on FP u FN, 0 elsewhere */
+  uint32_t *hist_out; /* optional 2*hist_len first-hit histograms (FP then FN), host   */
+  int32_t hist_len;   /* length of each histogram in hist_out                          */
+  int32_t reserved1;
+} hdm_opts;
+
+typedef struct hdm_result {
+  int32_t hd;         /* Hausdorff distance (robust HD when robustness < 1)         */
+  int32_t hd_fp;      /* max first-hit over FP (gold dilation) up to its cut        */
+  int32_t hd_fn;      /* max first-hit over FN (segm dilation) up to its cut        */
+  int32_t iterations; /* dilation iterations the device loop executed               */
+  uint64_t sum;       /* sum_fp + sum_fn                                             */
+  uint64_t count;     /* cnt_fp + cnt_fn                                             */
+  uint64_t sum_fp, cnt_fp, sum_fn, cnt_fn;
+  uint64_t n_gold, n_segm; /* voxels on in each mask                                */
+  uint64_t n_fp, n_fn;     /* |FP|, |FN| before any robust cut                       */
+  double avg_hd;      /* (double)sum / (double)count, 0 when count == 0             */
+  uint64_t queue_total; /* sum over iterations of queued (tile, mask) entries        */
+  uint32_t queue_max;   /* largest per-iteration queue                              */
+  uint32_t tiles;       /* tiles of the (cropped) tile grid                         */
+} hdm_result;
+
+/* ---- library ----------------------------------------------------------- */
+int hdm_abi_version(void);
+const char *hdm_strerror(int code);
+void hdm_opts_init(hdm_opts *opts);
+/* Number of visible HIP devices (0 when none); never fails. */
+int hdm_device_count(void);
+/* Device bytes one pair of the given dims needs (workspace, excluding inputs). */
+uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
+
+/* ---- single pair -------------------------------------------------------
+ * Replaces: template<T> int getHausdorffDistance_CUDA_Generic(T* goldStandard,
+ *   T* algoOutput, int WIDTH, int HEIGHT, int DEPTH, float robustnessPercent,
+ *   bool resIterNeeded, T numberToLookFor, bool res3DNeeded)
+ *   (hello.cu:2025-2062), for T = uint8_t.  The int it returned is out->hd. */
+int hdm_hausdorff_u8(const uint8_t *gold, const uint8_t *segm, int nx, int ny, int nz,
+                     uint8_t label, const hdm_opts *opts, hdm_result *out);
+
+/* Same, T = int (the reference instantiates its main pass for int,
+ * hello.cu:1999). */
+int hdm_hausdorff_i32(const int32_t *gold, const int32_t *segm, int nx, int ny, int nz,
+                      int32_t label, const hdm_opts *opts, hdm_result *out);
+
+/* Replaces: int HausdorffDistance::computeDistance(Volume* img1, Volume* img2)
+ * (forBench/HausdorffDistance.cuh:17, .cu:101-187) for bool volumes
+ * (forBench/Volume.h:10).  A voxel is on iff its byte is non-zero.  Uses
+ * 6-connectivity (the reference's CHEBYSHEV 1 18-neighbour element is a
+ * different metric). */
+int hdm_hausdorff_bool(const uint8_t *img1, const uint8_t *img2, int nx, int ny, int nz,
+                       const hdm_opts *opts, hdm_result *out);
+
+/* Replaces: getHausdorffDistance_CUDA_3Dres_local (hello.cu:2178-2241) and
+ * getHausdorffDistance_CUDA_FullResList_local (hello.cu:2065-2103): writes
+ * d(v) for every FP and FN voxel into map (nx*ny*nz uint16, x fastest),
+ * 0 elsewhere; with robustness < 1 only voxels at d <= the cut are kept. */
+int hdm_first_hit_map_u8(const uint8_t *gold, const uint8_t *segm, int nx, int ny, int nz,
+                         uint8_t label, const hdm_opts *opts, uint16_t *map,
+                         hdm_result *out);
+
+/* ---- batched pairs -----------------------------------------------------
+ * No single counterpart in the reference (it has one pair per call,
+ * hello.cu:2025): many pairs of uneven size go through one persistent
+ * launch; the create call captures pack + metadata + persistent dilation +
+ * finalize into one hipGraph, which run() replays. */
+typedef struct hdm_pair {
+  const void *gold; /* device pointers */
+  const void *segm;
+  int32_t nx, ny, nz;
+  int32_t dtype; /* HDM_DTYPE_* */
+  int32_t label;
+  int32_t reserved;
+} hdm_pair;
+
+typedef struct hdm_batch hdm_batch;
+
+int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *opts, int use_graph,
+                     hdm_batch **out);
+int hdm_batch_run(hdm_batch *b);                    /* enqueue one run (async) */
+int hdm_batch_sync(hdm_batch *b, hdm_result *out);  /* wait; out[n] results    */
+/* Device time of the last run, from HIP events around it (ms). */
+float hdm_batch_last_ms(hdm_batch *b);
+void hdm_batch_destroy(hdm_batch *b);
+
+/* One-shot convenience: create (no graph), run, sync, destroy. */
+int hdm_hausdorff_batch(const hdm_pair *pairs, int n, const hdm_opts *opts, hdm_result *out);
+
+/* ---- autotuner ---------------------------------------------------------
+ * No counterpart in the reference (it takes launch shapes from the CUDA
+ * occupancy API, hello.cu:1909-1954).  Times the batched launch for each
+ * (tile, chunk, grid) candidate on the given pairs; best first. */
+typedef struct hdm_tune_entry {
+  int32_t tile, chunk, grid_ctas, status;
+  float ms; /* median device ms of the batched run */
+} hdm_tune_entry;
+
+int hdm_autotune(const hdm_pair *pairs, int n, const int32_t *tiles, int n_tiles,
+                 const int32_t *chunks, int n_chunks, const int32_t *grids, int n_grids,
+                 int reps, hdm_tune_entry *out, int out_cap, int *n_out);
+
+/* ---- serving engine ----------------------------------------------------
+ * No counterpart in the reference.  Continuous batching: requests are
+ * admitted against an HBM budget, join the running persistent engine at an
+ * iteration boundary (each pair keeps its own iteration count), and retire
+ * as soon as both of their masks are covered. */
+typedef struct hdm_server hdm_server;
+
+typedef struct hdm_server_opts {
+  uint64_t hbm_budget;   /* bytes of device memory for inputs + workspace   */
+  int32_t max_slots;     /* requests in flight at once                      */
+  int32_t slice_iters;   /* dilation iterations per persistent launch       */
+  int32_t tile;          /* HDM_TILE_*                                      */
+  int32_t grid_ctas;     /* 0 = occupancy-derived                           */
+  double robustness;
+} hdm_server_opts;
+
+typedef struct hdm_completion {
+  uint64_t id;       /* request id from hdm_server_submit        */
+  int32_t status;    /* HDM_OK or an error code                  */
+  int32_t reserved;
+  hdm_result result;
+  double t_submit, t_admit, t_done; /* seconds, library clock     */
+} hdm_completion;
+
+void hdm_server_opts_init(hdm_server_opts *o);
+int hdm_server_create(const hdm_server_opts *o, hdm_server **out);
+/* Queue a request.  gold/segm are host pointers; the server copies them to
+ * device memory it owns once the request is admitted. */
+int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx, int ny,
+                      int nz, int dtype, int label, uint64_t *id_out);
+/* Drive the engine: admit, launch one slice, retire.  Fills up to cap
+ * completions.  Returns HDM_OK. */
+int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done);
+/* Requests queued, in flight, and device bytes held. */
+int hdm_server_load(hdm_server *s, int *queued, int *in_flight, uint64_t *bytes_used);
+double hdm_server_now(void);
+void hdm_server_destroy(hdm_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDM_H */
