@@ -1,0 +1,373 @@
+"""MI355X Hausdorff-distance engine: Python host mirror of the C ABI (libhdm.so).
+
+Volumes are 3-D arrays indexed [z, y, x] (x fastest, the reference's layout
+x + y*Nx + z*Nx*Ny, hello.cu:453).  torch tensors on the GPU are passed as
+device pointers on torch's current stream; numpy arrays are copied by the
+library.  The compute always runs in the HIP library; importing this package
+without libhdm.so built raises.
+
+Reference-shaped entry point (hello.cu:2025-2027):
+    getHausdorffDistance_CUDA_Generic(gold, segm, WIDTH, HEIGHT, DEPTH,
+        robustnessPercent, resIterNeeded, numberToLookFor, res3DNeeded) -> int
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhdm.so")
+
+HDM_OK = 0
+ERRORS = {-1: "HDM_E_INVALID", -2: "HDM_E_EMPTY_MASK", -3: "HDM_E_HIP", -4: "HDM_E_NOMEM",
+          -5: "HDM_E_TIMEOUT", -6: "HDM_E_BUDGET", -7: "HDM_E_NODEVICE", -8: "HDM_E_STUCK",
+          -9: "HDM_E_CAPACITY"}
+PTR_DEVICE, PTR_HOST = 0, 1
+DTYPE_U8, DTYPE_I32 = 0, 1
+TILE_AUTO, TILE_64x8x8, TILE_64x4x4, TILE_64x8x4, TILE_32x8x8 = 0, 1, 2, 3, 4
+TILE_NAMES = {TILE_64x8x8: "64x8x8", TILE_64x4x4: "64x4x4", TILE_64x8x4: "64x8x4",
+              TILE_32x8x8: "32x8x8"}
+
+
+class HDMError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)} ({code}) {what}".strip())
+
+
+class Opts(C.Structure):
+    _fields_ = [("robustness", C.c_double), ("ptr_kind", C.c_int32), ("tile", C.c_int32),
+                ("grid_ctas", C.c_int32), ("chunk", C.c_int32), ("crop", C.c_int32),
+                ("reserved0", C.c_int32), ("stream", C.c_void_p), ("map_out", C.c_void_p),
+                ("hist_out", C.c_void_p), ("hist_len", C.c_int32), ("reserved1", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("hd", C.c_int32), ("hd_fp", C.c_int32), ("hd_fn", C.c_int32),
+                ("iterations", C.c_int32), ("sum", C.c_uint64), ("count", C.c_uint64),
+                ("sum_fp", C.c_uint64), ("cnt_fp", C.c_uint64), ("sum_fn", C.c_uint64),
+                ("cnt_fn", C.c_uint64), ("n_gold", C.c_uint64), ("n_segm", C.c_uint64),
+                ("n_fp", C.c_uint64), ("n_fn", C.c_uint64), ("avg_hd", C.c_double),
+                ("queue_total", C.c_uint64), ("queue_max", C.c_uint32), ("tiles", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Pair(C.Structure):
+    _fields_ = [("gold", C.c_void_p), ("segm", C.c_void_p), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("nz", C.c_int32), ("dtype", C.c_int32), ("label", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("chunk", C.c_int32), ("grid_ctas", C.c_int32),
+                ("status", C.c_int32), ("ms", C.c_float)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_slots", C.c_int32), ("slice_iters", C.c_int32),
+                ("tile", C.c_int32), ("grid_ctas", C.c_int32), ("robustness", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("reserved", C.c_int32),
+                ("result", Result), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class Ell(C.Structure):
+    _fields_ = [("c", C.c_int32 * 3), ("s", C.c_int32 * 3), ("r", C.c_int32 * 9)]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_gold", C.c_int32),
+                ("n_segm", C.c_int32), ("flip_mode", C.c_int32), ("flip_seed", C.c_uint64),
+                ("flip_thresh", C.c_uint64), ("cube_lo", C.c_int32 * 3),
+                ("cube_hi", C.c_int32 * 3), ("gold", Ell * 16), ("segm", Ell * 16)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libhdm.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with "
+                          "`python -m housdorffhdf5cmake_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u64 = C.c_void_p, C.c_int, C.c_uint64
+    sig = {
+        "hdm_abi_version": ([], i32), "hdm_strerror": ([i32], C.c_char_p),
+        "hdm_opts_init": ([C.POINTER(Opts)], None), "hdm_device_count": ([], i32),
+        "hdm_workspace_bytes": ([i32, i32, i32, i32], u64),
+        "hdm_hausdorff_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts),
+                              C.POINTER(Result)], i32),
+        "hdm_hausdorff_i32": ([vp, vp, i32, i32, i32, C.c_int32, C.POINTER(Opts),
+                               C.POINTER(Result)], i32),
+        "hdm_hausdorff_bool": ([vp, vp, i32, i32, i32, C.POINTER(Opts), C.POINTER(Result)], i32),
+        "hdm_first_hit_map_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts), vp,
+                                  C.POINTER(Result)], i32),
+        "hdm_batch_create": ([C.POINTER(Pair), i32, C.POINTER(Opts), i32, C.POINTER(vp)], i32),
+        "hdm_batch_run": ([vp], i32), "hdm_batch_sync": ([vp, C.POINTER(Result)], i32),
+        "hdm_batch_last_ms": ([vp], C.c_float), "hdm_batch_destroy": ([vp], None),
+        "hdm_hausdorff_batch": ([C.POINTER(Pair), i32, C.POINTER(Opts), C.POINTER(Result)], i32),
+        "hdm_autotune": ([C.POINTER(Pair), i32, vp, i32, vp, i32, vp, i32, i32,
+                          C.POINTER(TuneEntry), i32, C.POINTER(C.c_int)], i32),
+        "hdm_server_opts_init": ([C.POINTER(ServerOpts)], None),
+        "hdm_server_create": ([C.POINTER(ServerOpts), C.POINTER(vp)], i32),
+        "hdm_server_submit": ([vp, vp, vp, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)], i32),
+        "hdm_server_step": ([vp, C.POINTER(Completion), i32, C.POINTER(C.c_int)], i32),
+        "hdm_server_load": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_uint64)], i32),
+        "hdm_server_now": ([], C.c_double), "hdm_server_destroy": ([vp], None),
+        "hdm_synth_spec_sphere": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32, i32,
+                                   i32, i32, C.c_double, u64], i32),
+        "hdm_synth_spec_ellipsoids": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32,
+                                       C.c_double, u64], i32),
+        "hdm_synth_spec_long": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32, i32,
+                                 i32, i32, i32], i32),
+        "hdm_synth_host": ([C.POINTER(SynthSpec), vp, vp], i32),
+        "hdm_synth_device": ([C.POINTER(SynthSpec), vp, vp, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != HDM_OK:
+        raise HDMError(rc, what)
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _current_stream_ptr():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _volume_args(gold, segm):
+    """(gold_ptr, segm_ptr, nx, ny, nz, dtype, ptr_kind, keepalive)."""
+    if gold.shape != segm.shape or len(gold.shape) != 3:
+        raise ValueError(f"need two equal 3-D volumes, got {tuple(gold.shape)} {tuple(segm.shape)}")
+    nz, ny, nx = (int(v) for v in gold.shape)
+    if _is_torch(gold):
+        import torch
+        if not (gold.is_cuda and segm.is_cuda):
+            gold, segm = gold.numpy(), segm.numpy()
+        else:
+            if gold.dtype != segm.dtype:
+                raise ValueError("dtype mismatch")
+            if gold.dtype in (torch.uint8, torch.bool):
+                dt = DTYPE_U8
+                if gold.dtype == torch.bool:
+                    gold, segm = gold.view(torch.uint8), segm.view(torch.uint8)
+            elif gold.dtype == torch.int32:
+                dt = DTYPE_I32
+            else:
+                raise TypeError(f"unsupported dtype {gold.dtype}")
+            gold, segm = gold.contiguous(), segm.contiguous()
+            return (C.c_void_p(gold.data_ptr()), C.c_void_p(segm.data_ptr()), nx, ny, nz, dt,
+                    PTR_DEVICE, (gold, segm))
+    g = np.ascontiguousarray(gold)
+    s = np.ascontiguousarray(segm)
+    if g.dtype == np.bool_:
+        g, s = g.view(np.uint8), s.view(np.uint8)
+    if g.dtype != s.dtype:
+        raise ValueError("dtype mismatch")
+    if g.dtype == np.uint8:
+        dt = DTYPE_U8
+    elif g.dtype == np.int32:
+        dt = DTYPE_I32
+    else:
+        raise TypeError(f"unsupported dtype {g.dtype}")
+    return (C.c_void_p(g.ctypes.data), C.c_void_p(s.ctypes.data), nx, ny, nz, dt, PTR_HOST,
+            (g, s))
+
+
+def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, stream=None,
+              ptr_kind=PTR_DEVICE) -> Opts:
+    o = Opts()
+    lib().hdm_opts_init(C.byref(o))
+    o.robustness = float(robustness)
+    o.tile = int(tile)
+    o.grid_ctas = int(grid_ctas)
+    o.chunk = int(chunk)
+    o.crop = 1 if crop else 0
+    o.ptr_kind = ptr_kind
+    if stream is not None:
+        o.stream = stream
+    return o
+
+
+def hausdorff(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO,
+              grid_ctas: int = 0, chunk: int = 0, crop: bool = True, raise_on_error: bool = True,
+              hist_len: int = 0):
+    """HD and average HD of two volumes.  Returns a dict of hdm_result fields
+    (plus 'status', and 'hist_fp'/'hist_fn' when hist_len > 0)."""
+    gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
+    o = make_opts(robustness, tile, grid_ctas, chunk, crop,
+                  _current_stream_ptr() if pk == PTR_DEVICE else None, pk)
+    hist = None
+    if hist_len > 0:
+        hist = np.zeros(2 * hist_len, np.uint32)
+        o.hist_out = hist.ctypes.data
+        o.hist_len = hist_len
+    res = Result()
+    if dt == DTYPE_U8:
+        rc = lib().hdm_hausdorff_u8(gp, sp, nx, ny, nz, label, C.byref(o), C.byref(res))
+    else:
+        rc = lib().hdm_hausdorff_i32(gp, sp, nx, ny, nz, label, C.byref(o), C.byref(res))
+    del keep
+    if raise_on_error:
+        check(rc, "hdm_hausdorff")
+    d = res.as_dict()
+    d["status"] = rc
+    if hist is not None:
+        d["hist_fp"] = hist[:hist_len].copy()
+        d["hist_fn"] = hist[hist_len:].copy()
+    return d
+
+
+def first_hit_map(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO):
+    """(result dict, uint16 map [z,y,x]) with d(v) on FP and FN voxels."""
+    gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
+    if dt != DTYPE_U8:
+        raise TypeError("first_hit_map takes uint8 volumes")
+    if pk == PTR_DEVICE:
+        import torch
+        mp = torch.empty((nz, ny, nx), dtype=torch.int16, device=gold.device)
+        mptr = C.c_void_p(mp.data_ptr())
+        o = make_opts(robustness, tile, stream=_current_stream_ptr(), ptr_kind=pk)
+    else:
+        mp = np.zeros((nz, ny, nx), np.uint16)
+        mptr = C.c_void_p(mp.ctypes.data)
+        o = make_opts(robustness, tile, ptr_kind=pk)
+    res = Result()
+    rc = lib().hdm_first_hit_map_u8(gp, sp, nx, ny, nz, label, C.byref(o), mptr, C.byref(res))
+    del keep
+    check(rc, "hdm_first_hit_map_u8")
+    return res.as_dict(), mp
+
+
+def getHausdorffDistance_CUDA_Generic(goldStandard, algoOutput, WIDTH: int, HEIGHT: int,
+                                      DEPTH: int, robustnessPercent: float = 1.0,
+                                      resIterNeeded: bool = False, numberToLookFor: int = 1,
+                                      res3DNeeded: bool = False) -> int:
+    """The reference's entry shape (hello.cu:2025-2062): returns the HD.
+    Raises HDMError where the reference would hang (empty mask) or return
+    garbage; identical masks give 0 (the reference returns 1)."""
+    for v in (goldStandard, algoOutput):
+        if tuple(int(s) for s in v.shape) != (DEPTH, HEIGHT, WIDTH):
+            raise ValueError("volume shape must be (DEPTH, HEIGHT, WIDTH)")
+    return int(hausdorff(goldStandard, algoOutput, label=numberToLookFor,
+                         robustness=robustnessPercent)["hd"])
+
+
+# ---------------------------------------------------------------- synthetic
+def synth_spec(kind: str, *args) -> SynthSpec:
+    sp = SynthSpec()
+    f = {"sphere": lib().hdm_synth_spec_sphere, "ellipsoids": lib().hdm_synth_spec_ellipsoids,
+         "long": lib().hdm_synth_spec_long}[kind]
+    check(f(C.byref(sp), *args), f"synth_spec {kind}")
+    return sp
+
+
+def synth_host(sp: SynthSpec):
+    g = np.empty((sp.nz, sp.ny, sp.nx), np.uint8)
+    s = np.empty_like(g)
+    check(lib().hdm_synth_host(C.byref(sp), C.c_void_p(g.ctypes.data), C.c_void_p(s.ctypes.data)))
+    return g, s
+
+
+def synth_device(sp: SynthSpec, device="cuda"):
+    import torch
+    g = torch.empty((sp.nz, sp.ny, sp.nx), dtype=torch.uint8, device=device)
+    s = torch.empty_like(g)
+    check(lib().hdm_synth_device(C.byref(sp), C.c_void_p(g.data_ptr()), C.c_void_p(s.data_ptr()),
+                                 _current_stream_ptr()))
+    return g, s
+
+
+# ---------------------------------------------------------------- batch
+def _pairs(vols, label=1):
+    arr = (Pair * len(vols))()
+    keep = []
+    for i, (g, s) in enumerate(vols):
+        gp, sp, nx, ny, nz, dt, pk, k = _volume_args(g, s)
+        if pk != PTR_DEVICE:
+            raise ValueError("batched pairs must be device tensors")
+        arr[i].gold, arr[i].segm = gp, sp
+        arr[i].nx, arr[i].ny, arr[i].nz = nx, ny, nz
+        arr[i].dtype, arr[i].label = dt, label
+        keep.append(k)
+    return arr, keep
+
+
+class Batch:
+    """Many pairs through one persistent launch; with graph=True the launch
+    sequence is captured once and replayed."""
+
+    def __init__(self, vols, label=1, robustness=1.0, tile=TILE_AUTO, chunk=0, grid_ctas=0,
+                 graph=True, crop=True):
+        self.pairs, self._keep = _pairs(vols, label)
+        self.n = len(vols)
+        o = make_opts(robustness, tile, grid_ctas, chunk, crop)
+        self.h = C.c_void_p()
+        check(lib().hdm_batch_create(self.pairs, self.n, C.byref(o), 1 if graph else 0,
+                                     C.byref(self.h)), "hdm_batch_create")
+
+    def run(self):
+        check(lib().hdm_batch_run(self.h), "hdm_batch_run")
+
+    def sync(self, results=True):
+        if not results:
+            check(lib().hdm_batch_sync(self.h, None), "hdm_batch_sync")
+            return None
+        out = (Result * self.n)()
+        rc = lib().hdm_batch_sync(self.h, out)
+        res = [r.as_dict() for r in out]
+        check(rc, "hdm_batch_sync")
+        return res
+
+    def last_ms(self) -> float:
+        return float(lib().hdm_batch_last_ms(self.h))
+
+    def close(self):
+        if self.h:
+            lib().hdm_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(vols, tiles=(TILE_64x8x8, TILE_64x4x4, TILE_64x8x4, TILE_32x8x8), chunks=(1, 2, 4),
+             grids=(0,), reps=3, label=1):
+    """Times the batched launch for each (tile, chunk, grid); fastest first."""
+    pairs, keep = _pairs(vols, label)
+    t = (C.c_int32 * len(tiles))(*tiles)
+    c = (C.c_int32 * len(chunks))(*chunks)
+    g = (C.c_int32 * len(grids))(*grids)
+    cap = len(tiles) * len(chunks) * len(grids)
+    out = (TuneEntry * cap)()
+    n = C.c_int()
+    check(lib().hdm_autotune(pairs, len(vols), t, len(tiles), c, len(chunks), g, len(grids), reps,
+                             out, cap, C.byref(n)), "hdm_autotune")
+    del keep
+    return [{"tile": TILE_NAMES.get(e.tile, e.tile), "tile_id": e.tile, "chunk": e.chunk,
+             "grid_ctas": e.grid_ctas, "status": e.status, "ms": e.ms} for e in out[:n.value]]

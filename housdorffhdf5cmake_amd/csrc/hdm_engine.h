@@ -1,0 +1,93 @@
+// Device-side data structures shared by the kernels and the host driver.
+//
+// Layout in HBM, per pair (tile = TXB(x) * TY(y) * TZ(z) voxels, R = TY*TZ rows
+// of TXB bits; one wavefront lane owns one row):
+//   orig[m]          packed mask m (0 gold, 1 segm), tile-major: word
+//                    tile*R + (y%TY) + TY*(z%TZ) holds voxels x = 64*tx .. +63
+//   buf[m][s]        two ping-pong slots for the dilated mask m
+//   xf_*[m]          per tile two R-bit words: bit r = bit 0 / bit TXB-1 of row r
+//                    (the tile's -x / +x faces), so a neighbour reads one word
+//                    instead of the whole tile
+//   state[m][tile]   0 idle; bit 31 full; else 1 + the pair-local iteration at
+//                    which the tile was first queued
+//   hist[m][t]       number of voxels of the other mask first covered at t
+// Where a neighbour's state at iteration t lives: full -> all ones; queued
+// before t -> buf[m][(t-1)&1]; otherwise (idle, or first queued at t) -> orig.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hdm {
+
+constexpr int kWave = 64;
+constexpr int kWgThreads = 256;
+constexpr int kWavesPerWg = kWgThreads / kWave;
+constexpr int kLocalQueue = 2048;  // entries staged in LDS per workgroup
+
+constexpr uint32_t kFull = 0x80000000u;
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
+// queue entry: tile (32) | pair (16) | mask (1) | first (1) | check (1)
+__host__ __device__ inline uint64_t make_entry(uint32_t tile, uint32_t pair, uint32_t m,
+                                               uint32_t first, uint32_t check) {
+  return (uint64_t)tile | ((uint64_t)pair << 32) | ((uint64_t)m << 48) |
+         ((uint64_t)first << 49) | ((uint64_t)check << 50);
+}
+
+struct PairDesc {
+  const void *gold;
+  const void *segm;
+  int32_t nx, ny, nz, wxn;  // wxn: TXB-bit words per x-row
+  int32_t gx, gy, gz;       // tile grid
+  uint32_t ntiles;
+  int32_t dtype, label;
+  int32_t hist_len;
+  uint32_t t_start;  // engine iteration at which the pair entered
+  int32_t crop;
+  int32_t tile_kind;
+  double robust;
+  uint64_t words;  // ntiles * R, words per packed mask
+  void *orig[2];
+  void *buf[2][2];
+  uint64_t *xf_orig[2];
+  uint64_t *xf_buf[2][2];
+  uint32_t *state[2];
+  uint32_t *hist[2];
+  uint16_t *map;
+};
+
+struct PairCtr {
+  unsigned long long n_gold, n_segm, n_fp, n_fn;
+  unsigned long long cov[2];
+  int32_t lo[3], hi[3];  // tile bounding box of non-empty tiles (inclusive)
+  int32_t status;
+  int32_t pad;
+};
+
+struct PairOut {
+  int32_t hd, hd_fp, hd_fn, status;
+  unsigned long long sum_fp, cnt_fp, sum_fn, cnt_fn;
+  unsigned long long n_gold, n_segm, n_fp, n_fn;
+};
+
+struct Engine {
+  uint32_t t;        // completed engine iterations
+  uint32_t qlen[3];  // queue lengths, rotating: read t%3, write (t+1)%3
+  uint32_t bar_count, bar_gen;
+  uint32_t error;
+  uint32_t qmax;
+  unsigned long long qtotal;
+  uint32_t qcap;
+  uint32_t pad;
+};
+
+// ceil(r * n) clamped to [0, n]; the robust cut (old/testMainPassesKernels.cu:81-82)
+__host__ __device__ inline unsigned long long robust_cut(double r, unsigned long long n) {
+  if (n == 0) return 0;
+  double c = ceil(r * (double)n);
+  if (c < 0.0) c = 0.0;
+  if (c > (double)n) c = (double)n;
+  return (unsigned long long)c;
+}
+
+}  // namespace hdm

@@ -1,0 +1,57 @@
+// Internal host-side helpers shared by the single-pair, batch and serving drivers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+
+#include "../../include/hdm.h"
+#include "hdm_plan.h"
+
+namespace hdm {
+
+// grow-only device buffer
+struct DevMem {
+  void *p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n);
+  void release();
+};
+
+// per-device state of the single-pair entry points
+struct Ctx {
+  int dev = -1;
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  int occ[HDM_TILE_COUNT] = {};  // resident dilate workgroups per CU, per tile kind
+  double clock_khz = 0;          // wall_clock64() rate
+  DevMem ws, in, map;
+  void *host_out = nullptr;  // pinned, 4 KiB
+  std::mutex mu;
+};
+
+struct GroupPtrs {
+  PairDesc *descs = nullptr;
+  PackIndex *idx = nullptr;
+  PairCtr *ctrs = nullptr;
+  PairOut *outs = nullptr;
+  Engine *engine = nullptr;
+  uint64_t *q0 = nullptr, *q1 = nullptr;
+  uint32_t *qstats = nullptr;
+};
+
+int map_hip(hipError_t e);
+int get_ctx(Ctx **out);
+int resident_grid(const Ctx *c, int tile, int requested);
+uint64_t spin_ticks(const Ctx *c);
+int norm_tile(int tile);
+GroupPtrs carve_common(char *base, const CommonLayout &L);
+hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
+                         const PairDesc *single, const PackIndex *single_idx,
+                         uint64_t total_rw, uint64_t total_tiles, int rows, uint32_t qcap,
+                         int grid, int chunk, uint64_t spin, int qstats_len);
+void fill_result(hdm_result *out, const PairOut &o, const Engine *es, uint32_t tiles,
+                 int iterations);
+
+}  // namespace hdm

@@ -1,0 +1,563 @@
+// Host driver of the Hausdorff dilation engine: the C ABI of include/hdm.h.
+//
+// One call of the single-pair entry points enqueues, on one stream:
+//   reset -> clear histograms -> pack -> tile metadata -> persistent dilation
+//   -> finalize, then one small device-to-host copy of the result.
+// There are no host round trips between the kernels (the reference reads its
+// counters back twice mid-pipeline to size allocations, hello.cu:952, 1012):
+// the workspace is sized up front from the dims, grows only, and is reused.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hdm.h"
+#include "hdm_host.h"
+
+using namespace hdm;
+
+namespace hdm {
+
+int map_hip(hipError_t e) {
+  if (e == hipSuccess) return HDM_OK;
+  if (e == hipErrorOutOfMemory) return HDM_E_NOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return HDM_E_NODEVICE;
+  return HDM_E_HIP;
+}
+
+int DevMem::ensure(size_t n) {
+  if (n <= cap) return HDM_OK;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t want = std::max(n, cap * 5 / 4);
+  if (hipMalloc(&p, want) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return HDM_E_NOMEM;
+  }
+  cap = want;
+  return HDM_OK;
+}
+
+void DevMem::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+static std::mutex g_ctx_mu;
+static std::map<int, Ctx *> g_ctx;
+
+int get_ctx(Ctx **out) {
+  int dev = -1;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return HDM_E_NODEVICE;
+  }
+  if (hipGetDevice(&dev) != hipSuccess) return HDM_E_NODEVICE;
+  std::lock_guard<std::mutex> lk(g_ctx_mu);
+  auto it = g_ctx.find(dev);
+  if (it != g_ctx.end()) {
+    *out = it->second;
+    return HDM_OK;
+  }
+  Ctx *c = new Ctx();
+  c->dev = dev;
+  int rc = HDM_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = HDM_E_HIP;
+  if (rc == HDM_OK &&
+      hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    rc = HDM_E_HIP;
+  int khz = 0;
+  if (rc == HDM_OK &&
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
+    rc = HDM_E_HIP;
+  c->clock_khz = khz > 0 ? khz : 100000;
+  for (int k = 1; rc == HDM_OK && k < HDM_TILE_COUNT; ++k) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->dilate_fn,
+                                                     kWgThreads, 0) != hipSuccess ||
+        nb < 1)
+      rc = HDM_E_HIP;
+    c->occ[k] = nb;
+  }
+  if (rc == HDM_OK && hipHostMalloc(&c->host_out, 4096, hipHostMallocDefault) != hipSuccess)
+    rc = HDM_E_HIP;
+  if (rc != HDM_OK) {
+    (void)hipGetLastError();
+    delete c;
+    return rc;
+  }
+  g_ctx[dev] = c;
+  *out = c;
+  return HDM_OK;
+}
+
+int resident_grid(const Ctx *c, int tile, int requested) {
+  const int maxres = c->occ[tile] * c->cus;
+  if (requested > 0) return std::min(requested, maxres);
+  return std::min(maxres, 2 * c->cus);
+}
+
+uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.0); }
+
+int norm_tile(int tile) { return (tile <= 0 || tile >= HDM_TILE_COUNT) ? HDM_TILE_64x8x8 : tile; }
+
+hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
+                         const PairDesc *single, const PackIndex *single_idx,
+                         uint64_t total_rw, uint64_t total_tiles, int rows, uint32_t qcap,
+                         int grid, int chunk, uint64_t spin, int qstats_len) {
+  hipError_t e = hdm_launch_reset(s, P.descs, single, single_idx, P.idx, P.ctrs, 0, npairs,
+                                  P.engine, 1, qcap);
+  if (e != hipSuccess) return e;
+  const uint64_t pb = (total_rw + kWgThreads - 1) / kWgThreads;
+  e = tl->pack(dim3((unsigned)pb), s, P.descs, P.idx, P.ctrs, npairs, total_rw);
+  if (e != hipSuccess) return e;
+  const uint64_t mb = (total_tiles * rows + kWgThreads - 1) / kWgThreads;
+  e = tl->meta(dim3((unsigned)mb), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q0, P.q1,
+               total_tiles, 0);
+  if (e != hipSuccess) return e;
+  e = tl->dilate(dim3(grid), s, P.descs, P.ctrs, P.engine, P.q0, P.q1, INT_MAX,
+                 std::max(1, chunk), spin, P.qstats, qstats_len);
+  if (e != hipSuccess) return e;
+  return hdm_launch_finalize(s, P.descs, P.ctrs, P.engine, P.outs, 0, npairs);
+}
+
+GroupPtrs carve_common(char *base, const CommonLayout &L) {
+  GroupPtrs P;
+  P.descs = (PairDesc *)(base + L.descs);
+  P.idx = (PackIndex *)(base + L.idx);
+  P.ctrs = (PairCtr *)(base + L.ctrs);
+  P.outs = (PairOut *)(base + L.outs);
+  P.engine = (Engine *)(base + L.engine);
+  P.q0 = (uint64_t *)(base + L.q0);
+  P.q1 = (uint64_t *)(base + L.q1);
+  P.qstats = (uint32_t *)(base + L.qstats);
+  return P;
+}
+
+void fill_result(hdm_result *out, const PairOut &o, const Engine *es, uint32_t tiles,
+                 int iterations) {
+  std::memset(out, 0, sizeof(*out));
+  out->hd = o.hd, out->hd_fp = o.hd_fp, out->hd_fn = o.hd_fn;
+  out->iterations = iterations;
+  out->sum_fp = o.sum_fp, out->cnt_fp = o.cnt_fp, out->sum_fn = o.sum_fn, out->cnt_fn = o.cnt_fn;
+  out->sum = o.sum_fp + o.sum_fn;
+  out->count = o.cnt_fp + o.cnt_fn;
+  out->n_gold = o.n_gold, out->n_segm = o.n_segm, out->n_fp = o.n_fp, out->n_fn = o.n_fn;
+  // the one floating-point step, on the host (hello.cu never forms the average)
+  out->avg_hd = out->count ? (double)out->sum / (double)out->count : 0.0;
+  if (es) {
+    out->queue_total = es->qtotal;
+    out->queue_max = es->qmax;
+  }
+  out->tiles = tiles;
+}
+
+}  // namespace hdm
+
+// ---------------------------------------------------------------- misc ABI
+extern "C" int hdm_abi_version(void) { return HDM_ABI_VERSION; }
+
+extern "C" const char *hdm_strerror(int code) {
+  switch (code) {
+    case HDM_OK: return "ok";
+    case HDM_E_INVALID: return "invalid argument";
+    case HDM_E_EMPTY_MASK: return "one mask is empty while the other has voxels";
+    case HDM_E_HIP: return "HIP runtime error";
+    case HDM_E_NOMEM: return "device memory allocation failed";
+    case HDM_E_TIMEOUT: return "device grid barrier timed out";
+    case HDM_E_BUDGET: return "request exceeds the HBM budget";
+    case HDM_E_NODEVICE: return "no HIP device";
+    case HDM_E_STUCK: return "work queue drained with targets uncovered";
+    case HDM_E_CAPACITY: return "capacity exceeded";
+    default: return "unknown error";
+  }
+}
+
+extern "C" void hdm_opts_init(hdm_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->robustness = 1.0;
+  o->ptr_kind = HDM_PTR_DEVICE;
+  o->tile = HDM_TILE_AUTO;
+  o->crop = 1;
+}
+
+extern "C" int hdm_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+extern "C" uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile) {
+  if (nx < 1 || ny < 1 || nz < 1) return 0;
+  const TileShape ts = tile_shape(norm_tile(tile));
+  const PairGeom g = pair_geom(nx, ny, nz, ts);
+  const CommonLayout L = common_layout(1, 2ull * g.ntiles, g.hist_len);
+  return L.total + g.total;
+}
+
+// ---------------------------------------------------------------- single pair
+static int hausdorff_core(const void *gold, const void *segm, int dtype, int label, int nx,
+                          int ny, int nz, const hdm_opts *opts_in, uint16_t *map_out,
+                          hdm_result *out) {
+  hdm_opts opts;
+  hdm_opts_init(&opts);
+  if (opts_in) opts = *opts_in;
+  if (!gold || !segm || !out || nx < 1 || ny < 1 || nz < 1 || nx > 65535 || ny > 65535 ||
+      nz > 65535)
+    return HDM_E_INVALID;
+  if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0)) return HDM_E_INVALID;
+  if (opts.ptr_kind != HDM_PTR_DEVICE && opts.ptr_kind != HDM_PTR_HOST) return HDM_E_INVALID;
+  if (opts.tile < 0 || opts.tile >= HDM_TILE_COUNT) return HDM_E_INVALID;
+  if (!map_out && opts.map_out) map_out = opts.map_out;
+  Ctx *c = nullptr;
+  int rc = get_ctx(&c);
+  if (rc != HDM_OK) return rc;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipStream_t s = opts.stream ? (hipStream_t)opts.stream : c->stream;
+  const int tile = norm_tile(opts.tile);
+  const TileLaunch *tl = hdm_tile_launch(tile);
+  const TileShape ts = tile_shape(tile);
+  const PairGeom g = pair_geom(nx, ny, nz, ts);
+  if ((uint64_t)g.ntiles * 2 > 0xFFFFFFFFull) return HDM_E_INVALID;
+  const uint32_t qcap = 2u * g.ntiles;
+  const CommonLayout L = common_layout(1, qcap, g.hist_len);
+  const uint64_t nvox = (uint64_t)nx * ny * nz;
+  const size_t esz = dtype == HDM_DTYPE_U8 ? 1 : 4;
+  if ((rc = c->ws.ensure(L.total + g.total)) != HDM_OK) return rc;
+  char *base = (char *)c->ws.p;
+  GroupPtrs P = carve_common(base, L);
+  PairDesc d{};
+  const void *gd = gold, *sd = segm;
+  if (opts.ptr_kind == HDM_PTR_HOST) {
+    if ((rc = c->in.ensure(2 * align256(nvox * esz))) != HDM_OK) return rc;
+    char *ib = (char *)c->in.p;
+    if (hipMemcpyAsync(ib, gold, nvox * esz, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ib + align256(nvox * esz), segm, nvox * esz, hipMemcpyHostToDevice, s) !=
+            hipSuccess)
+      return HDM_E_HIP;
+    gd = ib, sd = ib + align256(nvox * esz);
+  }
+  fill_desc(d, g, gd, sd, dtype, label, opts.robustness, opts.crop ? 1 : 0, tile, 0);
+  carve_pair(d, g, base + L.total);
+  if (map_out) {
+    if (opts.ptr_kind == HDM_PTR_DEVICE) {
+      d.map = map_out;
+    } else {
+      if ((rc = c->map.ensure(nvox * 2)) != HDM_OK) return rc;
+      d.map = (uint16_t *)c->map.p;
+    }
+  }
+  const PackIndex pi{0, 0};
+  const int grid = resident_grid(c, tile, opts.grid_ctas);
+  hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, ts.rows(), qcap,
+                               grid, opts.chunk > 0 ? opts.chunk : 1, spin_ticks(c), g.hist_len);
+  if (e != hipSuccess) return map_hip(e);
+  char *ho = (char *)c->host_out;
+  if (hipMemcpyAsync(ho, P.outs, sizeof(PairOut), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(ho + 256, P.engine, sizeof(Engine), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return HDM_E_HIP;
+  if (map_out && opts.ptr_kind == HDM_PTR_HOST &&
+      hipMemcpyAsync(map_out, d.map, nvox * 2, hipMemcpyDeviceToHost, s) != hipSuccess)
+    return HDM_E_HIP;
+  if (opts.hist_out && opts.hist_len > 0) {
+    const int n = std::min(opts.hist_len, g.hist_len);
+    if (hipMemcpyAsync(opts.hist_out, d.hist[0], n * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(opts.hist_out + opts.hist_len, d.hist[1], n * 4, hipMemcpyDeviceToHost,
+                       s) != hipSuccess)
+      return HDM_E_HIP;
+  }
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip(e);
+  PairOut o;
+  Engine es;
+  std::memcpy(&o, ho, sizeof(o));
+  std::memcpy(&es, ho + 256, sizeof(es));
+  fill_result(out, o, &es, g.ntiles, (int)es.t);
+  return o.status;
+}
+
+extern "C" int hdm_hausdorff_u8(const uint8_t *gold, const uint8_t *segm, int nx, int ny, int nz,
+                                uint8_t label, const hdm_opts *opts, hdm_result *out) {
+  return hausdorff_core(gold, segm, HDM_DTYPE_U8, label, nx, ny, nz, opts, nullptr, out);
+}
+
+extern "C" int hdm_hausdorff_i32(const int32_t *gold, const int32_t *segm, int nx, int ny,
+                                 int nz, int32_t label, const hdm_opts *opts, hdm_result *out) {
+  return hausdorff_core(gold, segm, HDM_DTYPE_I32, label, nx, ny, nz, opts, nullptr, out);
+}
+
+extern "C" int hdm_hausdorff_bool(const uint8_t *img1, const uint8_t *img2, int nx, int ny,
+                                  int nz, const hdm_opts *opts, hdm_result *out) {
+  // bool volumes hold 0 or 1 (forBench/Volume.h:35-38 allocates them with calloc)
+  return hausdorff_core(img1, img2, HDM_DTYPE_U8, 1, nx, ny, nz, opts, nullptr, out);
+}
+
+extern "C" int hdm_first_hit_map_u8(const uint8_t *gold, const uint8_t *segm, int nx, int ny,
+                                    int nz, uint8_t label, const hdm_opts *opts, uint16_t *map,
+                                    hdm_result *out) {
+  if (!map) return HDM_E_INVALID;
+  return hausdorff_core(gold, segm, HDM_DTYPE_U8, label, nx, ny, nz, opts, map, out);
+}
+
+// ---------------------------------------------------------------- batch
+struct hdm_batch {
+  Ctx *ctx = nullptr;
+  hipStream_t s = nullptr;
+  int n = 0, tile = 0, grid = 0, chunk = 1;
+  const TileLaunch *tl = nullptr;
+  TileShape ts{};
+  uint64_t spin = 0, total_rw = 0, total_tiles = 0;
+  uint32_t qcap = 0;
+  int qstats_len = 0;
+  void *ws = nullptr;
+  GroupPtrs P{};
+  std::vector<PairGeom> geoms;
+  bool graph = false;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void *host = nullptr;
+  float last_ms = 0.f;
+};
+
+static int batch_enqueue(hdm_batch *b) {
+  return map_hip(enqueue_group(b->s, b->tl, b->P, b->n, nullptr, nullptr, b->total_rw,
+                               b->total_tiles, b->ts.rows(), b->qcap, b->grid, b->chunk, b->spin,
+                               b->qstats_len));
+}
+
+extern "C" void hdm_batch_destroy(hdm_batch *b) {
+  if (!b) return;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->g) (void)hipGraphDestroy(b->g);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->ws) (void)hipFree(b->ws);
+  if (b->host) (void)hipHostFree(b->host);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  (void)hipGetLastError();
+  delete b;
+}
+
+extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *opts_in,
+                                int use_graph, hdm_batch **out) {
+  if (!pairs || n < 1 || n > 65535 || !out) return HDM_E_INVALID;
+  *out = nullptr;
+  hdm_opts opts;
+  hdm_opts_init(&opts);
+  if (opts_in) opts = *opts_in;
+  if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0) || opts.tile < 0 ||
+      opts.tile >= HDM_TILE_COUNT || opts.ptr_kind != HDM_PTR_DEVICE)
+    return HDM_E_INVALID;
+  for (int i = 0; i < n; ++i) {
+    const hdm_pair &p = pairs[i];
+    if (!p.gold || !p.segm || p.nx < 1 || p.ny < 1 || p.nz < 1 || p.nx > 65535 ||
+        p.ny > 65535 || p.nz > 65535 || (p.dtype != HDM_DTYPE_U8 && p.dtype != HDM_DTYPE_I32))
+      return HDM_E_INVALID;
+  }
+  Ctx *c = nullptr;
+  int rc = get_ctx(&c);
+  if (rc != HDM_OK) return rc;
+  hdm_batch *b = new hdm_batch();
+  b->ctx = c;
+  b->n = n;
+  b->tile = norm_tile(opts.tile);
+  b->tl = hdm_tile_launch(b->tile);
+  b->ts = tile_shape(b->tile);
+  b->grid = resident_grid(c, b->tile, opts.grid_ctas);
+  b->chunk = opts.chunk > 0 ? opts.chunk : 1;
+  b->spin = spin_ticks(c);
+  std::vector<PairDesc> descs(n);
+  std::vector<PackIndex> idx(n);
+  uint64_t pair_bytes = 0, qcap = 0;
+  int maxh = 0;
+  for (int i = 0; i < n; ++i) {
+    const hdm_pair &p = pairs[i];
+    const PairGeom g = pair_geom(p.nx, p.ny, p.nz, b->ts);
+    b->geoms.push_back(g);
+    idx[i] = PackIndex{b->total_rw, b->total_tiles};
+    b->total_rw += g.row_words;
+    b->total_tiles += g.ntiles;
+    qcap += 2ull * g.ntiles;
+    pair_bytes += g.total;
+    maxh = std::max(maxh, g.hist_len);
+  }
+  if (qcap > 0xFFFFFFFFull || b->total_tiles > 0xFFFFFFFFull) {
+    delete b;
+    return HDM_E_INVALID;
+  }
+  b->qcap = (uint32_t)qcap;
+  b->qstats_len = maxh;
+  const CommonLayout L = common_layout(n, qcap, maxh);
+  if (hipMalloc(&b->ws, L.total + pair_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b->ws = nullptr;
+    hdm_batch_destroy(b);
+    return HDM_E_NOMEM;
+  }
+  b->P = carve_common((char *)b->ws, L);
+  char *pb = (char *)b->ws + L.total;
+  for (int i = 0; i < n; ++i) {
+    const hdm_pair &p = pairs[i];
+    fill_desc(descs[i], b->geoms[i], p.gold, p.segm, p.dtype, p.label, opts.robustness,
+              opts.crop ? 1 : 0, b->tile, 0);
+    carve_pair(descs[i], b->geoms[i], pb);
+    pb += b->geoms[i].total;
+  }
+  if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess ||
+      hipHostMalloc(&b->host, sizeof(PairOut) * n + 256, hipHostMallocDefault) != hipSuccess ||
+      hipMemcpy(b->P.descs, descs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->P.idx, idx.data(), sizeof(PackIndex) * n, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    hdm_batch_destroy(b);
+    return HDM_E_HIP;
+  }
+  if (use_graph) {
+    if (hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      hdm_batch_destroy(b);
+      return HDM_E_HIP;
+    }
+    rc = batch_enqueue(b);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->s, &g);
+    b->g = g;
+    if (rc != HDM_OK || ee != hipSuccess ||
+        hipGraphInstantiate(&b->exec, b->g, nullptr, nullptr, 0) != hipSuccess) {
+      hdm_batch_destroy(b);
+      return rc != HDM_OK ? rc : HDM_E_HIP;
+    }
+    b->graph = true;
+  }
+  *out = b;
+  return HDM_OK;
+}
+
+extern "C" int hdm_batch_run(hdm_batch *b) {
+  if (!b) return HDM_E_INVALID;
+  if (hipEventRecord(b->e0, b->s) != hipSuccess) return HDM_E_HIP;
+  int rc = HDM_OK;
+  if (b->graph)
+    rc = map_hip(hipGraphLaunch(b->exec, b->s));
+  else
+    rc = batch_enqueue(b);
+  if (rc != HDM_OK) return rc;
+  if (hipEventRecord(b->e1, b->s) != hipSuccess) return HDM_E_HIP;
+  return HDM_OK;
+}
+
+extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
+  if (!b) return HDM_E_INVALID;
+  hipError_t e = hipStreamSynchronize(b->s);
+  if (e != hipSuccess) return map_hip(e);
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, b->e0, b->e1) == hipSuccess) b->last_ms = ms;
+  (void)hipGetLastError();
+  if (!out) return HDM_OK;
+  if (hipMemcpy(b->host, b->P.outs, sizeof(PairOut) * b->n, hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      hipMemcpy((char *)b->host + sizeof(PairOut) * b->n, b->P.engine, sizeof(Engine),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return HDM_E_HIP;
+  Engine es;
+  std::memcpy(&es, (char *)b->host + sizeof(PairOut) * b->n, sizeof(es));
+  int rc = HDM_OK;
+  for (int i = 0; i < b->n; ++i) {
+    PairOut o;
+    std::memcpy(&o, (char *)b->host + sizeof(PairOut) * i, sizeof(o));
+    fill_result(&out[i], o, &es, b->geoms[i].ntiles, std::max(o.hd_fp, o.hd_fn));
+    if (o.status != HDM_OK && rc == HDM_OK) rc = o.status;
+  }
+  return rc;
+}
+
+extern "C" float hdm_batch_last_ms(hdm_batch *b) { return b ? b->last_ms : 0.f; }
+
+extern "C" int hdm_hausdorff_batch(const hdm_pair *pairs, int n, const hdm_opts *opts,
+                                   hdm_result *out) {
+  hdm_batch *b = nullptr;
+  int rc = hdm_batch_create(pairs, n, opts, 0, &b);
+  if (rc != HDM_OK) return rc;
+  rc = hdm_batch_run(b);
+  if (rc == HDM_OK) rc = hdm_batch_sync(b, out);
+  hdm_batch_destroy(b);
+  return rc;
+}
+
+// ---------------------------------------------------------------- autotuner
+extern "C" int hdm_autotune(const hdm_pair *pairs, int n, const int32_t *tiles, int n_tiles,
+                            const int32_t *chunks, int n_chunks, const int32_t *grids,
+                            int n_grids, int reps, hdm_tune_entry *out, int out_cap,
+                            int *n_out) {
+  if (!pairs || n < 1 || !out || out_cap < 1 || !n_out || reps < 1) return HDM_E_INVALID;
+  static const int32_t def_tiles[] = {HDM_TILE_64x8x8, HDM_TILE_64x4x4, HDM_TILE_64x8x4,
+                                      HDM_TILE_32x8x8};
+  static const int32_t def_chunks[] = {1, 2, 4};
+  static const int32_t def_grids[] = {0};
+  if (!tiles || n_tiles < 1) tiles = def_tiles, n_tiles = 4;
+  if (!chunks || n_chunks < 1) chunks = def_chunks, n_chunks = 3;
+  if (!grids || n_grids < 1) grids = def_grids, n_grids = 1;
+  std::vector<hdm_result> ref(n), cur(n);
+  bool have_ref = false;
+  int k = 0;
+  for (int ti = 0; ti < n_tiles; ++ti)
+    for (int ci = 0; ci < n_chunks; ++ci)
+      for (int gi = 0; gi < n_grids; ++gi) {
+        if (k >= out_cap) break;
+        hdm_opts o;
+        hdm_opts_init(&o);
+        o.tile = tiles[ti];
+        o.chunk = chunks[ci];
+        o.grid_ctas = grids[gi];
+        hdm_tune_entry &E = out[k++];
+        E.tile = tiles[ti], E.chunk = chunks[ci], E.grid_ctas = grids[gi];
+        E.ms = 0.f;
+        hdm_batch *b = nullptr;
+        int rc = hdm_batch_create(pairs, n, &o, 1, &b);
+        if (rc == HDM_OK) rc = hdm_batch_run(b);
+        if (rc == HDM_OK) rc = hdm_batch_sync(b, cur.data());
+        std::vector<float> ms;
+        for (int r = 0; rc == HDM_OK && r < reps; ++r) {
+          rc = hdm_batch_run(b);
+          if (rc == HDM_OK) rc = hdm_batch_sync(b, nullptr);
+          ms.push_back(hdm_batch_last_ms(b));
+        }
+        if (b) E.grid_ctas = b->grid;
+        hdm_batch_destroy(b);
+        if (rc == HDM_OK) {
+          if (!have_ref) {
+            ref = cur;
+            have_ref = true;
+          } else {
+            for (int i = 0; i < n; ++i)
+              if (cur[i].hd != ref[i].hd || cur[i].sum != ref[i].sum ||
+                  cur[i].count != ref[i].count)
+                rc = HDM_E_STUCK;  // candidates must agree bit for bit
+          }
+          std::sort(ms.begin(), ms.end());
+          E.ms = ms.empty() ? 0.f : ms[ms.size() / 2];
+        }
+        E.status = rc;
+      }
+  std::stable_sort(out, out + k, [](const hdm_tune_entry &a, const hdm_tune_entry &b) {
+    if ((a.status == HDM_OK) != (b.status == HDM_OK)) return a.status == HDM_OK;
+    return a.ms < b.ms;
+  });
+  *n_out = k;
+  return HDM_OK;
+}

@@ -1,0 +1,835 @@
+// HIP kernels of the Hausdorff dilation engine, written for gfx950 (CDNA4,
+// wave64).  Integer bit logic only; the path is bound by memory traffic.
+//
+//   reset_kernel     clears per-pair counters and first-hit histograms
+//   pack_kernel      dense labels -> packed tile-major bits + tile bounding box
+//                    (replaces getMinMaxes and the packing half of
+//                    boolPrepareKernel, hello.cu:396-526, 537-748, with one read)
+//   meta_kernel      per-tile counts, full/empty/active class, face words and
+//                    the initial work queue (boolPrepareKernel metadata and
+//                    firstMetaPrepareKernel, hello.cu:537-748, 762-877)
+//   dilate_kernel    persistent kernel: one grid barrier per iteration, a
+//                    queue of the tiles still changing (mainPassKernel,
+//                    hello.cu:1066-1896)
+//   finalize_kernel  HD, sums and counts per pair from the histograms
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "hdm_plan.h"
+
+namespace hdm {
+
+#define AGENT __HIP_MEMORY_SCOPE_AGENT
+
+template <int TXB>
+using RowT = std::conditional_t<TXB == 64, uint64_t, uint32_t>;
+
+__device__ __forceinline__ uint32_t shfl32(uint32_t v, int src) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, int src) {
+  const uint32_t lo = shfl32((uint32_t)v, src), hi = shfl32((uint32_t)(v >> 32), src);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+template <typename Row>
+__device__ __forceinline__ Row shfl_row(Row v, int src) {
+  if constexpr (sizeof(Row) == 8)
+    return shfl64(v, src);
+  else
+    return shfl32(v, src);
+}
+template <typename Row>
+__device__ __forceinline__ int popc_row(Row v) {
+  if constexpr (sizeof(Row) == 8)
+    return __popcll(v);
+  else
+    return __popc(v);
+}
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// sum over the aligned group of R lanes (R a power of two <= 64)
+template <int R>
+__device__ __forceinline__ uint32_t group_sum(uint32_t v, int lane) {
+#pragma unroll
+  for (int off = R / 2; off >= 1; off >>= 1) v += shfl32(v, lane ^ off);
+  return v;
+}
+
+// bits of the R-lane group holding `lane` out of a wave ballot, shifted to bit 0
+template <int R>
+__device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int lane) {
+  if constexpr (R == 64)
+    return ballot;
+  else
+    return (ballot >> (lane & ~(R - 1))) & ((1ull << R) - 1);
+}
+
+template <typename Row>
+__device__ __forceinline__ Row valid_bits(int tx, int wxn, int nx) {
+  constexpr int TXB = sizeof(Row) * 8;
+  const int rem = nx - tx * TXB;
+  if (rem >= TXB) return ~Row(0);
+  return (Row(1) << rem) - 1;
+}
+
+// pair index of a global position by binary search over ascending bases
+template <typename F>
+__device__ __forceinline__ int find_pair(int npairs, uint64_t g, F base_of) {
+  int lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base_of(mid) <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- reset
+__global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_idx,
+                             PackIndex *idx, int write_single, PairCtr *ctrs, int p0, int npairs,
+                             Engine *es, int reset_engine, uint32_t qcap) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  if (write_single && gid == 0) {
+    descs[p0] = single;
+    idx[p0] = single_idx;
+  }
+  if (reset_engine && gid == 0) {
+    es->t = 0;
+    es->qlen[0] = es->qlen[1] = es->qlen[2] = 0;
+    es->bar_count = 0;
+    es->bar_gen = 0;
+    es->error = 0;
+    es->qmax = 0;
+    es->qtotal = 0;
+    es->qcap = qcap;
+  }
+  for (int p = p0 + gid; p < p0 + npairs; p += stride) {
+    PairCtr &c = ctrs[p];
+    c.n_gold = c.n_segm = c.n_fp = c.n_fn = 0;
+    c.cov[0] = c.cov[1] = 0;
+    c.lo[0] = c.lo[1] = c.lo[2] = 0x7FFFFFFF;
+    c.hi[0] = c.hi[1] = c.hi[2] = -1;
+    c.status = 0;
+  }
+}
+
+// zero the histograms (needs the descriptors, so after reset_kernel)
+__global__ void clear_hist_kernel(const PairDesc *__restrict__ descs, int p0, int npairs) {
+  for (int p = p0 + blockIdx.y; p < p0 + npairs; p += gridDim.y) {
+    const PairDesc &d = descs[p];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.hist_len;
+         i += gridDim.x * blockDim.x) {
+      d.hist[0][i] = 0;
+      d.hist[1][i] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- pack
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// 16 labels (bytes) -> 16 bits: bit k set iff byte k == label.
+__device__ __forceinline__ uint32_t eq_bits_u8x16(u32x4 v, uint32_t lab4) {
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint32_t out = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t x = w[j] ^ lab4;                           // zero byte where equal
+    const uint32_t t = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;  // byte msb = byte != 0
+    const uint32_t eq = (~t & 0x80808080u) >> 7;               // bit 8k = byte k equal
+    out |= ((eq * 0x10204080u) >> 28) << (4 * j);
+  }
+  return out;
+}
+
+template <typename Row>
+__device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol, uint64_t base,
+                                             int count) {
+  constexpr int TXB = sizeof(Row) * 8;
+  Row bits = 0;
+  if (d.dtype == 0) {
+    const uint8_t *p = (const uint8_t *)vol + base;
+    const uint32_t lab = (uint8_t)d.label;
+    if (count == TXB && (((uintptr_t)p) & 15) == 0) {
+      const u32x4 *q = (const u32x4 *)p;
+#pragma unroll
+      for (int k = 0; k < TXB / 16; ++k)
+        bits |= (Row)eq_bits_u8x16(__builtin_nontemporal_load(q + k), lab * 0x01010101u)
+                << (16 * k);
+    } else {
+      for (int k = 0; k < count; ++k) bits |= (Row)(p[k] == lab) << k;
+    }
+  } else {
+    const int32_t *p = (const int32_t *)vol + base;
+    const int32_t lab = d.label;
+    if (count == TXB && (((uintptr_t)p) & 15) == 0) {
+      const i32x4 *q = (const i32x4 *)p;
+#pragma unroll
+      for (int k = 0; k < TXB / 4; ++k) {
+        const i32x4 v = __builtin_nontemporal_load(q + k);
+        bits |= (Row)(v.x == lab) << (4 * k);
+        bits |= (Row)(v.y == lab) << (4 * k + 1);
+        bits |= (Row)(v.z == lab) << (4 * k + 2);
+        bits |= (Row)(v.w == lab) << (4 * k + 3);
+      }
+    } else {
+      for (int k = 0; k < count; ++k) bits |= (Row)(p[k] == lab) << k;
+    }
+  }
+  return bits;
+}
+
+template <int TXB, int TY, int TZ>
+__global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
+                                                          const PackIndex *__restrict__ idx,
+                                                          PairCtr *ctrs, int npairs,
+                                                          uint64_t total_words) {
+  using Row = RowT<TXB>;
+  constexpr int R = TY * TZ;
+  const uint64_t g = (uint64_t)blockIdx.x * kWgThreads + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool in = g < total_words;
+  int p = 0;
+  if (in && npairs > 1) p = find_pair(npairs, g, [&](int i) { return idx[i].rw_base; });
+  int tx = 0x7FFFFFFF, ty = 0x7FFFFFFF, tz = 0x7FFFFFFF;
+  bool nonempty = false;
+  if (in) {
+    const PairDesc &d = descs[p];
+    const uint64_t w = g - idx[p].rw_base;
+    const int wx = (int)(w % (uint64_t)d.wxn);
+    const uint64_t rest = w / (uint64_t)d.wxn;
+    const int y = (int)(rest % (uint64_t)d.ny);
+    const int z = (int)(rest / (uint64_t)d.ny);
+    const uint64_t vbase = (uint64_t)wx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+    const int count = min(TXB, d.nx - wx * TXB);
+    const Row a = load_row_bits<Row>(d, d.gold, vbase, count);
+    const Row b = load_row_bits<Row>(d, d.segm, vbase, count);
+    const uint32_t tile = (uint32_t)wx + (uint32_t)d.gx * ((uint32_t)(y / TY) + (uint32_t)d.gy * (uint32_t)(z / TZ));
+    const uint64_t o = (uint64_t)tile * R + (uint64_t)(y % TY) + (uint64_t)TY * (z % TZ);
+    ((Row *)d.orig[0])[o] = a;
+    ((Row *)d.orig[1])[o] = b;
+    if (w < d.ntiles) {  // idle state for every tile before the metadata pass
+      d.state[0][w] = 0;
+      d.state[1][w] = 0;
+    }
+    if (d.map) {
+      uint16_t *mp = d.map + vbase;
+      for (int k = 0; k < count; ++k) mp[k] = 0;
+    }
+    if (a | b) {
+      nonempty = true;
+      tx = wx, ty = y / TY, tz = z / TZ;
+    }
+  }
+  // bounding box of non-empty tiles: wave min/max, then one atomic per value
+  // that improves on what is already published (most waves publish nothing)
+  const int p_first = __builtin_amdgcn_readfirstlane(p);
+  const bool uniform = __ballot(in && p != p_first) == 0ull;
+  if (uniform) {
+    int mn[3] = {tx, ty, tz};
+    int mx[3] = {nonempty ? tx : -1, nonempty ? ty : -1, nonempty ? tz : -1};
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = min(mn[k], (int)shfl32((uint32_t)mn[k], lane ^ off));
+        mx[k] = max(mx[k], (int)shfl32((uint32_t)mx[k], lane ^ off));
+      }
+    }
+    if (lane == 0 && mx[0] >= 0) {
+      PairCtr &c = ctrs[p_first];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (mn[k] < __hip_atomic_load(&c.lo[k], __ATOMIC_RELAXED, AGENT)) atomicMin(&c.lo[k], mn[k]);
+        if (mx[k] > __hip_atomic_load(&c.hi[k], __ATOMIC_RELAXED, AGENT)) atomicMax(&c.hi[k], mx[k]);
+      }
+    }
+  } else if (nonempty) {
+    PairCtr &c = ctrs[p];
+    atomicMin(&c.lo[0], tx), atomicMin(&c.lo[1], ty), atomicMin(&c.lo[2], tz);
+    atomicMax(&c.hi[0], tx), atomicMax(&c.hi[1], ty), atomicMax(&c.hi[2], tz);
+  }
+}
+
+// ---------------------------------------------------------------- helpers
+struct Box {
+  int lo[3], hi[3];
+};
+__device__ __forceinline__ Box pair_box(const PairDesc &d, const PairCtr &c) {
+  Box b;
+  if (d.crop) {
+    b.lo[0] = c.lo[0], b.lo[1] = c.lo[1], b.lo[2] = c.lo[2];
+    b.hi[0] = c.hi[0], b.hi[1] = c.hi[1], b.hi[2] = c.hi[2];
+  } else {
+    b.lo[0] = b.lo[1] = b.lo[2] = 0;
+    b.hi[0] = d.gx - 1, b.hi[1] = d.gy - 1, b.hi[2] = d.gz - 1;
+  }
+  return b;
+}
+
+// neighbour tile of `tile` in direction dir (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z),
+// or kNoTile when it is outside the (cropped) grid
+__device__ __forceinline__ uint32_t neighbour(const PairDesc &d, const Box &b, uint32_t tile,
+                                              int tx, int ty, int tz, int dir) {
+  switch (dir) {
+    case 0: return tx > b.lo[0] ? tile - 1 : kNoTile;
+    case 1: return tx < b.hi[0] ? tile + 1 : kNoTile;
+    case 2: return ty > b.lo[1] ? tile - (uint32_t)d.gx : kNoTile;
+    case 3: return ty < b.hi[1] ? tile + (uint32_t)d.gx : kNoTile;
+    case 4: return tz > b.lo[2] ? tile - (uint32_t)d.gx * d.gy : kNoTile;
+    default: return tz < b.hi[2] ? tile + (uint32_t)d.gx * d.gy : kNoTile;
+  }
+}
+
+// Wave-aggregated push of one entry per lane with `want` set: reserve in the
+// workgroup's LDS queue, or straight in the global queue when LDS is full.
+__device__ __forceinline__ void push_entries(bool want, uint64_t ent, uint64_t *lq,
+                                             uint32_t *lq_n, uint64_t *gq, uint32_t *gq_len,
+                                             int lane) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const uint32_t n = (uint32_t)__popcll(m);
+  uint32_t base = 0;
+  bool global = false;
+  if (lane == 0) {  // reserve only if the whole run fits, so LDS stays hole-free
+    uint32_t cur = *lq_n;
+    for (;;) {
+      if (cur + n > (uint32_t)kLocalQueue) {
+        base = atomicAdd(gq_len, n);
+        global = true;
+        break;
+      }
+      const uint32_t prev = atomicCAS(lq_n, cur, cur + n);
+      if (prev == cur) {
+        base = cur;
+        break;
+      }
+      cur = prev;
+    }
+  }
+  base = (uint32_t)__shfl((int)base, 0);
+  global = __shfl((int)global, 0) != 0;
+  if (want) {
+    const uint32_t pos = base + lanes_below(m);
+    if (global)
+      gq[pos] = ent;
+    else
+      lq[pos] = ent;
+  }
+}
+
+__device__ __forceinline__ void flush_local_queue(uint64_t *lq, uint32_t *lq_n, uint32_t *lq_base,
+                                                  uint64_t *gq, uint32_t *gq_len) {
+  __syncthreads();
+  const uint32_t n = *lq_n;
+  if (n != 0) {
+    if (threadIdx.x == 0) *lq_base = atomicAdd(gq_len, n);
+    __syncthreads();
+    const uint32_t b = *lq_base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gq[b + i] = lq[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *lq_n = 0;
+}
+
+// ---------------------------------------------------------------- meta
+template <int TXB, int TY, int TZ>
+__global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__restrict__ descs,
+                                                          const PackIndex *__restrict__ idx,
+                                                          PairCtr *ctrs, int npairs, Engine *es,
+                                                          uint64_t *q0, uint64_t *q1,
+                                                          uint64_t total_tiles, int p0) {
+  using Row = RowT<TXB>;
+  constexpr int R = TY * TZ;
+  constexpr int TPW = kWave / R;
+  __shared__ uint64_t lq[kLocalQueue];
+  __shared__ uint32_t lq_n, lq_base;
+  if (threadIdx.x == 0) lq_n = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int r = lane % R;
+  const int yl = r % TY, zl = r / TY;
+  const uint32_t t = es->t;
+  uint64_t *gq = (t & 1) ? q1 : q0;
+  uint32_t *gq_len = &es->qlen[t % 3];
+  const uint64_t gtile = ((uint64_t)blockIdx.x * kWgThreads + threadIdx.x) / R;
+  const bool in = gtile < total_tiles;
+  int p = 0;
+  if (in && npairs > 1) p = find_pair(npairs, gtile, [&](int i) { return idx[i].tile_base; });
+  const PairDesc &d = descs[p];
+  const uint32_t tile = in ? (uint32_t)(gtile - idx[p].tile_base) : 0;
+  const int tx = (int)(tile % (uint32_t)d.gx);
+  const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+  const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+  const int y = ty * TY + yl, z = tz * TZ + zl;
+  const bool rowok = in && y < d.ny && z < d.nz;
+  const Row vb = valid_bits<Row>(tx, d.wxn, d.nx);
+  Row w[2] = {0, 0};
+  if (rowok) {
+    w[0] = ((const Row *)d.orig[0])[(uint64_t)tile * R + r];
+    w[1] = ((const Row *)d.orig[1])[(uint64_t)tile * R + r];
+  }
+  // counts: |gold|, |segm|, |FP| = |segm & ~gold|, |FN| = |gold & ~segm| (hello.cu:624-625)
+  uint32_t cg = group_sum<R>((uint32_t)popc_row(w[0]), lane);
+  uint32_t cs = group_sum<R>((uint32_t)popc_row(w[1]), lane);
+  uint32_t cfp = group_sum<R>((uint32_t)popc_row((Row)(w[1] & ~w[0])), lane);
+  uint32_t cfn = group_sum<R>((uint32_t)popc_row((Row)(w[0] & ~w[1])), lane);
+  PairCtr &c = ctrs[p];
+  if (in && r == 0) {
+    if (cg) atomicAdd(&c.n_gold, (unsigned long long)cg);
+    if (cs) atomicAdd(&c.n_segm, (unsigned long long)cs);
+    if (cfp) atomicAdd(&c.n_fp, (unsigned long long)cfp);
+    if (cfn) atomicAdd(&c.n_fn, (unsigned long long)cfn);
+  }
+  const Box b = pair_box(d, c);
+  const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] && ty <= b.hi[1] &&
+                      tz >= b.lo[2] && tz <= b.hi[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const uint64_t fxl = group_bits<R>(__ballot(rowok && (w[m] & 1)), lane);
+    const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w[m] >> (TXB - 1)) & 1)), lane);
+    const bool any = group_bits<R>(__ballot(w[m] != 0), lane) != 0;
+    const bool full = group_bits<R>(__ballot(rowok && w[m] != vb), lane) == 0;
+    if (in && r == 0) {
+      d.xf_orig[m][2 * (uint64_t)tile] = fxl;
+      d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
+    }
+    const bool fyl = group_bits<R>(__ballot(rowok && yl == 0 && w[m] != 0), lane) != 0;
+    const bool fyh = group_bits<R>(__ballot(rowok && yl == TY - 1 && w[m] != 0), lane) != 0;
+    const bool fzl = group_bits<R>(__ballot(rowok && zl == 0 && w[m] != 0), lane) != 0;
+    const bool fzh = group_bits<R>(__ballot(rowok && zl == TZ - 1 && w[m] != 0), lane) != 0;
+    // lane 6: the tile itself; lanes 0..5: its face neighbours
+    bool want = false;
+    uint64_t ent = 0;
+    if (in_box && any) {
+      if (r == 6) {
+        const uint32_t nv = full ? kFull : 1u;
+        if (atomicCAS(&d.state[m][tile], 0u, nv) == 0u && !full) {
+          want = true;
+          ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+        }
+      } else if (r < 6) {
+        const bool face = r == 0 ? fxl != 0 : r == 1 ? fxh != 0 : r == 2 ? fyl : r == 3 ? fyh
+                                                                  : r == 4 ? fzl : fzh;
+        const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, r);
+        if (face && nb != kNoTile && atomicCAS(&d.state[m][nb], 0u, 1u) == 0u) {
+          want = true;
+          ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+        }
+      }
+    }
+    push_entries(want, ent, lq, &lq_n, gq, gq_len, lane);
+  }
+  (void)TPW;
+  flush_local_queue(lq, &lq_n, &lq_base, gq, gq_len);
+}
+
+// ---------------------------------------------------------------- barrier
+// Sense-reversing grid barrier on an occupancy-bounded grid.  The spin is
+// bounded by wall-clock ticks: on expiry the engine error flag is set and the
+// caller leaves its loop, so no wave can wait forever.
+__device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t nblocks, uint64_t limit,
+                                             int *s_ok) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const uint32_t gen = __hip_atomic_load(&es->bar_gen, __ATOMIC_RELAXED, AGENT);
+    const uint32_t arrived = __hip_atomic_fetch_add(&es->bar_count, 1u, __ATOMIC_ACQ_REL, AGENT);
+    if (arrived == nblocks - 1) {
+      __hip_atomic_store(&es->bar_count, 0u, __ATOMIC_RELAXED, AGENT);
+      __hip_atomic_store(&es->bar_gen, gen + 1, __ATOMIC_RELEASE, AGENT);
+    } else {
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&es->bar_gen, __ATOMIC_ACQUIRE, AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+          __hip_atomic_fetch_or(&es->error, 1u, __ATOMIC_RELAXED, AGENT);
+          ok = 0;
+          break;
+        }
+      }
+    }
+    if (__hip_atomic_load(&es->error, __ATOMIC_RELAXED, AGENT)) ok = 0;
+    *s_ok = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *s_ok != 0;
+}
+
+// ---------------------------------------------------------------- dilate
+template <int TXB, int TY, int TZ>
+__global__ __launch_bounds__(kWgThreads) void dilate_kernel(
+    const PairDesc *__restrict__ descs, PairCtr *ctrs, Engine *es, uint64_t *q0, uint64_t *q1,
+    int max_iters, int chunk, uint64_t spin_limit, uint32_t *qstats, int qstats_len) {
+  using Row = RowT<TXB>;
+  constexpr int R = TY * TZ;
+  constexpr int TPW = kWave / R;
+  __shared__ uint64_t lq[kLocalQueue];
+  __shared__ uint32_t lq_n, lq_base;
+  __shared__ int s_ok;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int sub = lane / R;
+  const int r = lane % R;
+  const int yl = r % TY, zl = r / TY;
+  const int gbase = lane - r;  // first lane of this lane's group
+  const uint32_t gwave = blockIdx.x * kWavesPerWg + wave;
+  const uint32_t nwave = gridDim.x * kWavesPerWg;
+  const uint32_t step = (uint32_t)chunk * TPW;
+  if (threadIdx.x == 0) lq_n = 0;
+  __syncthreads();
+  uint32_t t = __hip_atomic_load(&es->t, __ATOMIC_RELAXED, AGENT);
+  for (int it = 0; it < max_iters; ++it) {
+    const uint32_t L = __hip_atomic_load(&es->qlen[t % 3], __ATOMIC_RELAXED, AGENT);
+    if (L == 0) break;
+    const uint64_t *Qin = (t & 1) ? q1 : q0;
+    uint64_t *Qout = (t & 1) ? q0 : q1;
+    uint32_t *qout_len = &es->qlen[(t + 1) % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      es->qlen[(t + 2) % 3] = 0;
+      es->qtotal += L;
+      es->qmax = max(es->qmax, L);
+      if (qstats && (int)t < qstats_len) qstats[t] = L;
+    }
+    for (uint32_t base = gwave * step; base < L; base += nwave * step) {
+      for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
+        const uint32_t ei = base + k + sub;
+        bool live = ei < L;
+        const uint64_t ent = live ? Qin[ei] : 0;
+        const uint32_t tile = (uint32_t)ent;
+        const int p = (int)((ent >> 32) & 0xFFFF);
+        const int m = (int)((ent >> 48) & 1);
+        const bool first = (ent >> 49) & 1;
+        const bool check = (ent >> 50) & 1;
+        const PairDesc &d = descs[p];
+        PairCtr &c = ctrs[p];
+        if (live) {  // per-mask termination (hello.cu:1748-1759)
+          const unsigned long long n = m == 0 ? c.n_fp : c.n_fn;
+          const unsigned long long cov = __hip_atomic_load(&c.cov[m], __ATOMIC_RELAXED, AGENT);
+          if (cov >= robust_cut(d.robust, n)) live = false;
+        }
+        const uint32_t tp = t - d.t_start;  // pair-local iteration
+        if (live && tp + 1 >= (uint32_t)d.hist_len) {
+          c.status = -8;
+          live = false;
+        }
+        const int tx = (int)(tile % (uint32_t)d.gx);
+        const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+        const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+        const int y = ty * TY + yl, z = tz * TZ + zl;
+        const bool exists = y < d.ny && z < d.nz;
+        const bool rowok = live && exists;
+        const Row vb = valid_bits<Row>(tx, d.wxn, d.nx);
+        const Box b = pair_box(d, c);
+        const int rd = (tp + 1) & 1;  // slot holding iteration tp's input
+        const int wr = tp & 1;        // slot receiving iteration tp's output
+        const Row *orig_m = (const Row *)d.orig[m];
+        const Row *bufr = (const Row *)d.buf[m][rd];
+        const Row *src = first ? orig_m : bufr;
+        const uint64_t ro = (uint64_t)tile * R + r;
+        // -- issue every load of this step before using any of them
+        const Row own = rowok ? src[ro] : Row(0);
+        const Row other = (rowok && check) ? ((const Row *)d.orig[1 - m])[ro] : Row(0);
+        uint32_t nb_dir = kNoTile, nst = 0;
+        if (live && r < 6) {
+          nb_dir = neighbour(d, b, tile, tx, ty, tz, r);
+          if (nb_dir != kNoTile) nst = d.state[m][nb_dir];
+        }
+        const uint32_t nb_xl = live ? neighbour(d, b, tile, tx, ty, tz, 0) : kNoTile;
+        const uint32_t nb_xh = live ? neighbour(d, b, tile, tx, ty, tz, 1) : kNoTile;
+        uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
+        if (nb_xl != kNoTile) {
+          fxl_o = d.xf_orig[m][2 * (uint64_t)nb_xl + 1];
+          fxl_b = d.xf_buf[m][rd][2 * (uint64_t)nb_xl + 1];
+        }
+        if (nb_xh != kNoTile) {
+          fxh_o = d.xf_orig[m][2 * (uint64_t)nb_xh];
+          fxh_b = d.xf_buf[m][rd][2 * (uint64_t)nb_xh];
+        }
+        // y / z face rows from the neighbour tiles, both candidate sources
+        uint32_t nb_yz[4] = {kNoTile, kNoTile, kNoTile, kNoTile};
+        Row fo[4] = {0, 0, 0, 0}, fb[4] = {0, 0, 0, 0};
+        if (rowok) {
+          const int on_face[4] = {yl == 0, yl == TY - 1, zl == 0, zl == TZ - 1};
+          const int src_row[4] = {(TY - 1) + TY * zl, TY * zl, yl + TY * (TZ - 1), yl};
+#pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            if (!on_face[f]) continue;
+            nb_yz[f] = neighbour(d, b, tile, tx, ty, tz, 2 + f);
+            if (nb_yz[f] == kNoTile) continue;
+            const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
+            fo[f] = orig_m[o];
+            fb[f] = bufr[o];
+          }
+        }
+        // -- neighbour states -> which candidate holds iteration tp's input
+        const uint32_t st_xl = shfl32(nst, gbase + 0), st_xh = shfl32(nst, gbase + 1);
+        const uint32_t st_f[4] = {shfl32(nst, gbase + 2), shfl32(nst, gbase + 3),
+                                  shfl32(nst, gbase + 4), shfl32(nst, gbase + 5)};
+        auto pick64 = [&](uint32_t nb, uint32_t st, uint64_t o, uint64_t bsel) -> uint64_t {
+          if (nb == kNoTile) return 0;
+          if (st & kFull) return ~0ull;
+          if (st != 0 && st - 1 < tp) return bsel;
+          return o;
+        };
+        const uint64_t wxl = pick64(nb_xl, st_xl, fxl_o, fxl_b);
+        const uint64_t wxh = pick64(nb_xh, st_xh, fxh_o, fxh_b);
+        Row face[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const uint32_t nb = nb_yz[f];
+          const uint32_t st = st_f[f];
+          face[f] = nb == kNoTile ? Row(0)
+                    : (st & kFull) ? ~Row(0)
+                    : (st != 0 && st - 1 < tp) ? fb[f] : fo[f];
+        }
+        // -- 6-connectivity dilation of the row (bitDilatate + faces,
+        //    hello.cu:373-376, 1310-1542)
+        const Row up = shfl_row<Row>(own, lane - 1);
+        const Row dn = shfl_row<Row>(own, lane + 1);
+        const Row bk = shfl_row<Row>(own, lane - TY);
+        const Row fw = shfl_row<Row>(own, lane + TY);
+        Row dl = own | (Row)(own << 1) | (Row)(own >> 1);
+        dl |= (Row)((wxl >> r) & 1);
+        dl |= (Row)((wxh >> r) & 1) << (TXB - 1);
+        dl |= yl > 0 ? up : face[0];
+        dl |= yl < TY - 1 ? dn : face[1];
+        dl |= zl > 0 ? bk : face[2];
+        dl |= zl < TZ - 1 ? fw : face[3];
+        dl &= vb;
+        if (!rowok) dl = 0;
+        // -- input full -> the tile leaves the queue (hello.cu:1319, 1634-1643)
+        const bool full = group_bits<R>(__ballot(live && exists && own != vb), lane) == 0;
+        // -- newly covered voxels of the other mask: res & ~src & ref (hello.cu:1590-1626)
+        const Row nw = dl & (Row)~own & other;
+        const uint32_t cnt = group_sum<R>((uint32_t)popc_row(nw), lane);
+        const uint32_t rem = group_sum<R>((uint32_t)popc_row((Row)(other & ~dl)), lane);
+        if (nw && d.map) {
+          Row bits = nw;
+          const uint64_t vrow = (uint64_t)tx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+          while (bits) {
+            const int k = (int)__builtin_ctzll((uint64_t)bits);
+            bits &= bits - 1;
+            d.map[vrow + k] = (uint16_t)(tp + 1);
+          }
+        }
+        if (live && r == 0 && cnt) {
+          atomicAdd(&d.hist[m][tp + 1], cnt);
+          atomicAdd(&c.cov[m], (unsigned long long)cnt);
+        }
+        // -- outputs
+        const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
+        const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
+        const bool oyl = group_bits<R>(__ballot(yl == 0 && dl != 0), lane) != 0;
+        const bool oyh = group_bits<R>(__ballot(yl == TY - 1 && dl != 0), lane) != 0;
+        const bool ozl = group_bits<R>(__ballot(zl == 0 && dl != 0), lane) != 0;
+        const bool ozh = group_bits<R>(__ballot(zl == TZ - 1 && dl != 0), lane) != 0;
+        bool want = false;
+        uint64_t pent = 0;
+        if (live) {
+          if (full) {
+            if (r == 0) atomicOr(&d.state[m][tile], kFull);
+          } else {
+            ((Row *)d.buf[m][wr])[ro] = dl;
+            if (r == 0) {
+              d.xf_buf[m][wr][2 * (uint64_t)tile] = oxl;
+              d.xf_buf[m][wr][2 * (uint64_t)tile + 1] = oxh;
+            }
+            if (r == 6) {  // stays queued while not full
+              want = true;
+              pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0);
+            }
+            if (r < 6 && nb_dir != kNoTile && nst == 0) {  // activate idle neighbours
+              const bool face_on = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
+                                   : r == 3 ? oyh : r == 4 ? ozl : ozh;
+              if (face_on && atomicCAS(&d.state[m][nb_dir], 0u, tp + 2) == 0u) {
+                want = true;
+                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1);
+              }
+            }
+          }
+        }
+        push_entries(want, pent, lq, &lq_n, Qout, qout_len, lane);
+      }
+    }
+    flush_local_queue(lq, &lq_n, &lq_base, Qout, qout_len);
+    ++t;
+    if (!grid_barrier(es, gridDim.x, spin_limit, &s_ok)) break;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) __hip_atomic_store(&es->t, t, __ATOMIC_RELAXED, AGENT);
+}
+
+// ---------------------------------------------------------------- finalize
+// HD, sums and counts per pair from the first-hit histograms, with the robust
+// cut: the pass of mask m stops at the first t where the covered count reaches
+// ceil(r * total) (hello.cu:1748-1759, 1887-1895).
+__global__ __launch_bounds__(kWgThreads) void finalize_kernel(const PairDesc *__restrict__ descs,
+                                                              const PairCtr *__restrict__ ctrs,
+                                                              const Engine *es, PairOut *outs,
+                                                              int p0, int npairs) {
+  __shared__ unsigned long long s_pref[kWgThreads];
+  __shared__ int s_stop;
+  const int p = p0 + blockIdx.x;
+  if (p >= p0 + npairs) return;
+  const PairDesc &d = descs[p];
+  const PairCtr &c = ctrs[p];
+  int hd[2] = {0, 0};
+  unsigned long long sum[2] = {0, 0}, cnt[2] = {0, 0};
+  bool reached[2] = {true, true};
+  for (int m = 0; m < 2; ++m) {
+    const unsigned long long tgt = robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn);
+    if (tgt == 0) continue;
+    reached[m] = false;
+    unsigned long long carry = 0, lsum = 0, lcnt = 0;
+    int lhd = 0;
+    for (int t0 = 1; t0 < d.hist_len; t0 += kWgThreads) {
+      const int t = t0 + threadIdx.x;
+      const unsigned long long h = t < d.hist_len ? d.hist[m][t] : 0;
+      s_pref[threadIdx.x] = h;
+      __syncthreads();
+      for (int off = 1; off < kWgThreads; off <<= 1) {  // inclusive scan
+        const unsigned long long v = threadIdx.x >= off ? s_pref[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_pref[threadIdx.x] += v;
+        __syncthreads();
+      }
+      const unsigned long long pre = carry + s_pref[threadIdx.x];
+      if (threadIdx.x == 0) s_stop = 0x7FFFFFFF;
+      __syncthreads();
+      if (t < d.hist_len && pre >= tgt) atomicMin(&s_stop, t);
+      __syncthreads();
+      const int stop = s_stop;
+      if (t < d.hist_len && t <= stop) {
+        lsum += (unsigned long long)t * h;
+        lcnt += h;
+        if (h) lhd = t;
+      }
+      carry += s_pref[kWgThreads - 1];
+      __syncthreads();
+      if (stop != 0x7FFFFFFF) {
+        reached[m] = true;
+        break;
+      }
+    }
+    // reduce lsum, lcnt, max lhd over the block
+    s_pref[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) s_pref[threadIdx.x] += s_pref[threadIdx.x + off];
+      __syncthreads();
+    }
+    sum[m] = s_pref[0];
+    __syncthreads();
+    s_pref[threadIdx.x] = lcnt;
+    __syncthreads();
+    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) s_pref[threadIdx.x] += s_pref[threadIdx.x + off];
+      __syncthreads();
+    }
+    cnt[m] = s_pref[0];
+    __syncthreads();
+    s_pref[threadIdx.x] = (unsigned long long)lhd;
+    __syncthreads();
+    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) s_pref[threadIdx.x] = max(s_pref[threadIdx.x], s_pref[threadIdx.x + off]);
+      __syncthreads();
+    }
+    hd[m] = (int)s_pref[0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    PairOut o;
+    o.hd_fp = hd[0], o.hd_fn = hd[1];
+    o.hd = max(hd[0], hd[1]);
+    o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
+    o.n_gold = c.n_gold, o.n_segm = c.n_segm, o.n_fp = c.n_fp, o.n_fn = c.n_fn;
+    int st = c.status;
+    if ((c.n_fp && !c.n_gold) || (c.n_fn && !c.n_segm))
+      st = -2;  // HDM_E_EMPTY_MASK
+    else if (es->error)
+      st = -5;  // HDM_E_TIMEOUT
+    else if (st == 0 && (!reached[0] || !reached[1]))
+      st = -8;  // HDM_E_STUCK
+    o.status = st;
+    outs[p] = o;
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+template <int TXB, int TY, int TZ>
+hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
+                       PairCtr *c, int np, uint64_t tw) {
+  hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw);
+  return hipGetLastError();
+}
+template <int TXB, int TY, int TZ>
+hipError_t launch_meta(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
+                       PairCtr *c, int np, Engine *es, uint64_t *q0, uint64_t *q1,
+                       uint64_t tt, int p0) {
+  hipLaunchKernelGGL((meta_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, es,
+                     q0, q1, tt, p0);
+  return hipGetLastError();
+}
+template <int TXB, int TY, int TZ>
+hipError_t launch_dilate(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c, Engine *es,
+                         uint64_t *q0, uint64_t *q1, int max_iters, int chunk, uint64_t spin,
+                         uint32_t *qs, int qsl) {
+  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, c, es, q0,
+                     q1, max_iters, chunk, spin, qs, qsl);
+  return hipGetLastError();
+}
+
+template <int TXB, int TY, int TZ>
+TileLaunch make_launch() {
+  return TileLaunch{&launch_pack<TXB, TY, TZ>, &launch_meta<TXB, TY, TZ>,
+                    &launch_dilate<TXB, TY, TZ>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ>, TXB, TY, TZ};
+}
+
+}  // namespace hdm
+
+// table indexed by HDM_TILE_* (1..4)
+extern "C" const hdm::TileLaunch *hdm_tile_launch(int kind) {
+  static const hdm::TileLaunch table[5] = {
+      hdm::make_launch<64, 8, 8>(), hdm::make_launch<64, 8, 8>(), hdm::make_launch<64, 4, 4>(),
+      hdm::make_launch<64, 8, 4>(), hdm::make_launch<32, 8, 8>()};
+  if (kind < 0 || kind > 4) return nullptr;
+  return &table[kind];
+}
+
+extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
+                                       const hdm::PairDesc *single,
+                                       const hdm::PackIndex *single_idx, hdm::PackIndex *idx,
+                                       hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
+                                       int reset_engine, uint32_t qcap) {
+  hdm::PairDesc one{};
+  hdm::PackIndex one_idx{};
+  if (single) one = *single;
+  if (single_idx) one_idx = *single_idx;
+  const int blocks = std::max(1, std::min(64, (npairs + 255) / 256));
+  hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(256), 0, s, descs, one, one_idx, idx,
+                     single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(hdm::clear_hist_kernel, dim3(4, std::min(npairs, 1024)), dim3(256), 0, s,
+                     descs, p0, npairs);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t hdm_launch_finalize(hipStream_t s, const hdm::PairDesc *descs,
+                                          const hdm::PairCtr *ctrs, const hdm::Engine *es,
+                                          hdm::PairOut *outs, int p0, int npairs) {
+  hipLaunchKernelGGL(hdm::finalize_kernel, dim3(npairs), dim3(hdm::kWgThreads), 0, s, descs,
+                     ctrs, es, outs, p0, npairs);
+  return hipGetLastError();
+}

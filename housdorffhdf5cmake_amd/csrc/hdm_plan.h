@@ -1,0 +1,149 @@
+// Host-side planning: tile geometry, workspace layout and launch shapes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/hdm.h"
+#include "hdm_engine.h"
+
+namespace hdm {
+
+struct PackIndex {
+  uint64_t rw_base;
+  uint64_t tile_base;
+};
+
+struct TileLaunch {
+  hipError_t (*pack)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
+                     uint64_t);
+  hipError_t (*meta)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
+                     Engine *, uint64_t *, uint64_t *, uint64_t, int);
+  hipError_t (*dilate)(dim3, hipStream_t, const PairDesc *, PairCtr *, Engine *, uint64_t *,
+                       uint64_t *, int, int, uint64_t, uint32_t *, int);
+  const void *dilate_fn;
+  int txb, ty, tz;
+};
+
+struct TileShape {
+  int txb, ty, tz;
+  int rows() const { return ty * tz; }
+};
+
+inline TileShape tile_shape(int kind) {
+  switch (kind) {
+    case HDM_TILE_64x4x4: return {64, 4, 4};
+    case HDM_TILE_64x8x4: return {64, 8, 4};
+    case HDM_TILE_32x8x8: return {32, 8, 8};
+    default: return {64, 8, 8};
+  }
+}
+
+inline uint64_t align256(uint64_t x) { return (x + 255) & ~uint64_t(255); }
+
+// Geometry and byte offsets (relative to the pair's block) of one pair.
+struct PairGeom {
+  int nx, ny, nz, wxn, gx, gy, gz;
+  uint32_t ntiles;
+  uint64_t words;      // ntiles * rows
+  uint64_t row_words;  // wxn * ny * nz, the pack grid
+  int hist_len;
+  uint64_t buf_bytes, xf_bytes, st_bytes, hist_bytes, total;
+};
+
+inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
+  PairGeom g{};
+  g.nx = nx, g.ny = ny, g.nz = nz;
+  g.wxn = (nx + ts.txb - 1) / ts.txb;
+  g.gx = g.wxn;
+  g.gy = (ny + ts.ty - 1) / ts.ty;
+  g.gz = (nz + ts.tz - 1) / ts.tz;
+  g.ntiles = (uint32_t)((uint64_t)g.gx * g.gy * g.gz);
+  g.words = (uint64_t)g.ntiles * ts.rows();
+  g.row_words = (uint64_t)g.wxn * ny * nz;
+  g.hist_len = nx + ny + nz + 2;
+  g.buf_bytes = align256(g.words * (ts.txb / 8));
+  g.xf_bytes = align256((uint64_t)g.ntiles * 16);
+  g.st_bytes = align256((uint64_t)g.ntiles * 4);
+  g.hist_bytes = align256((uint64_t)g.hist_len * 4);
+  g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + 2 * g.hist_bytes;
+  return g;
+}
+
+// Fill the device pointers of a descriptor from the pair's block base.
+inline void carve_pair(PairDesc &d, const PairGeom &g, char *base) {
+  char *p = base;
+  auto take = [&](uint64_t n) {
+    char *q = p;
+    p += n;
+    return q;
+  };
+  d.orig[0] = take(g.buf_bytes);
+  d.orig[1] = take(g.buf_bytes);
+  for (int m = 0; m < 2; ++m)
+    for (int s = 0; s < 2; ++s) d.buf[m][s] = take(g.buf_bytes);
+  d.xf_orig[0] = (uint64_t *)take(g.xf_bytes);
+  d.xf_orig[1] = (uint64_t *)take(g.xf_bytes);
+  for (int m = 0; m < 2; ++m)
+    for (int s = 0; s < 2; ++s) d.xf_buf[m][s] = (uint64_t *)take(g.xf_bytes);
+  d.state[0] = (uint32_t *)take(g.st_bytes);
+  d.state[1] = (uint32_t *)take(g.st_bytes);
+  d.hist[0] = (uint32_t *)take(g.hist_bytes);
+  d.hist[1] = (uint32_t *)take(g.hist_bytes);
+}
+
+inline void fill_desc(PairDesc &d, const PairGeom &g, const void *gold, const void *segm,
+                      int dtype, int label, double robust, int crop, int tile_kind,
+                      uint32_t t_start) {
+  d.gold = gold, d.segm = segm;
+  d.nx = g.nx, d.ny = g.ny, d.nz = g.nz, d.wxn = g.wxn;
+  d.gx = g.gx, d.gy = g.gy, d.gz = g.gz;
+  d.ntiles = g.ntiles;
+  d.dtype = dtype, d.label = label;
+  d.hist_len = g.hist_len;
+  d.t_start = t_start;
+  d.crop = crop;
+  d.tile_kind = tile_kind;
+  d.robust = robust;
+  d.words = g.words;
+  d.map = nullptr;
+}
+
+// Shared (per launch group) region: descriptors, indices, counters, outputs,
+// engine state and the two queue arrays.
+struct CommonLayout {
+  uint64_t descs, idx, ctrs, outs, engine, q0, q1, qstats, total;
+};
+
+inline CommonLayout common_layout(int npairs, uint64_t qcap, int qstats_len) {
+  CommonLayout L{};
+  uint64_t o = 0;
+  auto take = [&](uint64_t n) {
+    const uint64_t r = o;
+    o += align256(n);
+    return r;
+  };
+  L.descs = take(sizeof(PairDesc) * npairs);
+  L.idx = take(sizeof(PackIndex) * npairs);
+  L.ctrs = take(sizeof(PairCtr) * npairs);
+  L.outs = take(sizeof(PairOut) * npairs);
+  L.engine = take(sizeof(Engine));
+  L.q0 = take(8 * qcap);
+  L.q1 = take(8 * qcap);
+  L.qstats = take(4 * (uint64_t)qstats_len);
+  L.total = o;
+  return L;
+}
+
+}  // namespace hdm
+
+extern "C" const hdm::TileLaunch *hdm_tile_launch(int kind);
+extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
+                                       const hdm::PairDesc *single,
+                                       const hdm::PackIndex *single_idx, hdm::PackIndex *idx,
+                                       hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
+                                       int reset_engine, uint32_t qcap);
+extern "C" hipError_t hdm_launch_finalize(hipStream_t s, const hdm::PairDesc *descs,
+                                          const hdm::PairCtr *ctrs, const hdm::Engine *es,
+                                          hdm::PairOut *outs, int p0, int npairs);

@@ -1,0 +1,89 @@
+"""CPU-only checks of the C-ABI library: it loads, exports every declared
+symbol, and its host-side logic (synthetic generator, sizing, errors) behaves."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import housdorffhdf5cmake_amd as hdm
+from tests.golden_cases import case, manifest, volumes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("hdm.h", "hdm_synth.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(hdm_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    L = hdm.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 28
+    for s in syms:
+        assert hasattr(L, s), s
+    assert L.hdm_abi_version() == 1
+
+
+def test_strerror_and_opts_defaults():
+    L = hdm.lib()
+    assert L.hdm_strerror(0) == b"ok"
+    assert b"empty" in L.hdm_strerror(-2)
+    o = hdm.Opts()
+    L.hdm_opts_init(C.byref(o))
+    assert o.robustness == 1.0 and o.crop == 1 and o.tile == 0 and o.ptr_kind == 0
+
+
+def test_device_count_without_gpu_does_not_fail():
+    n = hdm.lib().hdm_device_count()
+    assert n >= 0
+
+
+def test_workspace_bytes():
+    L = hdm.lib()
+    # 512^3 with 64x8x8 tiles: 6 packed copies of 16 MiB plus small metadata
+    b = L.hdm_workspace_bytes(512, 512, 512, 1)
+    assert 96 << 20 <= b <= 104 << 20
+    assert L.hdm_workspace_bytes(0, 5, 5, 1) == 0
+    assert L.hdm_workspace_bytes(67, 13, 9, 4) > 0
+
+
+@pytest.mark.parametrize("name", [c["name"] for c in manifest() if "builder" in c])
+def test_host_generator_matches_fixture(name):
+    """The library's generator reproduces the numpy restatement bit for bit."""
+    c = case(name)
+    kind, args = c["builder"]
+    sp = hdm.synth_spec(kind, *args)
+    g, s = hdm.synth_host(sp)
+    eg, es = volumes(c)
+    assert np.array_equal(g, eg) and np.array_equal(s, es)
+
+
+def test_generator_rejects_bad_specs():
+    with pytest.raises(hdm.HDMError):
+        hdm.synth_spec("sphere", 0, 8, 8, 4, 4, 4, 2, 0, 0, 0, 0.0, 1)
+    with pytest.raises(hdm.HDMError):
+        hdm.synth_spec("long", 64, 64, 64, 10, 32, 32, 8, 8, 8, 100, 5)  # cube outside
+
+
+def test_invalid_arguments_are_rejected_before_any_device_work():
+    L = hdm.lib()
+    r = hdm.Result()
+    g = np.zeros((4, 4, 4), np.uint8)
+    p = C.c_void_p(g.ctypes.data)
+    assert L.hdm_hausdorff_u8(None, p, 4, 4, 4, 1, None, C.byref(r)) == -1
+    assert L.hdm_hausdorff_u8(p, p, 0, 4, 4, 1, None, C.byref(r)) == -1
+    o = hdm.Opts()
+    L.hdm_opts_init(C.byref(o))
+    o.robustness = 1.5
+    assert L.hdm_hausdorff_u8(p, p, 4, 4, 4, 1, C.byref(o), C.byref(r)) == -1
+    assert L.hdm_batch_create(None, 0, None, 0, None) == -1
