@@ -40,7 +40,7 @@ class HDMError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("robustness", C.c_double), ("ptr_kind", C.c_int32), ("tile", C.c_int32),
                 ("grid_ctas", C.c_int32), ("chunk", C.c_int32), ("crop", C.c_int32),
-                ("reserved0", C.c_int32), ("stream", C.c_void_p), ("map_out", C.c_void_p),
+                ("timing", C.c_int32), ("stream", C.c_void_p), ("map_out", C.c_void_p),
                 ("hist_out", C.c_void_p), ("hist_len", C.c_int32), ("reserved1", C.c_int32)]
 
 
@@ -106,6 +106,7 @@ def lib() -> C.CDLL:
         "hdm_abi_version": ([], i32), "hdm_strerror": ([i32], C.c_char_p),
         "hdm_opts_init": ([C.POINTER(Opts)], None), "hdm_device_count": ([], i32),
         "hdm_workspace_bytes": ([i32, i32, i32, i32], u64),
+        "hdm_last_timings": ([C.POINTER(C.c_float), i32], i32),
         "hdm_hausdorff_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts),
                               C.POINTER(Result)], i32),
         "hdm_hausdorff_i32": ([vp, vp, i32, i32, i32, C.c_int32, C.POINTER(Opts),
@@ -197,7 +198,7 @@ def _volume_args(gold, segm):
 
 
 def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, stream=None,
-              ptr_kind=PTR_DEVICE) -> Opts:
+              ptr_kind=PTR_DEVICE, timing=False) -> Opts:
     o = Opts()
     lib().hdm_opts_init(C.byref(o))
     o.robustness = float(robustness)
@@ -206,19 +207,28 @@ def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, s
     o.chunk = int(chunk)
     o.crop = 1 if crop else 0
     o.ptr_kind = ptr_kind
+    o.timing = 1 if timing else 0
     if stream is not None:
         o.stream = stream
     return o
 
 
+def last_timings() -> dict:
+    """Device ms of the phases of the last call made with timing=True."""
+    ms = (C.c_float * 5)()
+    n = lib().hdm_last_timings(ms, 5)
+    names = ("pack", "meta", "dilate", "finalize", "total")
+    return {names[i]: float(ms[i]) for i in range(n)}
+
+
 def hausdorff(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO,
               grid_ctas: int = 0, chunk: int = 0, crop: bool = True, raise_on_error: bool = True,
-              hist_len: int = 0):
+              hist_len: int = 0, timing: bool = False):
     """HD and average HD of two volumes.  Returns a dict of hdm_result fields
     (plus 'status', and 'hist_fp'/'hist_fn' when hist_len > 0)."""
     gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
     o = make_opts(robustness, tile, grid_ctas, chunk, crop,
-                  _current_stream_ptr() if pk == PTR_DEVICE else None, pk)
+                  _current_stream_ptr() if pk == PTR_DEVICE else None, pk, timing)
     hist = None
     if hist_len > 0:
         hist = np.zeros(2 * hist_len, np.uint32)

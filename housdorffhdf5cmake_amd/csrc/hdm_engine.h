@@ -3,7 +3,7 @@
 // Layout in HBM, per pair (tile = TXB(x) * TY(y) * TZ(z) voxels, R = TY*TZ rows
 // of TXB bits; one wavefront lane owns one row):
 //   orig[m]          packed mask m (0 gold, 1 segm), tile-major: word
-//                    tile*R + (y%TY) + TY*(z%TZ) holds voxels x = 64*tx .. +63
+//                    tile*R + (y%TY) + TY*(z%TZ) holds voxels x = TXB*tx .. +TXB-1
 //   buf[m][s]        two ping-pong slots for the dilated mask m
 //   xf_*[m]          per tile two R-bit words: bit r = bit 0 / bit TXB-1 of row r
 //                    (the tile's -x / +x faces), so a neighbour reads one word
@@ -13,6 +13,11 @@
 //   hist[m][t]       number of voxels of the other mask first covered at t
 // Where a neighbour's state at iteration t lives: full -> all ones; queued
 // before t -> buf[m][(t-1)&1]; otherwise (idle, or first queued at t) -> orig.
+//
+// Inside the persistent kernel every word that one workgroup writes and
+// another reads (tile rows, face words, states, queue entries, counters) is
+// accessed with agent-scope atomic loads and stores, so the grid barrier
+// needs no L2 write-back or invalidate.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -23,6 +28,11 @@ constexpr int kWave = 64;
 constexpr int kWgThreads = 256;
 constexpr int kWavesPerWg = kWgThreads / kWave;
 constexpr int kLocalQueue = 2048;  // entries staged in LDS per workgroup
+constexpr int kSubQ = 16;          // sub-queues: workgroup b flushes into b % kSubQ
+constexpr int kHitSlots = 16;      // per (pair, mask) per-iteration hit counters
+constexpr int kSumSlots = 64;      // per (pair, field) partial voxel counts
+constexpr int kMaxGrid = 4096;     // persistent workgroups (arrival flags)
+constexpr int kLineWords = 32;     // one 128-byte line
 
 constexpr uint32_t kFull = 0x80000000u;
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
@@ -57,11 +67,17 @@ struct PairDesc {
 };
 
 struct PairCtr {
-  unsigned long long n_gold, n_segm, n_fp, n_fn;
-  unsigned long long cov[2];
+  unsigned long long n_gold, n_segm, n_fp, n_fn;  // totals (written before the dilation)
+  unsigned long long cov[2];                      // covered so far; barrier master only
   int32_t lo[3], hi[3];  // tile bounding box of non-empty tiles (inclusive)
   int32_t status;
-  int32_t pad;
+  // First pair-local iteration at which pass m has nothing left to do; written
+  // by the barrier master between iterations, so every workgroup reads the
+  // same value during an iteration.
+  int32_t done_from[2];
+  int32_t pad[3];
+  uint32_t hits[2][kHitSlots];                  // new hits of the running iteration
+  unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
 };
 
 struct PairOut {
@@ -71,15 +87,21 @@ struct PairOut {
 };
 
 struct Engine {
-  uint32_t t;        // completed engine iterations
-  uint32_t qlen[3];  // queue lengths, rotating: read t%3, write (t+1)%3
-  uint32_t bar_count, bar_gen;
-  uint32_t error;
+  // head, copied back to the host (kEngineHostBytes)
+  uint32_t t;      // completed engine iterations
+  uint32_t error;  // 1: a barrier wait exceeded its limit
   uint32_t qmax;
+  uint32_t nwg;
   unsigned long long qtotal;
   uint32_t qcap;
-  uint32_t pad;
+  uint32_t pad0;
+  uint32_t pad1[8];
+  // device only
+  uint32_t gen[kLineWords];            // barrier release epoch
+  uint32_t qlen[3][kLineWords];        // [t % 3][sub-queue]: read t%3, fill (t+1)%3
+  uint32_t arrive[kMaxGrid];           // last epoch each workgroup arrived at
 };
+constexpr size_t kEngineHostBytes = 64;
 
 // ceil(r * n) clamped to [0, n]; the robust cut (old/testMainPassesKernels.cu:81-82)
 __host__ __device__ inline unsigned long long robust_cut(double r, unsigned long long n) {

@@ -24,11 +24,22 @@ struct Ctx {
   int dev = -1;
   hipStream_t stream = nullptr;
   int cus = 0;
-  int occ[HDM_TILE_COUNT] = {};  // resident dilate workgroups per CU, per tile kind
-  double clock_khz = 0;          // wall_clock64() rate
+  int occ[HDM_TILE_COUNT] = {};       // resident dilate workgroups per CU, per tile kind
+  int occ_pack[HDM_TILE_COUNT] = {};  // resident pack workgroups per CU
+  double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
+  hipEvent_t ev[5] = {};     // phase boundaries of a timed call
+  float last_ms[5] = {};
   std::mutex mu;
+};
+
+// launch shapes and optional phase events of one enqueue
+struct LaunchCfg {
+  unsigned pack_blocks = 1, meta_blocks = 1;
+  int grid = 1, chunk = 1;
+  uint64_t spin = 0;
+  const hipEvent_t *ev = nullptr;  // 5 events or null
 };
 
 struct GroupPtrs {
@@ -37,7 +48,7 @@ struct GroupPtrs {
   PairCtr *ctrs = nullptr;
   PairOut *outs = nullptr;
   Engine *engine = nullptr;
-  uint64_t *q0 = nullptr, *q1 = nullptr;
+  uint64_t *q = nullptr;
   uint32_t *qstats = nullptr;
 };
 
@@ -46,11 +57,14 @@ int get_ctx(Ctx **out);
 int resident_grid(const Ctx *c, int tile, int requested);
 uint64_t spin_ticks(const Ctx *c);
 int norm_tile(int tile);
+int barrier_mode();
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,
-                         uint64_t total_rw, uint64_t total_tiles, int rows, uint32_t qcap,
-                         int grid, int chunk, uint64_t spin, int qstats_len);
+                         uint64_t total_rw, uint64_t total_tiles, uint32_t qcap,
+                         const LaunchCfg &cfg, int qstats_len);
+LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
+                     int grid_req, int chunk);
 void fill_result(hdm_result *out, const PairOut &o, const Engine *es, uint32_t tiles,
                  int iterations);
 

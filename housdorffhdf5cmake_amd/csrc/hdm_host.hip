@@ -12,6 +12,7 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -88,9 +89,17 @@ int get_ctx(Ctx **out) {
         nb < 1)
       rc = HDM_E_HIP;
     c->occ[k] = nb;
+    if (rc == HDM_OK &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->pack_fn,
+                                                      kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ_pack[k] = nb;
   }
   if (rc == HDM_OK && hipHostMalloc(&c->host_out, 4096, hipHostMallocDefault) != hipSuccess)
     rc = HDM_E_HIP;
+  for (int i = 0; rc == HDM_OK && i < 5; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = HDM_E_HIP;
   if (rc != HDM_OK) {
     (void)hipGetLastError();
     delete c;
@@ -102,33 +111,69 @@ int get_ctx(Ctx **out) {
 }
 
 int resident_grid(const Ctx *c, int tile, int requested) {
-  const int maxres = c->occ[tile] * c->cus;
+  const int maxres = std::min(c->occ[tile] * c->cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
-  return std::min(maxres, 2 * c->cus);
+  return std::min(maxres, c->cus);
 }
 
 uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.0); }
 
+// Grid barrier: 0 (default) flag array with agent-coherent data accesses and
+// no cache maintenance; 1 the same plus L2 write-back / invalidate at every
+// barrier (HDM_BARRIER_FENCE=1, a conservative setting for comparison).
+int barrier_mode() {
+  static int mode = [] {
+    const char *e = std::getenv("HDM_BARRIER_FENCE");
+    return (e && std::atoi(e) == 1) ? 1 : 0;
+  }();
+  return mode;
+}
+
 int norm_tile(int tile) { return (tile <= 0 || tile >= HDM_TILE_COUNT) ? HDM_TILE_64x8x8 : tile; }
+
+// Launch shapes.  Pack: exactly the resident waves, each walking one
+// contiguous range of row words (one round, no tail).  Metadata: grid-stride
+// over tiles with at most 1024 workgroups so the queue flushes (one per
+// workgroup) stay few per sub-queue counter.  Dilation: occupancy-bounded.
+LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
+                     int grid_req, int chunk) {
+  LaunchCfg L;
+  const uint64_t rows = (uint64_t)tile_shape(tile).rows();
+  const uint64_t pack_want = (total_rw + kWgThreads - 1) / kWgThreads;
+  const uint64_t pack_res = (uint64_t)std::max(1, c->occ_pack[tile]) * c->cus;
+  L.pack_blocks = (unsigned)std::max<uint64_t>(1, std::min(pack_want, pack_res));
+  const uint64_t meta_want = (total_tiles * rows + kWgThreads - 1) / kWgThreads;
+  L.meta_blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(meta_want, 1024));
+  L.grid = resident_grid(c, tile, grid_req);
+  L.chunk = std::max(1, chunk);
+  L.spin = spin_ticks(c);
+  return L;
+}
 
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,
-                         uint64_t total_rw, uint64_t total_tiles, int rows, uint32_t qcap,
-                         int grid, int chunk, uint64_t spin, int qstats_len) {
+                         uint64_t total_rw, uint64_t total_tiles, uint32_t qcap,
+                         const LaunchCfg &cfg, int qstats_len) {
+  auto mark = [&](int i) {
+    return cfg.ev ? hipEventRecord(cfg.ev[i], s) : hipSuccess;
+  };
   hipError_t e = hdm_launch_reset(s, P.descs, single, single_idx, P.idx, P.ctrs, 0, npairs,
                                   P.engine, 1, qcap);
+  if (e != hipSuccess || (e = mark(0)) != hipSuccess) return e;
+  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_rw);
   if (e != hipSuccess) return e;
-  const uint64_t pb = (total_rw + kWgThreads - 1) / kWgThreads;
-  e = tl->pack(dim3((unsigned)pb), s, P.descs, P.idx, P.ctrs, npairs, total_rw);
-  if (e != hipSuccess) return e;
-  const uint64_t mb = (total_tiles * rows + kWgThreads - 1) / kWgThreads;
-  e = tl->meta(dim3((unsigned)mb), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q0, P.q1,
+  e = hdm_launch_totals(s, P.ctrs, 0, npairs);
+  if (e != hipSuccess || (e = mark(1)) != hipSuccess) return e;
+  e = tl->meta(dim3(cfg.meta_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,
                total_tiles, 0);
+  if (e != hipSuccess || (e = mark(2)) != hipSuccess) return e;
+  const int flags = barrier_mode() | (npairs == 1 ? 4 : 0);
+  e = tl->dilate(dim3(cfg.grid), s, P.descs, P.ctrs, npairs, P.engine, P.q, qcap, INT_MAX,
+                 cfg.chunk, cfg.spin, P.qstats, qstats_len, flags);
+  if (e != hipSuccess || (e = mark(3)) != hipSuccess) return e;
+  e = hdm_launch_finalize(s, P.descs, P.ctrs, P.engine, P.outs, 0, npairs);
   if (e != hipSuccess) return e;
-  e = tl->dilate(dim3(grid), s, P.descs, P.ctrs, P.engine, P.q0, P.q1, INT_MAX,
-                 std::max(1, chunk), spin, P.qstats, qstats_len);
-  if (e != hipSuccess) return e;
-  return hdm_launch_finalize(s, P.descs, P.ctrs, P.engine, P.outs, 0, npairs);
+  return mark(4);
 }
 
 GroupPtrs carve_common(char *base, const CommonLayout &L) {
@@ -138,8 +183,7 @@ GroupPtrs carve_common(char *base, const CommonLayout &L) {
   P.ctrs = (PairCtr *)(base + L.ctrs);
   P.outs = (PairOut *)(base + L.outs);
   P.engine = (Engine *)(base + L.engine);
-  P.q0 = (uint64_t *)(base + L.q0);
-  P.q1 = (uint64_t *)(base + L.q1);
+  P.q = (uint64_t *)(base + L.q);
   P.qstats = (uint32_t *)(base + L.qstats);
   return P;
 }
@@ -227,7 +271,9 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   int rc = get_ctx(&c);
   if (rc != HDM_OK) return rc;
   std::lock_guard<std::mutex> lk(c->mu);
-  hipStream_t s = opts.stream ? (hipStream_t)opts.stream : c->stream;
+  // NULL = the legacy default stream, so the call is ordered after work the
+  // caller queued on it or on any blocking stream (torch's default stream)
+  hipStream_t s = (hipStream_t)opts.stream;
   const int tile = norm_tile(opts.tile);
   const TileLaunch *tl = hdm_tile_launch(tile);
   const TileShape ts = tile_shape(tile);
@@ -262,13 +308,14 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     }
   }
   const PackIndex pi{0, 0};
-  const int grid = resident_grid(c, tile, opts.grid_ctas);
-  hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, ts.rows(), qcap,
-                               grid, opts.chunk > 0 ? opts.chunk : 1, spin_ticks(c), g.hist_len);
+  LaunchCfg cfg = launch_cfg(c, tile, g.row_words, g.ntiles, opts.grid_ctas, opts.chunk);
+  if (opts.timing) cfg.ev = c->ev;
+  hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, qcap, cfg,
+                               g.hist_len);
   if (e != hipSuccess) return map_hip(e);
   char *ho = (char *)c->host_out;
   if (hipMemcpyAsync(ho, P.outs, sizeof(PairOut), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemcpyAsync(ho + 256, P.engine, sizeof(Engine), hipMemcpyDeviceToHost, s) != hipSuccess)
+      hipMemcpyAsync(ho + 256, P.engine, kEngineHostBytes, hipMemcpyDeviceToHost, s) != hipSuccess)
     return HDM_E_HIP;
   if (map_out && opts.ptr_kind == HDM_PTR_HOST &&
       hipMemcpyAsync(map_out, d.map, nvox * 2, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -281,10 +328,20 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
       return HDM_E_HIP;
   }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip(e);
+  if (opts.timing) {
+    float ms[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; ++i)
+      if (hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]) != hipSuccess) ms[i] = -1.f;
+    float all = 0.f;
+    if (hipEventElapsedTime(&all, c->ev[0], c->ev[4]) != hipSuccess) all = -1.f;
+    for (int i = 0; i < 4; ++i) c->last_ms[i] = ms[i];
+    c->last_ms[4] = all;
+    (void)hipGetLastError();
+  }
   PairOut o;
-  Engine es;
+  Engine es{};
   std::memcpy(&o, ho, sizeof(o));
-  std::memcpy(&es, ho + 256, sizeof(es));
+  std::memcpy(&es, ho + 256, kEngineHostBytes);
   fill_result(out, o, &es, g.ntiles, (int)es.t);
   return o.status;
 }
@@ -316,10 +373,11 @@ extern "C" int hdm_first_hit_map_u8(const uint8_t *gold, const uint8_t *segm, in
 struct hdm_batch {
   Ctx *ctx = nullptr;
   hipStream_t s = nullptr;
-  int n = 0, tile = 0, grid = 0, chunk = 1;
+  int n = 0, tile = 0, grid = 0;
   const TileLaunch *tl = nullptr;
   TileShape ts{};
-  uint64_t spin = 0, total_rw = 0, total_tiles = 0;
+  LaunchCfg cfg{};
+  uint64_t total_rw = 0, total_tiles = 0;
   uint32_t qcap = 0;
   int qstats_len = 0;
   void *ws = nullptr;
@@ -335,8 +393,7 @@ struct hdm_batch {
 
 static int batch_enqueue(hdm_batch *b) {
   return map_hip(enqueue_group(b->s, b->tl, b->P, b->n, nullptr, nullptr, b->total_rw,
-                               b->total_tiles, b->ts.rows(), b->qcap, b->grid, b->chunk, b->spin,
-                               b->qstats_len));
+                               b->total_tiles, b->qcap, b->cfg, b->qstats_len));
 }
 
 extern "C" void hdm_batch_destroy(hdm_batch *b) {
@@ -379,8 +436,6 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   b->tl = hdm_tile_launch(b->tile);
   b->ts = tile_shape(b->tile);
   b->grid = resident_grid(c, b->tile, opts.grid_ctas);
-  b->chunk = opts.chunk > 0 ? opts.chunk : 1;
-  b->spin = spin_ticks(c);
   std::vector<PairDesc> descs(n);
   std::vector<PackIndex> idx(n);
   uint64_t pair_bytes = 0, qcap = 0;
@@ -402,6 +457,8 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   }
   b->qcap = (uint32_t)qcap;
   b->qstats_len = maxh;
+  b->cfg = launch_cfg(c, b->tile, b->total_rw, b->total_tiles, opts.grid_ctas, opts.chunk);
+  b->grid = b->cfg.grid;
   const CommonLayout L = common_layout(n, qcap, maxh);
   if (hipMalloc(&b->ws, L.total + pair_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -418,7 +475,9 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
     carve_pair(descs[i], b->geoms[i], pb);
     pb += b->geoms[i].total;
   }
-  if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess ||
+  // a blocking stream: a run is ordered after work on the legacy default stream
+  // (where callers such as torch usually produced the input volumes)
+  if (hipStreamCreate(&b->s) != hipSuccess ||
       hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess ||
       hipHostMalloc(&b->host, sizeof(PairOut) * n + 256, hipHostMallocDefault) != hipSuccess ||
       hipMemcpy(b->P.descs, descs.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice) !=
@@ -471,11 +530,11 @@ extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
   if (!out) return HDM_OK;
   if (hipMemcpy(b->host, b->P.outs, sizeof(PairOut) * b->n, hipMemcpyDeviceToHost) !=
           hipSuccess ||
-      hipMemcpy((char *)b->host + sizeof(PairOut) * b->n, b->P.engine, sizeof(Engine),
+      hipMemcpy((char *)b->host + sizeof(PairOut) * b->n, b->P.engine, kEngineHostBytes,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return HDM_E_HIP;
-  Engine es;
-  std::memcpy(&es, (char *)b->host + sizeof(PairOut) * b->n, sizeof(es));
+  Engine es{};
+  std::memcpy(&es, (char *)b->host + sizeof(PairOut) * b->n, kEngineHostBytes);
   int rc = HDM_OK;
   for (int i = 0; i < b->n; ++i) {
     PairOut o;
@@ -560,4 +619,14 @@ extern "C" int hdm_autotune(const hdm_pair *pairs, int n, const int32_t *tiles, 
   });
   *n_out = k;
   return HDM_OK;
+}
+
+extern "C" int hdm_last_timings(float *ms, int n) {
+  if (!ms || n < 1) return 0;
+  Ctx *c = nullptr;
+  if (get_ctx(&c) != HDM_OK) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  const int k = std::min(n, 5);
+  for (int i = 0; i < k; ++i) ms[i] = c->last_ms[i];
+  return k;
 }

@@ -1,17 +1,25 @@
 // HIP kernels of the Hausdorff dilation engine, written for gfx950 (CDNA4,
-// wave64).  Integer bit logic only; the path is bound by memory traffic.
+// wave64).  Integer bit logic only; the path is bound by memory traffic and
+// by the latency of the per-iteration grid barrier.
 //
-//   reset_kernel     clears per-pair counters and first-hit histograms
-//   pack_kernel      dense labels -> packed tile-major bits + tile bounding box
+//   reset_kernel     clears per-pair counters, engine state, histograms
+//   pack_kernel      dense labels -> packed tile-major bits, partial voxel
+//                    counts and the tile bounding box, in one read of the input
 //                    (replaces getMinMaxes and the packing half of
-//                    boolPrepareKernel, hello.cu:396-526, 537-748, with one read)
-//   meta_kernel      per-tile counts, full/empty/active class, face words and
-//                    the initial work queue (boolPrepareKernel metadata and
+//                    boolPrepareKernel, hello.cu:396-526, 537-748)
+//   totals_kernel    folds the partial counts into |gold|, |segm|, |FP|, |FN|
+//   meta_kernel      per-tile full/empty class, face words and the initial
+//                    work queue (boolPrepareKernel metadata and
 //                    firstMetaPrepareKernel, hello.cu:537-748, 762-877)
 //   dilate_kernel    persistent kernel: one grid barrier per iteration, a
 //                    queue of the tiles still changing (mainPassKernel,
 //                    hello.cu:1066-1896)
 //   finalize_kernel  HD, sums and counts per pair from the histograms
+//
+// Same-address atomics serialize (tens of ns each on MI355X), so every hot
+// counter is spread: partial voxel counts over kSumSlots, per-iteration hits
+// over kHitSlots after aggregation in LDS, queue flushes over kSubQ
+// sub-queues, and the grid barrier uses one arrival flag per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -24,6 +32,16 @@ namespace hdm {
 
 template <int TXB>
 using RowT = std::conditional_t<TXB == 64, uint64_t, uint32_t>;
+
+// agent-coherent load / store (global_load / global_store with sc1)
+template <typename T>
+__device__ __forceinline__ T ld_sc(const T *p) {
+  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_sc(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT);
+}
 
 __device__ __forceinline__ uint32_t shfl32(uint32_t v, int src) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
@@ -58,6 +76,17 @@ __device__ __forceinline__ uint32_t group_sum(uint32_t v, int lane) {
   for (int off = R / 2; off >= 1; off >>= 1) v += shfl32(v, lane ^ off);
   return v;
 }
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v, int lane) { return group_sum<64>(v, lane); }
+__device__ __forceinline__ int wave_min(int v, int lane) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = min(v, (int)shfl32((uint32_t)v, lane ^ off));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v, int lane) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, (int)shfl32((uint32_t)v, lane ^ off));
+  return v;
+}
 
 // bits of the R-lane group holding `lane` out of a wave ballot, shifted to bit 0
 template <int R>
@@ -69,7 +98,7 @@ __device__ __forceinline__ uint64_t group_bits(uint64_t ballot, int lane) {
 }
 
 template <typename Row>
-__device__ __forceinline__ Row valid_bits(int tx, int wxn, int nx) {
+__device__ __forceinline__ Row valid_bits(int tx, int nx) {
   constexpr int TXB = sizeof(Row) * 8;
   const int rem = nx - tx * TXB;
   if (rem >= TXB) return ~Row(0);
@@ -90,6 +119,11 @@ __device__ __forceinline__ int find_pair(int npairs, uint64_t g, F base_of) {
   return lo;
 }
 
+__device__ __forceinline__ uint64_t *sub_queue(uint64_t *q, uint32_t qcap, uint32_t parity,
+                                               uint32_t k) {
+  return q + ((uint64_t)(parity * kSubQ + k)) * qcap;
+}
+
 // ---------------------------------------------------------------- reset
 __global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_idx,
                              PackIndex *idx, int write_single, PairCtr *ctrs, int p0, int npairs,
@@ -100,15 +134,19 @@ __global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_
     descs[p0] = single;
     idx[p0] = single_idx;
   }
-  if (reset_engine && gid == 0) {
-    es->t = 0;
-    es->qlen[0] = es->qlen[1] = es->qlen[2] = 0;
-    es->bar_count = 0;
-    es->bar_gen = 0;
-    es->error = 0;
-    es->qmax = 0;
-    es->qtotal = 0;
-    es->qcap = qcap;
+  if (reset_engine) {
+    if (gid == 0) {
+      es->t = 0;
+      es->error = 0;
+      es->qmax = 0;
+      es->qtotal = 0;
+      es->qcap = qcap;
+    }
+    for (int i = gid; i < kLineWords; i += stride) {
+      es->gen[i] = 0;
+      es->qlen[0][i] = es->qlen[1][i] = es->qlen[2][i] = 0;
+    }
+    for (int i = gid; i < kMaxGrid; i += stride) es->arrive[i] = 0;
   }
   for (int p = p0 + gid; p < p0 + npairs; p += stride) {
     PairCtr &c = ctrs[p];
@@ -117,6 +155,11 @@ __global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_
     c.lo[0] = c.lo[1] = c.lo[2] = 0x7FFFFFFF;
     c.hi[0] = c.hi[1] = c.hi[2] = -1;
     c.status = 0;
+    c.done_from[0] = c.done_from[1] = 0x7FFFFFFF;
+    for (int m = 0; m < 2; ++m)
+      for (int s = 0; s < kHitSlots; ++s) c.hits[m][s] = 0;
+    for (int f = 0; f < 4; ++f)
+      for (int s = 0; s < kSumSlots; ++s) c.sums[f][s] = 0;
   }
 }
 
@@ -130,6 +173,13 @@ __global__ void clear_hist_kernel(const PairDesc *__restrict__ descs, int p0, in
       d.hist[1][i] = 0;
     }
   }
+}
+
+// Pass m of a pair has nothing left to do at pair-local iteration t: its cut
+// is zero, or the cut was reached by the hits of an iteration before t.
+__device__ __forceinline__ bool pass_done(const PairDesc &d, const PairCtr &c, int m, int t) {
+  if (robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn) == 0) return true;
+  return ld_sc(&c.done_from[m]) <= t;
 }
 
 // ---------------------------------------------------------------- pack
@@ -150,6 +200,7 @@ __device__ __forceinline__ uint32_t eq_bits_u8x16(u32x4 v, uint32_t lab4) {
   return out;
 }
 
+// The label compare of hello.cu:453 / 615 for TXB consecutive voxels of a row.
 template <typename Row>
 __device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol, uint64_t base,
                                              int count) {
@@ -187,6 +238,42 @@ __device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol,
   return bits;
 }
 
+// Per-wave accumulator of the pack statistics of one pair.
+struct PackAcc {
+  uint32_t c[4];  // |gold|, |segm|, |FP|, |FN| bits seen by this lane
+  int mn[3], mx[3];
+};
+__device__ __forceinline__ void acc_clear(PackAcc &a) {
+  a.c[0] = a.c[1] = a.c[2] = a.c[3] = 0;
+  a.mn[0] = a.mn[1] = a.mn[2] = 0x7FFFFFFF;
+  a.mx[0] = a.mx[1] = a.mx[2] = -1;
+}
+__device__ __forceinline__ void acc_publish(PairCtr &c, const PackAcc &a, int slot) {
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+    if (a.c[f]) atomicAdd(&c.sums[f][slot], (unsigned long long)a.c[f]);
+  if (a.mx[0] >= 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {  // publish only improvements: most waves skip
+      if (a.mn[k] < ld_sc(&c.lo[k])) atomicMin(&c.lo[k], a.mn[k]);
+      if (a.mx[k] > ld_sc(&c.hi[k])) atomicMax(&c.hi[k], a.mx[k]);
+    }
+  }
+}
+// wave-reduce the accumulator, lane 0 publishes
+__device__ __forceinline__ void acc_flush(PairCtr &c, PackAcc &a, int lane, int slot) {
+  PackAcc r;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) r.c[f] = wave_sum(a.c[f], lane);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) r.mn[k] = wave_min(a.mn[k], lane), r.mx[k] = wave_max(a.mx[k], lane);
+  if (lane == 0) acc_publish(c, r, slot);
+  acc_clear(a);
+}
+
+// One lane per TXB-voxel row word, waves walk contiguous word ranges so the
+// pair index changes rarely; statistics accumulate in registers and are
+// published once per (wave, pair) into spread slots.
 template <int TXB, int TY, int TZ>
 __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
@@ -194,25 +281,39 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
                                                           uint64_t total_words) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
-  const uint64_t g = (uint64_t)blockIdx.x * kWgThreads + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const bool in = g < total_words;
-  int p = 0;
-  if (in && npairs > 1) p = find_pair(npairs, g, [&](int i) { return idx[i].rw_base; });
-  int tx = 0x7FFFFFFF, ty = 0x7FFFFFFF, tz = 0x7FFFFFFF;
-  bool nonempty = false;
-  if (in) {
+  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+  const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
+  const uint64_t per = ((total_words + nw - 1) / nw + 63) & ~uint64_t(63);
+  const uint64_t w0 = gw * per, w1 = min(total_words, w0 + per);
+  const int slot = (int)(gw % kSumSlots);
+  PackAcc acc;
+  acc_clear(acc);
+  int ap = -1;  // pair the accumulator belongs to (wave-uniform)
+  for (uint64_t base = w0; base < w1; base += 64) {
+    const uint64_t g = base + lane;
+    const bool in = g < w1;
+    int p = 0;
+    if (npairs > 1) p = find_pair(npairs, in ? g : w1 - 1, [&](int i) { return idx[i].rw_base; });
+    const int lead = __builtin_amdgcn_readfirstlane(p);
+    if (ap != lead) {
+      if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+      ap = lead;
+    }
+    if (!in) continue;
     const PairDesc &d = descs[p];
     const uint64_t w = g - idx[p].rw_base;
     const int wx = (int)(w % (uint64_t)d.wxn);
     const uint64_t rest = w / (uint64_t)d.wxn;
     const int y = (int)(rest % (uint64_t)d.ny);
     const int z = (int)(rest / (uint64_t)d.ny);
-    const uint64_t vbase = (uint64_t)wx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+    const uint64_t vbase =
+        (uint64_t)wx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
     const int count = min(TXB, d.nx - wx * TXB);
     const Row a = load_row_bits<Row>(d, d.gold, vbase, count);
     const Row b = load_row_bits<Row>(d, d.segm, vbase, count);
-    const uint32_t tile = (uint32_t)wx + (uint32_t)d.gx * ((uint32_t)(y / TY) + (uint32_t)d.gy * (uint32_t)(z / TZ));
+    const uint32_t tile =
+        (uint32_t)wx + (uint32_t)d.gx * ((uint32_t)(y / TY) + (uint32_t)d.gy * (uint32_t)(z / TZ));
     const uint64_t o = (uint64_t)tile * R + (uint64_t)(y % TY) + (uint64_t)TY * (z % TZ);
     ((Row *)d.orig[0])[o] = a;
     ((Row *)d.orig[1])[o] = b;
@@ -224,39 +325,48 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
       uint16_t *mp = d.map + vbase;
       for (int k = 0; k < count; ++k) mp[k] = 0;
     }
-    if (a | b) {
-      nonempty = true;
-      tx = wx, ty = y / TY, tz = z / TZ;
-    }
-  }
-  // bounding box of non-empty tiles: wave min/max, then one atomic per value
-  // that improves on what is already published (most waves publish nothing)
-  const int p_first = __builtin_amdgcn_readfirstlane(p);
-  const bool uniform = __ballot(in && p != p_first) == 0ull;
-  if (uniform) {
-    int mn[3] = {tx, ty, tz};
-    int mx[3] = {nonempty ? tx : -1, nonempty ? ty : -1, nonempty ? tz : -1};
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        mn[k] = min(mn[k], (int)shfl32((uint32_t)mn[k], lane ^ off));
-        mx[k] = max(mx[k], (int)shfl32((uint32_t)mx[k], lane ^ off));
+    // counts of hello.cu:624-625 plus the non-empty tile box (hello.cu:396-526)
+    const uint32_t c0 = popc_row(a), c1 = popc_row(b);
+    const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
+    const bool ne = (a | b) != 0;
+    const int ty_ = y / TY, tz_ = z / TZ;
+    if (p == ap) {
+      acc.c[0] += c0, acc.c[1] += c1, acc.c[2] += c2, acc.c[3] += c3;
+      if (ne) {
+        acc.mn[0] = min(acc.mn[0], wx), acc.mx[0] = max(acc.mx[0], wx);
+        acc.mn[1] = min(acc.mn[1], ty_), acc.mx[1] = max(acc.mx[1], ty_);
+        acc.mn[2] = min(acc.mn[2], tz_), acc.mx[2] = max(acc.mx[2], tz_);
+      }
+    } else {  // a lane past a pair boundary publishes its own word
+      PairCtr &cp = ctrs[p];
+      if (c0) atomicAdd(&cp.sums[0][slot], (unsigned long long)c0);
+      if (c1) atomicAdd(&cp.sums[1][slot], (unsigned long long)c1);
+      if (c2) atomicAdd(&cp.sums[2][slot], (unsigned long long)c2);
+      if (c3) atomicAdd(&cp.sums[3][slot], (unsigned long long)c3);
+      if (ne) {
+        atomicMin(&cp.lo[0], wx), atomicMax(&cp.hi[0], wx);
+        atomicMin(&cp.lo[1], ty_), atomicMax(&cp.hi[1], ty_);
+        atomicMin(&cp.lo[2], tz_), atomicMax(&cp.hi[2], tz_);
       }
     }
-    if (lane == 0 && mx[0] >= 0) {
-      PairCtr &c = ctrs[p_first];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        if (mn[k] < __hip_atomic_load(&c.lo[k], __ATOMIC_RELAXED, AGENT)) atomicMin(&c.lo[k], mn[k]);
-        if (mx[k] > __hip_atomic_load(&c.hi[k], __ATOMIC_RELAXED, AGENT)) atomicMax(&c.hi[k], mx[k]);
-      }
-    }
-  } else if (nonempty) {
-    PairCtr &c = ctrs[p];
-    atomicMin(&c.lo[0], tx), atomicMin(&c.lo[1], ty), atomicMin(&c.lo[2], tz);
-    atomicMax(&c.hi[0], tx), atomicMax(&c.hi[1], ty), atomicMax(&c.hi[2], tz);
   }
+  if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+}
+
+// |gold|, |segm|, |FP|, |FN| per pair from the spread partials (one wave per pair)
+__global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
+  const int p = p0 + blockIdx.x;
+  if (p >= p0 + npairs) return;
+  const int lane = threadIdx.x;
+  PairCtr &c = ctrs[p];
+  unsigned long long v[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    v[f] = c.sums[f][lane];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+  }
+  if (lane == 0) c.n_gold = v[0], c.n_segm = v[1], c.n_fp = v[2], c.n_fn = v[3];
 }
 
 // ---------------------------------------------------------------- helpers
@@ -289,25 +399,29 @@ __device__ __forceinline__ uint32_t neighbour(const PairDesc &d, const Box &b, u
   }
 }
 
-// Wave-aggregated push of one entry per lane with `want` set: reserve in the
-// workgroup's LDS queue, or straight in the global queue when LDS is full.
-__device__ __forceinline__ void push_entries(bool want, uint64_t ent, uint64_t *lq,
-                                             uint32_t *lq_n, uint64_t *gq, uint32_t *gq_len,
-                                             int lane) {
+// Workgroup-local staging of queue pushes: LDS, flushed once per iteration
+// into sub-queue blockIdx % kSubQ; straight to that sub-queue when LDS is full.
+struct LocalQueue {
+  uint64_t e[kLocalQueue];
+  uint32_t n, base;
+};
+
+__device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue &lq,
+                                             uint64_t *gq, uint32_t *gq_len, int lane) {
   const uint64_t m = __ballot(want);
   if (m == 0) return;
   const uint32_t n = (uint32_t)__popcll(m);
   uint32_t base = 0;
   bool global = false;
   if (lane == 0) {  // reserve only if the whole run fits, so LDS stays hole-free
-    uint32_t cur = *lq_n;
+    uint32_t cur = lq.n;
     for (;;) {
       if (cur + n > (uint32_t)kLocalQueue) {
         base = atomicAdd(gq_len, n);
         global = true;
         break;
       }
-      const uint32_t prev = atomicCAS(lq_n, cur, cur + n);
+      const uint32_t prev = atomicCAS(&lq.n, cur, cur + n);
       if (prev == cur) {
         base = cur;
         break;
@@ -320,24 +434,23 @@ __device__ __forceinline__ void push_entries(bool want, uint64_t ent, uint64_t *
   if (want) {
     const uint32_t pos = base + lanes_below(m);
     if (global)
-      gq[pos] = ent;
+      st_sc(&gq[pos], ent);
     else
-      lq[pos] = ent;
+      lq.e[pos] = ent;
   }
 }
 
-__device__ __forceinline__ void flush_local_queue(uint64_t *lq, uint32_t *lq_n, uint32_t *lq_base,
-                                                  uint64_t *gq, uint32_t *gq_len) {
+__device__ __forceinline__ void flush_local_queue(LocalQueue &lq, uint64_t *gq, uint32_t *gq_len) {
   __syncthreads();
-  const uint32_t n = *lq_n;
+  const uint32_t n = lq.n;
   if (n != 0) {
-    if (threadIdx.x == 0) *lq_base = atomicAdd(gq_len, n);
+    if (threadIdx.x == 0) lq.base = atomicAdd(gq_len, n);
     __syncthreads();
-    const uint32_t b = *lq_base;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) gq[b + i] = lq[i];
+    const uint32_t b = lq.base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) st_sc(&gq[b + i], lq.e[i]);
   }
   __syncthreads();
-  if (threadIdx.x == 0) *lq_n = 0;
+  if (threadIdx.x == 0) lq.n = 0;
 }
 
 // ---------------------------------------------------------------- meta
@@ -345,138 +458,192 @@ template <int TXB, int TY, int TZ>
 __global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs, Engine *es,
-                                                          uint64_t *q0, uint64_t *q1,
+                                                          uint64_t *q, uint32_t qcap,
                                                           uint64_t total_tiles, int p0) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
-  constexpr int TPW = kWave / R;
-  __shared__ uint64_t lq[kLocalQueue];
-  __shared__ uint32_t lq_n, lq_base;
-  if (threadIdx.x == 0) lq_n = 0;
+  constexpr int TPB = kWgThreads / R;  // tiles per workgroup step
+  __shared__ LocalQueue lq;
+  if (threadIdx.x == 0) lq.n = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int r = lane % R;
   const int yl = r % TY, zl = r / TY;
-  const uint32_t t = es->t;
-  uint64_t *gq = (t & 1) ? q1 : q0;
-  uint32_t *gq_len = &es->qlen[t % 3];
-  const uint64_t gtile = ((uint64_t)blockIdx.x * kWgThreads + threadIdx.x) / R;
-  const bool in = gtile < total_tiles;
-  int p = 0;
-  if (in && npairs > 1) p = find_pair(npairs, gtile, [&](int i) { return idx[i].tile_base; });
-  const PairDesc &d = descs[p];
-  const uint32_t tile = in ? (uint32_t)(gtile - idx[p].tile_base) : 0;
-  const int tx = (int)(tile % (uint32_t)d.gx);
-  const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
-  const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
-  const int y = ty * TY + yl, z = tz * TZ + zl;
-  const bool rowok = in && y < d.ny && z < d.nz;
-  const Row vb = valid_bits<Row>(tx, d.wxn, d.nx);
-  Row w[2] = {0, 0};
-  if (rowok) {
-    w[0] = ((const Row *)d.orig[0])[(uint64_t)tile * R + r];
-    w[1] = ((const Row *)d.orig[1])[(uint64_t)tile * R + r];
-  }
-  // counts: |gold|, |segm|, |FP| = |segm & ~gold|, |FN| = |gold & ~segm| (hello.cu:624-625)
-  uint32_t cg = group_sum<R>((uint32_t)popc_row(w[0]), lane);
-  uint32_t cs = group_sum<R>((uint32_t)popc_row(w[1]), lane);
-  uint32_t cfp = group_sum<R>((uint32_t)popc_row((Row)(w[1] & ~w[0])), lane);
-  uint32_t cfn = group_sum<R>((uint32_t)popc_row((Row)(w[0] & ~w[1])), lane);
-  PairCtr &c = ctrs[p];
-  if (in && r == 0) {
-    if (cg) atomicAdd(&c.n_gold, (unsigned long long)cg);
-    if (cs) atomicAdd(&c.n_segm, (unsigned long long)cs);
-    if (cfp) atomicAdd(&c.n_fp, (unsigned long long)cfp);
-    if (cfn) atomicAdd(&c.n_fn, (unsigned long long)cfn);
-  }
-  const Box b = pair_box(d, c);
-  const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] && ty <= b.hi[1] &&
-                      tz >= b.lo[2] && tz <= b.hi[2];
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    const uint64_t fxl = group_bits<R>(__ballot(rowok && (w[m] & 1)), lane);
-    const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w[m] >> (TXB - 1)) & 1)), lane);
-    const bool any = group_bits<R>(__ballot(w[m] != 0), lane) != 0;
-    const bool full = group_bits<R>(__ballot(rowok && w[m] != vb), lane) == 0;
-    if (in && r == 0) {
-      d.xf_orig[m][2 * (uint64_t)tile] = fxl;
-      d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
+  const uint32_t t0 = es->t;
+  const uint32_t kq = blockIdx.x % kSubQ;
+  uint64_t *gq = sub_queue(q, qcap, t0 & 1, kq);
+  uint32_t *gq_len = &es->qlen[t0 % 3][kq];
+  for (uint64_t tb = (uint64_t)blockIdx.x * TPB; tb < total_tiles; tb += (uint64_t)gridDim.x * TPB) {
+    const uint64_t gtile = tb + threadIdx.x / R;
+    const bool in = gtile < total_tiles;
+    int p = 0;
+    if (npairs > 1)
+      p = find_pair(npairs, in ? gtile : total_tiles - 1, [&](int i) { return idx[i].tile_base; });
+    const PairDesc &d = descs[p];
+    const PairCtr &c = ctrs[p];
+    const uint32_t tile = in ? (uint32_t)(gtile - idx[p].tile_base) : 0;
+    const int tx = (int)(tile % (uint32_t)d.gx);
+    const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+    const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+    const int y = ty * TY + yl, z = tz * TZ + zl;
+    const bool rowok = in && y < d.ny && z < d.nz;
+    const Row vb = valid_bits<Row>(tx, d.nx);
+    Row w[2] = {0, 0};
+    if (rowok) {
+      w[0] = ((const Row *)d.orig[0])[(uint64_t)tile * R + r];
+      w[1] = ((const Row *)d.orig[1])[(uint64_t)tile * R + r];
     }
-    const bool fyl = group_bits<R>(__ballot(rowok && yl == 0 && w[m] != 0), lane) != 0;
-    const bool fyh = group_bits<R>(__ballot(rowok && yl == TY - 1 && w[m] != 0), lane) != 0;
-    const bool fzl = group_bits<R>(__ballot(rowok && zl == 0 && w[m] != 0), lane) != 0;
-    const bool fzh = group_bits<R>(__ballot(rowok && zl == TZ - 1 && w[m] != 0), lane) != 0;
-    // lane 6: the tile itself; lanes 0..5: its face neighbours
-    bool want = false;
-    uint64_t ent = 0;
-    if (in_box && any) {
-      if (r == 6) {
-        const uint32_t nv = full ? kFull : 1u;
-        if (atomicCAS(&d.state[m][tile], 0u, nv) == 0u && !full) {
-          want = true;
-          ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
-        }
-      } else if (r < 6) {
-        const bool face = r == 0 ? fxl != 0 : r == 1 ? fxh != 0 : r == 2 ? fyl : r == 3 ? fyh
-                                                                  : r == 4 ? fzl : fzh;
-        const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, r);
-        if (face && nb != kNoTile && atomicCAS(&d.state[m][nb], 0u, 1u) == 0u) {
-          want = true;
-          ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+    const Box b = pair_box(d, c);
+    const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
+                        ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const uint64_t fxl = group_bits<R>(__ballot(rowok && (w[m] & 1)), lane);
+      const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w[m] >> (TXB - 1)) & 1)), lane);
+      const bool any = group_bits<R>(__ballot(w[m] != 0), lane) != 0;
+      const bool full = group_bits<R>(__ballot(rowok && w[m] != vb), lane) == 0;
+      if (in && r == 0) {
+        d.xf_orig[m][2 * (uint64_t)tile] = fxl;
+        d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
+      }
+      const bool fyl = group_bits<R>(__ballot(rowok && yl == 0 && w[m] != 0), lane) != 0;
+      const bool fyh = group_bits<R>(__ballot(rowok && yl == TY - 1 && w[m] != 0), lane) != 0;
+      const bool fzl = group_bits<R>(__ballot(rowok && zl == 0 && w[m] != 0), lane) != 0;
+      const bool fzh = group_bits<R>(__ballot(rowok && zl == TZ - 1 && w[m] != 0), lane) != 0;
+      // a pass whose cut is zero queues nothing
+      const bool needed = robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn) != 0;
+      // lane 6: the tile itself; lanes 0..5: its face neighbours (hello.cu:1649-1663)
+      bool want = false;
+      uint64_t ent = 0;
+      if (in_box && any) {
+        if (r == 6) {
+          const uint32_t nv = full ? kFull : 1u;
+          if (atomicCAS(&d.state[m][tile], 0u, nv) == 0u && !full && needed) {
+            want = true;
+            ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+          }
+        } else if (r < 6 && needed) {
+          const bool face = r == 0 ? fxl != 0 : r == 1 ? fxh != 0 : r == 2 ? fyl : r == 3 ? fyh
+                                                                    : r == 4 ? fzl : fzh;
+          const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, r);
+          if (face && nb != kNoTile && atomicCAS(&d.state[m][nb], 0u, 1u) == 0u) {
+            want = true;
+            ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+          }
         }
       }
+      push_entries(want, ent, lq, gq, gq_len, lane);
     }
-    push_entries(want, ent, lq, &lq_n, gq, gq_len, lane);
   }
-  (void)TPW;
-  flush_local_queue(lq, &lq_n, &lq_base, gq, gq_len);
+  flush_local_queue(lq, gq, gq_len);
 }
 
 // ---------------------------------------------------------------- barrier
-// Sense-reversing grid barrier on an occupancy-bounded grid.  The spin is
-// bounded by wall-clock ticks: on expiry the engine error flag is set and the
-// caller leaves its loop, so no wave can wait forever.
-__device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t nblocks, uint64_t limit,
-                                             int *s_ok) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+// Grid barrier on an occupancy-bounded grid (every workgroup resident).
+// Arrival: each wave drains its own memory operations (s_waitcnt 0: its
+// stores, all agent-coherent, are then performed), then the workgroup stores
+// its epoch into its own flag.  Workgroup 0 waits for every flag, folds the
+// iteration's hit counters (fold), then publishes the epoch; the others wait
+// for it.  Waits are bounded by wall-clock ticks: on expiry the engine error
+// flag is set and the caller leaves its loop, so no wave waits forever.
+// With fence set, the arrival and release also write back / invalidate L2.
+template <typename Fold>
+__device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_t limit,
+                                             bool fence, int *s_ok, Fold fold) {
+  if (fence) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
+    *s_ok = 1;
+    st_sc(&es->arrive[blockIdx.x], epoch);
+  }
+  if (blockIdx.x == 0) {
+    __syncthreads();
     int ok = 1;
-    const uint32_t gen = __hip_atomic_load(&es->bar_gen, __ATOMIC_RELAXED, AGENT);
-    const uint32_t arrived = __hip_atomic_fetch_add(&es->bar_count, 1u, __ATOMIC_ACQ_REL, AGENT);
-    if (arrived == nblocks - 1) {
-      __hip_atomic_store(&es->bar_count, 0u, __ATOMIC_RELAXED, AGENT);
-      __hip_atomic_store(&es->bar_gen, gen + 1, __ATOMIC_RELEASE, AGENT);
-    } else {
-      const uint64_t t0 = wall_clock64();
-      while (__hip_atomic_load(&es->bar_gen, __ATOMIC_ACQUIRE, AGENT) == gen) {
+    const uint64_t t0 = wall_clock64();
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
+      while (ld_sc(&es->arrive[i]) != epoch) {
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
-          __hip_atomic_fetch_or(&es->error, 1u, __ATOMIC_RELAXED, AGENT);
           ok = 0;
           break;
         }
       }
+      if (!ok) break;
     }
-    if (__hip_atomic_load(&es->error, __ATOMIC_RELAXED, AGENT)) ok = 0;
-    *s_ok = ok;
+    if (!ok) {
+      atomicOr(&es->error, 1u);
+      *s_ok = 0;
+    }
+    __syncthreads();
+    if (*s_ok) fold();
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (fence) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      st_sc(&es->gen[0], epoch);
+    }
+  } else if (threadIdx.x == 0) {
+    const uint64_t t0 = wall_clock64();
+    while (ld_sc(&es->gen[0]) != epoch) {
+      __builtin_amdgcn_s_sleep(1);
+      if (wall_clock64() - t0 > limit) {
+        atomicOr(&es->error, 1u);
+        *s_ok = 0;
+        break;
+      }
+    }
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  if (ld_sc(&es->error)) return false;
   return *s_ok != 0;
+}
+
+// LDS table of per-(pair, mask) hit counts of the running iteration
+constexpr int kHitTable = 64;
+struct HitTable {
+  uint32_t key[kHitTable];  // pair * 2 + mask, or ~0u
+  uint32_t cnt[kHitTable];
+};
+__device__ __forceinline__ void hit_add(HitTable &h, PairCtr *ctrs, uint32_t key, uint32_t v) {
+  uint32_t slot = key % kHitTable;
+  for (int probe = 0; probe < 8; ++probe, slot = (slot + 1) % kHitTable) {
+    const uint32_t k = atomicCAS(&h.key[slot], ~0u, key);
+    if (k == ~0u || k == key) {
+      atomicAdd(&h.cnt[slot], v);
+      return;
+    }
+  }
+  atomicAdd(&ctrs[key >> 1].hits[key & 1][blockIdx.x % kHitSlots], v);  // table full
+}
+__device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
+  __syncthreads();
+  if (threadIdx.x < kHitTable) {
+    const uint32_t k = h.key[threadIdx.x];
+    if (k != ~0u) {
+      atomicAdd(&ctrs[k >> 1].hits[k & 1][blockIdx.x % kHitSlots], h.cnt[threadIdx.x]);
+      h.key[threadIdx.x] = ~0u;
+      h.cnt[threadIdx.x] = 0;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- dilate
 template <int TXB, int TY, int TZ>
 __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
-    const PairDesc *__restrict__ descs, PairCtr *ctrs, Engine *es, uint64_t *q0, uint64_t *q1,
-    int max_iters, int chunk, uint64_t spin_limit, uint32_t *qstats, int qstats_len) {
+    const PairDesc *__restrict__ descs, PairCtr *ctrs, int npairs, Engine *es, uint64_t *q,
+    uint32_t qcap, int max_iters, int chunk, uint64_t spin_limit, uint32_t *qstats,
+    int qstats_len, int flags) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;
-  __shared__ uint64_t lq[kLocalQueue];
-  __shared__ uint32_t lq_n, lq_base;
+  __shared__ LocalQueue lq;
+  __shared__ HitTable ht;
+  __shared__ uint32_t s_pre[kSubQ + 1];
   __shared__ int s_ok;
+  const bool fence = (flags & 1) != 0;
+  const bool single = (flags & 4) != 0;  // one pair: stop as soon as both passes are done
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = lane / R;
@@ -486,26 +653,40 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
   const uint32_t gwave = blockIdx.x * kWavesPerWg + wave;
   const uint32_t nwave = gridDim.x * kWavesPerWg;
   const uint32_t step = (uint32_t)chunk * TPW;
-  if (threadIdx.x == 0) lq_n = 0;
+  const uint32_t kq = blockIdx.x % kSubQ;
+  if (threadIdx.x == 0) lq.n = 0;
+  if (threadIdx.x < kHitTable) ht.key[threadIdx.x] = ~0u, ht.cnt[threadIdx.x] = 0;
   __syncthreads();
-  uint32_t t = __hip_atomic_load(&es->t, __ATOMIC_RELAXED, AGENT);
+  uint32_t t = ld_sc(&es->t);
   for (int it = 0; it < max_iters; ++it) {
-    const uint32_t L = __hip_atomic_load(&es->qlen[t % 3], __ATOMIC_RELAXED, AGENT);
-    if (L == 0) break;
-    const uint64_t *Qin = (t & 1) ? q1 : q0;
-    uint64_t *Qout = (t & 1) ? q0 : q1;
-    uint32_t *qout_len = &es->qlen[(t + 1) % 3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      es->qlen[(t + 2) % 3] = 0;
-      es->qtotal += L;
-      es->qmax = max(es->qmax, L);
-      if (qstats && (int)t < qstats_len) qstats[t] = L;
+    // queue of iteration t: sub-queue lengths -> prefix offsets
+    if (threadIdx.x == 0) {
+      uint32_t acc = 0;
+      for (int k = 0; k < kSubQ; ++k) {
+        s_pre[k] = acc;
+        acc += ld_sc(&es->qlen[t % 3][k]);
+      }
+      s_pre[kSubQ] = acc;
     }
+    __syncthreads();
+    const uint32_t L = s_pre[kSubQ];
+    if (L == 0) break;
+    if (single) {
+      const int tp0 = (int)(t - descs[0].t_start);
+      if (pass_done(descs[0], ctrs[0], 0, tp0) && pass_done(descs[0], ctrs[0], 1, tp0)) break;
+    }
+    uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
+    uint32_t *qout_len = &es->qlen[(t + 1) % 3][kq];
     for (uint32_t base = gwave * step; base < L; base += nwave * step) {
       for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
         const uint32_t ei = base + k + sub;
         bool live = ei < L;
-        const uint64_t ent = live ? Qin[ei] : 0;
+        uint64_t ent = 0;
+        if (live) {
+          int sq = 0;
+          while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
+          ent = ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
+        }
         const uint32_t tile = (uint32_t)ent;
         const int p = (int)((ent >> 32) & 0xFFFF);
         const int m = (int)((ent >> 48) & 1);
@@ -513,12 +694,8 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
         const bool check = (ent >> 50) & 1;
         const PairDesc &d = descs[p];
         PairCtr &c = ctrs[p];
-        if (live) {  // per-mask termination (hello.cu:1748-1759)
-          const unsigned long long n = m == 0 ? c.n_fp : c.n_fn;
-          const unsigned long long cov = __hip_atomic_load(&c.cov[m], __ATOMIC_RELAXED, AGENT);
-          if (cov >= robust_cut(d.robust, n)) live = false;
-        }
         const uint32_t tp = t - d.t_start;  // pair-local iteration
+        if (live && pass_done(d, c, m, (int)tp)) live = false;  // hello.cu:1748-1759
         if (live && tp + 1 >= (uint32_t)d.hist_len) {
           c.status = -8;
           live = false;
@@ -529,32 +706,32 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
         const int y = ty * TY + yl, z = tz * TZ + zl;
         const bool exists = y < d.ny && z < d.nz;
         const bool rowok = live && exists;
-        const Row vb = valid_bits<Row>(tx, d.wxn, d.nx);
+        const Row vb = valid_bits<Row>(tx, d.nx);
         const Box b = pair_box(d, c);
         const int rd = (tp + 1) & 1;  // slot holding iteration tp's input
         const int wr = tp & 1;        // slot receiving iteration tp's output
         const Row *orig_m = (const Row *)d.orig[m];
         const Row *bufr = (const Row *)d.buf[m][rd];
-        const Row *src = first ? orig_m : bufr;
         const uint64_t ro = (uint64_t)tile * R + r;
         // -- issue every load of this step before using any of them
-        const Row own = rowok ? src[ro] : Row(0);
+        Row own = 0;
+        if (rowok) own = first ? orig_m[ro] : ld_sc(&bufr[ro]);
         const Row other = (rowok && check) ? ((const Row *)d.orig[1 - m])[ro] : Row(0);
         uint32_t nb_dir = kNoTile, nst = 0;
         if (live && r < 6) {
           nb_dir = neighbour(d, b, tile, tx, ty, tz, r);
-          if (nb_dir != kNoTile) nst = d.state[m][nb_dir];
+          if (nb_dir != kNoTile) nst = ld_sc(&d.state[m][nb_dir]);
         }
         const uint32_t nb_xl = live ? neighbour(d, b, tile, tx, ty, tz, 0) : kNoTile;
         const uint32_t nb_xh = live ? neighbour(d, b, tile, tx, ty, tz, 1) : kNoTile;
         uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
         if (nb_xl != kNoTile) {
           fxl_o = d.xf_orig[m][2 * (uint64_t)nb_xl + 1];
-          fxl_b = d.xf_buf[m][rd][2 * (uint64_t)nb_xl + 1];
+          fxl_b = ld_sc(&d.xf_buf[m][rd][2 * (uint64_t)nb_xl + 1]);
         }
         if (nb_xh != kNoTile) {
           fxh_o = d.xf_orig[m][2 * (uint64_t)nb_xh];
-          fxh_b = d.xf_buf[m][rd][2 * (uint64_t)nb_xh];
+          fxh_b = ld_sc(&d.xf_buf[m][rd][2 * (uint64_t)nb_xh]);
         }
         // y / z face rows from the neighbour tiles, both candidate sources
         uint32_t nb_yz[4] = {kNoTile, kNoTile, kNoTile, kNoTile};
@@ -569,7 +746,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
             if (nb_yz[f] == kNoTile) continue;
             const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
             fo[f] = orig_m[o];
-            fb[f] = bufr[o];
+            fb[f] = ld_sc(&bufr[o]);
           }
         }
         // -- neighbour states -> which candidate holds iteration tp's input
@@ -616,17 +793,15 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
         const uint32_t rem = group_sum<R>((uint32_t)popc_row((Row)(other & ~dl)), lane);
         if (nw && d.map) {
           Row bits = nw;
-          const uint64_t vrow = (uint64_t)tx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+          const uint64_t vrow =
+              (uint64_t)tx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
           while (bits) {
-            const int k = (int)__builtin_ctzll((uint64_t)bits);
+            const int kb = (int)__builtin_ctzll((uint64_t)bits);
             bits &= bits - 1;
-            d.map[vrow + k] = (uint16_t)(tp + 1);
+            d.map[vrow + kb] = (uint16_t)(tp + 1);
           }
         }
-        if (live && r == 0 && cnt) {
-          atomicAdd(&d.hist[m][tp + 1], cnt);
-          atomicAdd(&c.cov[m], (unsigned long long)cnt);
-        }
+        if (live && r == 0 && cnt) hit_add(ht, ctrs, (uint32_t)p * 2 + m, cnt);
         // -- outputs
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
@@ -640,10 +815,10 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
           if (full) {
             if (r == 0) atomicOr(&d.state[m][tile], kFull);
           } else {
-            ((Row *)d.buf[m][wr])[ro] = dl;
+            st_sc(&((Row *)d.buf[m][wr])[ro], dl);
             if (r == 0) {
-              d.xf_buf[m][wr][2 * (uint64_t)tile] = oxl;
-              d.xf_buf[m][wr][2 * (uint64_t)tile + 1] = oxh;
+              st_sc(&d.xf_buf[m][wr][2 * (uint64_t)tile], oxl);
+              st_sc(&d.xf_buf[m][wr][2 * (uint64_t)tile + 1], oxh);
             }
             if (r == 6) {  // stays queued while not full
               want = true;
@@ -659,14 +834,49 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
             }
           }
         }
-        push_entries(want, pent, lq, &lq_n, Qout, qout_len, lane);
+        push_entries(want, pent, lq, Qout, qout_len, lane);
       }
     }
-    flush_local_queue(lq, &lq_n, &lq_base, Qout, qout_len);
+    flush_local_queue(lq, Qout, qout_len);
+    hit_flush(ht, ctrs);
+    const uint32_t tdone = t;  // iteration that just ran
     ++t;
-    if (!grid_barrier(es, gridDim.x, spin_limit, &s_ok)) break;
+    // master bookkeeping between iterations: fold the spread hit counters into
+    // the histogram and the covered counts, decide each pass's stop
+    // (hello.cu:1748-1759), reset the sub-queue counters that iteration t+1 fills
+    auto fold = [&]() {
+      for (int i = threadIdx.x; i < 2 * npairs; i += blockDim.x) {
+        const int pp = i >> 1, mm = i & 1;
+        const PairDesc &dd = descs[pp];
+        if (tdone < dd.t_start) continue;
+        const int tpp = (int)(tdone - dd.t_start);
+        PairCtr &cc = ctrs[pp];
+        uint32_t v = 0;
+#pragma unroll
+        for (int s = 0; s < kHitSlots; ++s) {
+          const uint32_t h = ld_sc(&cc.hits[mm][s]);
+          if (h) {
+            v += h;
+            st_sc(&cc.hits[mm][s], 0u);
+          }
+        }
+        if (v == 0) continue;
+        if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = v;
+        const unsigned long long before = cc.cov[mm];
+        cc.cov[mm] = before + v;
+        const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
+        if (before < cut && before + v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
+      }
+      if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 1) % 3][threadIdx.x], 0u);
+      if (threadIdx.x == 0) {
+        es->qtotal += L;
+        es->qmax = max(es->qmax, L);
+        if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
+      }
+    };
+    if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold)) break;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) __hip_atomic_store(&es->t, t, __ATOMIC_RELAXED, AGENT);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st_sc(&es->t, t);
 }
 
 // ---------------------------------------------------------------- finalize
@@ -741,7 +951,8 @@ __global__ __launch_bounds__(kWgThreads) void finalize_kernel(const PairDesc *__
     s_pref[threadIdx.x] = (unsigned long long)lhd;
     __syncthreads();
     for (int off = kWgThreads / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) s_pref[threadIdx.x] = max(s_pref[threadIdx.x], s_pref[threadIdx.x + off]);
+      if (threadIdx.x < off)
+        s_pref[threadIdx.x] = max(s_pref[threadIdx.x], s_pref[threadIdx.x + off]);
       __syncthreads();
     }
     hd[m] = (int)s_pref[0];
@@ -774,26 +985,26 @@ hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
 }
 template <int TXB, int TY, int TZ>
 hipError_t launch_meta(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
-                       PairCtr *c, int np, Engine *es, uint64_t *q0, uint64_t *q1,
-                       uint64_t tt, int p0) {
+                       PairCtr *c, int np, Engine *es, uint64_t *q, uint32_t qcap, uint64_t tt,
+                       int p0) {
   hipLaunchKernelGGL((meta_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, es,
-                     q0, q1, tt, p0);
+                     q, qcap, tt, p0);
   return hipGetLastError();
 }
 template <int TXB, int TY, int TZ>
-hipError_t launch_dilate(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c, Engine *es,
-                         uint64_t *q0, uint64_t *q1, int max_iters, int chunk, uint64_t spin,
-                         uint32_t *qs, int qsl) {
-  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, c, es, q0,
-                     q1, max_iters, chunk, spin, qs, qsl);
+hipError_t launch_dilate(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c, int np,
+                         Engine *es, uint64_t *q, uint32_t qcap, int max_iters, int chunk,
+                         uint64_t spin, uint32_t *qs, int qsl, int flags) {
+  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, c, np, es, q,
+                     qcap, max_iters, chunk, spin, qs, qsl, flags);
   return hipGetLastError();
 }
 
 template <int TXB, int TY, int TZ>
 TileLaunch make_launch() {
   return TileLaunch{&launch_pack<TXB, TY, TZ>, &launch_meta<TXB, TY, TZ>,
-                    &launch_dilate<TXB, TY, TZ>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ>, TXB, TY, TZ};
+                    &launch_dilate<TXB, TY, TZ>, (const void *)&dilate_kernel<TXB, TY, TZ>,
+                    (const void *)&pack_kernel<TXB, TY, TZ>, TXB, TY, TZ};
 }
 
 }  // namespace hdm
@@ -823,6 +1034,11 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(hdm::clear_hist_kernel, dim3(4, std::min(npairs, 1024)), dim3(256), 0, s,
                      descs, p0, npairs);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs) {
+  hipLaunchKernelGGL(hdm::totals_kernel, dim3(npairs), dim3(64), 0, s, ctrs, p0, npairs);
   return hipGetLastError();
 }
 

@@ -19,10 +19,11 @@ struct TileLaunch {
   hipError_t (*pack)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      uint64_t);
   hipError_t (*meta)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
-                     Engine *, uint64_t *, uint64_t *, uint64_t, int);
-  hipError_t (*dilate)(dim3, hipStream_t, const PairDesc *, PairCtr *, Engine *, uint64_t *,
-                       uint64_t *, int, int, uint64_t, uint32_t *, int);
+                     Engine *, uint64_t *, uint32_t, uint64_t, int);
+  hipError_t (*dilate)(dim3, hipStream_t, const PairDesc *, PairCtr *, int, Engine *, uint64_t *,
+                       uint32_t, int, int, uint64_t, uint32_t *, int, int);
   const void *dilate_fn;
+  const void *pack_fn;
   int txb, ty, tz;
 };
 
@@ -112,8 +113,11 @@ inline void fill_desc(PairDesc &d, const PairGeom &g, const void *gold, const vo
 
 // Shared (per launch group) region: descriptors, indices, counters, outputs,
 // engine state and the two queue arrays.
+// The queue is 2 (parity) x kSubQ sub-queues of qcap entries each: a (tile,
+// mask) is queued at most once per iteration, so any one sub-queue can take the
+// whole iteration.
 struct CommonLayout {
-  uint64_t descs, idx, ctrs, outs, engine, q0, q1, qstats, total;
+  uint64_t descs, idx, ctrs, outs, engine, q, qstats, total;
 };
 
 inline CommonLayout common_layout(int npairs, uint64_t qcap, int qstats_len) {
@@ -129,8 +133,7 @@ inline CommonLayout common_layout(int npairs, uint64_t qcap, int qstats_len) {
   L.ctrs = take(sizeof(PairCtr) * npairs);
   L.outs = take(sizeof(PairOut) * npairs);
   L.engine = take(sizeof(Engine));
-  L.q0 = take(8 * qcap);
-  L.q1 = take(8 * qcap);
+  L.q = take(8 * qcap * 2 * kSubQ);
   L.qstats = take(4 * (uint64_t)qstats_len);
   L.total = o;
   return L;
@@ -144,6 +147,7 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
                                        const hdm::PackIndex *single_idx, hdm::PackIndex *idx,
                                        hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
                                        int reset_engine, uint32_t qcap);
+extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs);
 extern "C" hipError_t hdm_launch_finalize(hipStream_t s, const hdm::PairDesc *descs,
                                           const hdm::PairCtr *ctrs, const hdm::Engine *es,
                                           hdm::PairOut *outs, int p0, int npairs);

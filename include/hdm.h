@@ -17,7 +17,9 @@
  *   - return value 0 = HDM_OK, negative = HDM_E_* (the library never exits
  *     the process and never resets the device).
  *   - device pointers are plain HIP device addresses; streams are hipStream_t
- *     passed as void*.  NULL stream = the library's own per-device stream.
+ *     passed as void*.  NULL stream = the legacy default stream (ordered with
+ *     every blocking stream); the single-pair calls return only after their
+ *     work on that stream is done.
  *
  * Differences from the reference that are deliberate (its quirks are not
  * reproduced; results follow the mathematical definitions above):
@@ -74,7 +76,7 @@ typedef struct hdm_opts {
   int32_t grid_ctas;  /* persistent-kernel workgroups; 0 = occupancy-derived           */
   int32_t chunk;      /* queue entries a wave takes per claim; 0 = default             */
   int32_t crop;       /* 1 (default) = restrict dilation to the tile bounding box      */
-  int32_t reserved0;
+  int32_t timing;     /* 1 = time the phases with HIP events (hdm_last_timings)        */
   void *stream;       /* hipStream_t or NULL                                           */
   uint16_t *map_out;  /* optional nx*ny*nz first-hit map: d(v) on FP u FN, 0 elsewhere */
   uint32_t *hist_out; /* optional 2*hist_len first-hit histograms (FP then FN), host   */
@@ -106,6 +108,11 @@ void hdm_opts_init(hdm_opts *opts);
 int hdm_device_count(void);
 /* Device bytes one pair of the given dims needs (workspace, excluding inputs). */
 uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
+/* Device milliseconds of the phases of the last single-pair call on the
+ * current device that ran with opts->timing = 1, from HIP events on its
+ * stream: ms[0] pack (+ voxel totals), ms[1] tile metadata, ms[2] persistent
+ * dilation, ms[3] finalize, ms[4] all kernels.  Returns how many were written. */
+int hdm_last_timings(float *ms, int n);
 
 /* ---- single pair -------------------------------------------------------
  * Replaces: template<T> int getHausdorffDistance_CUDA_Generic(T* goldStandard,

@@ -50,9 +50,10 @@ def test_device_count_without_gpu_does_not_fail():
 
 def test_workspace_bytes():
     L = hdm.lib()
-    # 512^3 with 64x8x8 tiles: 6 packed copies of 16 MiB plus small metadata
+    # 512^3 with 64x8x8 tiles: 6 packed copies of 16 MiB, 2 x 16 sub-queues of
+    # 2 x 32768 entries (16 MiB) plus small metadata
     b = L.hdm_workspace_bytes(512, 512, 512, 1)
-    assert 96 << 20 <= b <= 104 << 20
+    assert 112 << 20 <= b <= 120 << 20
     assert L.hdm_workspace_bytes(0, 5, 5, 1) == 0
     assert L.hdm_workspace_bytes(67, 13, 9, 4) > 0
 
