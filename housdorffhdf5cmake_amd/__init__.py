@@ -69,7 +69,9 @@ class TuneEntry(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_slots", C.c_int32), ("slice_iters", C.c_int32),
-                ("tile", C.c_int32), ("grid_ctas", C.c_int32), ("robustness", C.c_double)]
+                ("tile", C.c_int32), ("grid_ctas", C.c_int32), ("robustness", C.c_double),
+                ("max_nx", C.c_int32), ("max_ny", C.c_int32), ("max_nz", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Completion(C.Structure):
@@ -107,6 +109,7 @@ def lib() -> C.CDLL:
         "hdm_opts_init": ([C.POINTER(Opts)], None), "hdm_device_count": ([], i32),
         "hdm_workspace_bytes": ([i32, i32, i32, i32], u64),
         "hdm_last_timings": ([C.POINTER(C.c_float), i32], i32),
+        "hdm_last_trace": ([C.POINTER(C.c_uint64), i32, C.POINTER(C.c_double)], i32),
         "hdm_hausdorff_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts),
                               C.POINTER(Result)], i32),
         "hdm_hausdorff_i32": ([vp, vp, i32, i32, i32, C.c_int32, C.POINTER(Opts),
@@ -122,7 +125,8 @@ def lib() -> C.CDLL:
                           C.POINTER(TuneEntry), i32, C.POINTER(C.c_int)], i32),
         "hdm_server_opts_init": ([C.POINTER(ServerOpts)], None),
         "hdm_server_create": ([C.POINTER(ServerOpts), C.POINTER(vp)], i32),
-        "hdm_server_submit": ([vp, vp, vp, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)], i32),
+        "hdm_server_submit": ([vp, vp, vp, i32, i32, i32, i32, i32, i32,
+                               C.POINTER(C.c_uint64)], i32),
         "hdm_server_step": ([vp, C.POINTER(Completion), i32, C.POINTER(C.c_int)], i32),
         "hdm_server_load": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_uint64)], i32),
@@ -207,7 +211,7 @@ def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, s
     o.chunk = int(chunk)
     o.crop = 1 if crop else 0
     o.ptr_kind = ptr_kind
-    o.timing = 1 if timing else 0
+    o.timing = int(timing)
     if stream is not None:
         o.stream = stream
     return o
@@ -219,6 +223,24 @@ def last_timings() -> dict:
     n = lib().hdm_last_timings(ms, 5)
     names = ("pack", "meta", "dilate", "finalize", "total")
     return {names[i]: float(ms[i]) for i in range(n)}
+
+
+def last_trace(max_iters: int = 4096):
+    """Per-iteration trace of the last call made with timing=2: a list of dicts
+    with the phase times in microseconds from the iteration start and the
+    queue length."""
+    buf = (C.c_uint64 * (8 * max_iters))()
+    tick = C.c_double()
+    n = lib().hdm_last_trace(buf, max_iters, C.byref(tick))
+    out = []
+    for i in range(n):
+        r = buf[8 * i: 8 * i + 8]
+        us = lambda k: round((r[k] - r[0]) * tick.value / 1e3, 2) if r[k] else None
+        nxt = buf[8 * (i + 1)] if i + 1 < n else 0
+        out.append({"t": i, "queued": int(r[5]), "all_done_us": us(1), "wg0_done_us": us(2),
+                    "arrived_us": us(3), "folded_us": us(4),
+                    "next_start_us": round((nxt - r[0]) * tick.value / 1e3, 2) if nxt else None})
+    return out
 
 
 def hausdorff(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO,

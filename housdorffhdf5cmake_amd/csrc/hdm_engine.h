@@ -33,16 +33,21 @@ constexpr int kHitSlots = 16;      // per (pair, mask) per-iteration hit counter
 constexpr int kSumSlots = 64;      // per (pair, field) partial voxel counts
 constexpr int kMaxGrid = 4096;     // persistent workgroups (arrival flags)
 constexpr int kLineWords = 32;     // one 128-byte line
+constexpr int kReleaseCopies = 32; // replicas of the barrier release line
 
 constexpr uint32_t kFull = 0x80000000u;
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
-// queue entry: tile (32) | pair (16) | mask (1) | first (1) | check (1)
+// queue entry: tile (32) | pair (16) | mask (1) | first (1) | check (1) |
+// generation of the pair slot (13): a serving slot reused by a new request
+// gets a new generation, so entries left by its previous owner are dropped
+constexpr uint32_t kGenMask = 0x1FFF;
 __host__ __device__ inline uint64_t make_entry(uint32_t tile, uint32_t pair, uint32_t m,
-                                               uint32_t first, uint32_t check) {
+                                               uint32_t first, uint32_t check, uint32_t gen) {
   return (uint64_t)tile | ((uint64_t)pair << 32) | ((uint64_t)m << 48) |
-         ((uint64_t)first << 49) | ((uint64_t)check << 50);
+         ((uint64_t)first << 49) | ((uint64_t)check << 50) | ((uint64_t)(gen & kGenMask) << 51);
 }
+__host__ __device__ inline uint32_t entry_gen(uint64_t e) { return (uint32_t)(e >> 51) & kGenMask; }
 
 struct PairDesc {
   const void *gold;
@@ -55,6 +60,8 @@ struct PairDesc {
   uint32_t t_start;  // engine iteration at which the pair entered
   int32_t crop;
   int32_t tile_kind;
+  uint32_t gen;  // slot generation carried by this pair's queue entries
+  uint32_t pad;
   double robust;
   uint64_t words;  // ntiles * R, words per packed mask
   void *orig[2];
@@ -76,7 +83,7 @@ struct PairCtr {
   // same value during an iteration.
   int32_t done_from[2];
   int32_t pad[3];
-  uint32_t hits[2][kHitSlots];                  // new hits of the running iteration
+  unsigned long long hits[2][kHitSlots];        // cumulative hits, spread over slots
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
 };
 
@@ -84,6 +91,8 @@ struct PairOut {
   int32_t hd, hd_fp, hd_fn, status;
   unsigned long long sum_fp, cnt_fp, sum_fn, cnt_fn;
   unsigned long long n_gold, n_segm, n_fp, n_fn;
+  int32_t done;  // both passes finished at the engine's current iteration
+  int32_t pad;
 };
 
 struct Engine {
@@ -94,12 +103,16 @@ struct Engine {
   uint32_t nwg;
   unsigned long long qtotal;
   uint32_t qcap;
-  uint32_t pad0;
+  uint32_t qnext;  // entries queued for the next iteration when the kernel left
   uint32_t pad1[8];
   // device only
-  uint32_t gen[kLineWords];            // barrier release epoch
+  uint32_t gen[kLineWords];            // unused (kept for layout)
   uint32_t qlen[3][kLineWords];        // [t % 3][sub-queue]: read t%3, fill (t+1)%3
   uint32_t arrive[kMaxGrid];           // last epoch each workgroup arrived at
+  // Release lines, kReleaseCopies replicas so polling spreads over channels:
+  // word 0 the epoch, words 1..kSubQ+1 the prefix offsets of the sub-queues of
+  // the iteration the epoch starts.  Workgroup b polls replica b % kReleaseCopies.
+  uint32_t rel[kReleaseCopies][kLineWords];
 };
 constexpr size_t kEngineHostBytes = 64;
 

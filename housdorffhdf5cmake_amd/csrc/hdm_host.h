@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "../../include/hdm.h"
 #include "hdm_plan.h"
@@ -31,6 +32,8 @@ struct Ctx {
   void *host_out = nullptr;  // pinned, 4 KiB
   hipEvent_t ev[5] = {};     // phase boundaries of a timed call
   float last_ms[5] = {};
+  std::vector<uint64_t> last_trace;  // 8 words per iteration of a traced call
+  DevMem trace;
   std::mutex mu;
 };
 
@@ -40,6 +43,7 @@ struct LaunchCfg {
   int grid = 1, chunk = 1;
   uint64_t spin = 0;
   const hipEvent_t *ev = nullptr;  // 5 events or null
+  uint64_t *trace = nullptr;       // per-iteration device trace or null
 };
 
 struct GroupPtrs {

@@ -89,6 +89,12 @@ int get_ctx(Ctx **out) {
         nb < 1)
       rc = HDM_E_HIP;
     c->occ[k] = nb;
+    if (rc == HDM_OK &&  // both dilation variants must fit the same resident grid
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->dilate_single_fn,
+                                                      kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ[k] = std::min(c->occ[k], nb);
     if (rc == HDM_OK &&
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->pack_fn,
                                                       kWgThreads, 0) != hipSuccess ||
@@ -113,7 +119,9 @@ int get_ctx(Ctx **out) {
 int resident_grid(const Ctx *c, int tile, int requested) {
   const int maxres = std::min(c->occ[tile] * c->cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
-  return std::min(maxres, c->cus);
+  // 2 workgroups per CU: dense iterations want many waves, sparse ones a
+  // short barrier; the autotuner picks per workload
+  return std::min(maxres, 2 * c->cus);
 }
 
 uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.0); }
@@ -169,7 +177,7 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   if (e != hipSuccess || (e = mark(2)) != hipSuccess) return e;
   const int flags = barrier_mode() | (npairs == 1 ? 4 : 0);
   e = tl->dilate(dim3(cfg.grid), s, P.descs, P.ctrs, npairs, P.engine, P.q, qcap, INT_MAX,
-                 cfg.chunk, cfg.spin, P.qstats, qstats_len, flags);
+                 cfg.chunk, cfg.spin, P.qstats, qstats_len, flags, cfg.trace);
   if (e != hipSuccess || (e = mark(3)) != hipSuccess) return e;
   e = hdm_launch_finalize(s, P.descs, P.ctrs, P.engine, P.outs, 0, npairs);
   if (e != hipSuccess) return e;
@@ -310,6 +318,12 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const PackIndex pi{0, 0};
   LaunchCfg cfg = launch_cfg(c, tile, g.row_words, g.ntiles, opts.grid_ctas, opts.chunk);
   if (opts.timing) cfg.ev = c->ev;
+  const size_t trace_bytes = (size_t)g.hist_len * 8 * sizeof(uint64_t);
+  if (opts.timing == 2) {  // per-iteration device trace (diagnostic, not graph-safe)
+    if ((rc = c->trace.ensure(trace_bytes)) != HDM_OK) return rc;
+    if (hipMemsetAsync(c->trace.p, 0, trace_bytes, s) != hipSuccess) return HDM_E_HIP;
+    cfg.trace = (uint64_t *)c->trace.p;
+  }
   hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, qcap, cfg,
                                g.hist_len);
   if (e != hipSuccess) return map_hip(e);
@@ -338,11 +352,21 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     c->last_ms[4] = all;
     (void)hipGetLastError();
   }
+  if (opts.timing == 2) {
+    Engine head{};
+    std::memcpy(&head, ho + 256, kEngineHostBytes);
+    const size_t nrec = std::min<size_t>(head.t, (size_t)g.hist_len);
+    c->last_trace.assign(nrec * 8, 0);
+    if (nrec && hipMemcpy(c->last_trace.data(), c->trace.p, nrec * 8 * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost) != hipSuccess)
+      return HDM_E_HIP;
+  }
   PairOut o;
   Engine es{};
   std::memcpy(&o, ho, sizeof(o));
   std::memcpy(&es, ho + 256, kEngineHostBytes);
   fill_result(out, o, &es, g.ntiles, (int)es.t);
+  if (o.status == HDM_OK && !o.done) return HDM_E_STUCK;  // queue drained first
   return o.status;
 }
 
@@ -540,7 +564,8 @@ extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
     PairOut o;
     std::memcpy(&o, (char *)b->host + sizeof(PairOut) * i, sizeof(o));
     fill_result(&out[i], o, &es, b->geoms[i].ntiles, std::max(o.hd_fp, o.hd_fn));
-    if (o.status != HDM_OK && rc == HDM_OK) rc = o.status;
+    const int st = (o.status == HDM_OK && !o.done) ? HDM_E_STUCK : o.status;
+    if (st != HDM_OK && rc == HDM_OK) rc = st;
   }
   return rc;
 }
@@ -619,6 +644,16 @@ extern "C" int hdm_autotune(const hdm_pair *pairs, int n, const int32_t *tiles, 
   });
   *n_out = k;
   return HDM_OK;
+}
+
+extern "C" int hdm_last_trace(uint64_t *out, int max_iters, double *tick_ns) {
+  Ctx *c = nullptr;
+  if (get_ctx(&c) != HDM_OK) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (tick_ns) *tick_ns = 1e6 / c->clock_khz;
+  const int n = std::min<int>(max_iters, (int)(c->last_trace.size() / 8));
+  if (out && n > 0) std::memcpy(out, c->last_trace.data(), (size_t)n * 8 * sizeof(uint64_t));
+  return n;
 }
 
 extern "C" int hdm_last_timings(float *ms, int n) {

@@ -141,12 +141,14 @@ __global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_
       es->qmax = 0;
       es->qtotal = 0;
       es->qcap = qcap;
+      es->qnext = 0;
     }
     for (int i = gid; i < kLineWords; i += stride) {
       es->gen[i] = 0;
       es->qlen[0][i] = es->qlen[1][i] = es->qlen[2][i] = 0;
     }
     for (int i = gid; i < kMaxGrid; i += stride) es->arrive[i] = 0;
+    for (int i = gid; i < kReleaseCopies * kLineWords; i += stride) es->rel[0][i] = 0;
   }
   for (int p = p0 + gid; p < p0 + npairs; p += stride) {
     PairCtr &c = ctrs[p];
@@ -284,34 +286,56 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
   const int lane = threadIdx.x & 63;
   const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
   const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
-  const uint64_t per = ((total_words + nw - 1) / nw + 63) & ~uint64_t(63);
+  constexpr int U = 2;  // row words per lane per step, all loads issued first
+  const uint64_t per = ((total_words + nw - 1) / nw + 64 * U - 1) & ~uint64_t(64 * U - 1);
   const uint64_t w0 = gw * per, w1 = min(total_words, w0 + per);
   const int slot = (int)(gw % kSumSlots);
   PackAcc acc;
   acc_clear(acc);
   int ap = -1;  // pair the accumulator belongs to (wave-uniform)
-  for (uint64_t base = w0; base < w1; base += 64) {
-    const uint64_t g = base + lane;
-    const bool in = g < w1;
-    int p = 0;
-    if (npairs > 1) p = find_pair(npairs, in ? g : w1 - 1, [&](int i) { return idx[i].rw_base; });
+  for (uint64_t base = w0; base < w1; base += 64 * U) {
+    bool inU[U];
+    int pU[U], wxU[U], yU[U], zU[U];
+    uint64_t wU[U], vbU[U];
+    Row aU[U], bU[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint64_t g = base + 64 * u + lane;
+      inU[u] = g < w1;
+      pU[u] = 0;
+      if (npairs > 1)
+        pU[u] = find_pair(npairs, inU[u] ? g : w1 - 1, [&](int i) { return idx[i].rw_base; });
+      aU[u] = bU[u] = 0;
+      if (inU[u]) {
+        const PairDesc &d = descs[pU[u]];
+        const uint64_t w = g - idx[pU[u]].rw_base;
+        wU[u] = w;
+        wxU[u] = (int)(w % (uint64_t)d.wxn);
+        const uint64_t rest = w / (uint64_t)d.wxn;
+        yU[u] = (int)(rest % (uint64_t)d.ny);
+        zU[u] = (int)(rest / (uint64_t)d.ny);
+        vbU[u] = (uint64_t)wxU[u] * TXB + (uint64_t)d.nx * ((uint64_t)yU[u] + (uint64_t)d.ny * zU[u]);
+        const int count = min(TXB, d.nx - wxU[u] * TXB);
+        aU[u] = load_row_bits<Row>(d, d.gold, vbU[u], count);
+        bU[u] = load_row_bits<Row>(d, d.segm, vbU[u], count);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+    const int p = pU[u];
     const int lead = __builtin_amdgcn_readfirstlane(p);
     if (ap != lead) {
       if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
       ap = lead;
     }
-    if (!in) continue;
+    if (!inU[u]) continue;
     const PairDesc &d = descs[p];
-    const uint64_t w = g - idx[p].rw_base;
-    const int wx = (int)(w % (uint64_t)d.wxn);
-    const uint64_t rest = w / (uint64_t)d.wxn;
-    const int y = (int)(rest % (uint64_t)d.ny);
-    const int z = (int)(rest / (uint64_t)d.ny);
-    const uint64_t vbase =
-        (uint64_t)wx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+    const uint64_t w = wU[u];
+    const int wx = wxU[u], y = yU[u], z = zU[u];
+    const uint64_t vbase = vbU[u];
     const int count = min(TXB, d.nx - wx * TXB);
-    const Row a = load_row_bits<Row>(d, d.gold, vbase, count);
-    const Row b = load_row_bits<Row>(d, d.segm, vbase, count);
+    const Row a = aU[u];
+    const Row b = bU[u];
     const uint32_t tile =
         (uint32_t)wx + (uint32_t)d.gx * ((uint32_t)(y / TY) + (uint32_t)d.gy * (uint32_t)(z / TZ));
     const uint64_t o = (uint64_t)tile * R + (uint64_t)(y % TY) + (uint64_t)TY * (z % TZ);
@@ -348,6 +372,7 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
         atomicMin(&cp.lo[1], ty_), atomicMax(&cp.hi[1], ty_);
         atomicMin(&cp.lo[2], tz_), atomicMax(&cp.hi[2], tz_);
       }
+    }
     }
   }
   if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
@@ -520,7 +545,7 @@ __global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__rest
           const uint32_t nv = full ? kFull : 1u;
           if (atomicCAS(&d.state[m][tile], 0u, nv) == 0u && !full && needed) {
             want = true;
-            ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+            ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1, d.gen);
           }
         } else if (r < 6 && needed) {
           const bool face = r == 0 ? fxl != 0 : r == 1 ? fxh != 0 : r == 2 ? fyl : r == 3 ? fyh
@@ -528,7 +553,7 @@ __global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__rest
           const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, r);
           if (face && nb != kNoTile && atomicCAS(&d.state[m][nb], 0u, 1u) == 0u) {
             want = true;
-            ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1);
+            ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1, d.gen);
           }
         }
       }
@@ -547,9 +572,13 @@ __global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__rest
 // for it.  Waits are bounded by wall-clock ticks: on expiry the engine error
 // flag is set and the caller leaves its loop, so no wave waits forever.
 // With fence set, the arrival and release also write back / invalidate L2.
+// The master also publishes, with the epoch, the prefix offsets of the
+// sub-queues of the next iteration (next_qlen), so one read of its release
+// replica gives every workgroup both; s_pre receives them.
 template <typename Fold>
 __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_t limit,
-                                             bool fence, int *s_ok, Fold fold) {
+                                             bool fence, int *s_ok, Fold fold, uint64_t *tr,
+                                             const uint32_t *next_qlen, uint32_t *s_pre) {
   if (fence) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -576,16 +605,28 @@ __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_
       *s_ok = 0;
     }
     __syncthreads();
+    if (tr && threadIdx.x == 0) tr[3] = wall_clock64();
     if (*s_ok) fold();
-    __builtin_amdgcn_s_waitcnt(0);
+    if (threadIdx.x < kSubQ) s_pre[threadIdx.x + 1] = ld_sc(&next_qlen[threadIdx.x]);
     __syncthreads();
     if (threadIdx.x == 0) {
+      s_pre[0] = 0;
+      for (int k = 1; k <= kSubQ; ++k) s_pre[k] += s_pre[k - 1];
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < kReleaseCopies * (kSubQ + 1); j += blockDim.x)
+      st_sc(&es->rel[j / (kSubQ + 1)][1 + j % (kSubQ + 1)], s_pre[j % (kSubQ + 1)]);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tr && threadIdx.x == 0) tr[4] = wall_clock64();
+    if (threadIdx.x < kReleaseCopies) {
       if (fence) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      st_sc(&es->gen[0], epoch);
+      st_sc(&es->rel[threadIdx.x][0], epoch);
     }
   } else if (threadIdx.x == 0) {
+    const uint32_t *line = es->rel[blockIdx.x % kReleaseCopies];
     const uint64_t t0 = wall_clock64();
-    while (ld_sc(&es->gen[0]) != epoch) {
+    while (ld_sc(&line[0]) != epoch) {
       __builtin_amdgcn_s_sleep(1);
       if (wall_clock64() - t0 > limit) {
         atomicOr(&es->error, 1u);
@@ -593,6 +634,7 @@ __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_
         break;
       }
     }
+    for (int k = 0; k <= kSubQ; ++k) s_pre[k] = ld_sc(&line[1 + k]);
   }
   __syncthreads();
   if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -615,14 +657,16 @@ __device__ __forceinline__ void hit_add(HitTable &h, PairCtr *ctrs, uint32_t key
       return;
     }
   }
-  atomicAdd(&ctrs[key >> 1].hits[key & 1][blockIdx.x % kHitSlots], v);  // table full
+  atomicAdd(&ctrs[key >> 1].hits[key & 1][blockIdx.x % kHitSlots],
+            (unsigned long long)v);  // table full
 }
 __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
   __syncthreads();
   if (threadIdx.x < kHitTable) {
     const uint32_t k = h.key[threadIdx.x];
     if (k != ~0u) {
-      atomicAdd(&ctrs[k >> 1].hits[k & 1][blockIdx.x % kHitSlots], h.cnt[threadIdx.x]);
+      atomicAdd(&ctrs[k >> 1].hits[k & 1][blockIdx.x % kHitSlots],
+                (unsigned long long)h.cnt[threadIdx.x]);
       h.key[threadIdx.x] = ~0u;
       h.cnt[threadIdx.x] = 0;
     }
@@ -630,11 +674,13 @@ __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
 }
 
 // ---------------------------------------------------------------- dilate
-template <int TXB, int TY, int TZ>
+// SINGLE: the launch holds one pair, so its descriptor and its per-iteration
+// pass state are loop invariants instead of per-entry loads.
+template <int TXB, int TY, int TZ, bool SINGLE>
 __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
     const PairDesc *__restrict__ descs, PairCtr *ctrs, int npairs, Engine *es, uint64_t *q,
     uint32_t qcap, int max_iters, int chunk, uint64_t spin_limit, uint32_t *qstats,
-    int qstats_len, int flags) {
+    int qstats_len, int flags, uint64_t *trace) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;
@@ -643,7 +689,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
   __shared__ uint32_t s_pre[kSubQ + 1];
   __shared__ int s_ok;
   const bool fence = (flags & 1) != 0;
-  const bool single = (flags & 4) != 0;  // one pair: stop as soon as both passes are done
+  constexpr bool single = SINGLE;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int sub = lane / R;
@@ -659,21 +705,34 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
   __syncthreads();
   uint32_t t = ld_sc(&es->t);
   for (int it = 0; it < max_iters; ++it) {
-    // queue of iteration t: sub-queue lengths -> prefix offsets
-    if (threadIdx.x == 0) {
-      uint32_t acc = 0;
-      for (int k = 0; k < kSubQ; ++k) {
-        s_pre[k] = acc;
-        acc += ld_sc(&es->qlen[t % 3][k]);
+    // queue of iteration t: sub-queue lengths -> prefix offsets (after the
+    // first iteration the barrier delivers them with the release)
+    if (it == 0) {
+      if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int k = 0; k < kSubQ; ++k) {
+          s_pre[k] = acc;
+          acc += ld_sc(&es->qlen[t % 3][k]);
+        }
+        s_pre[kSubQ] = acc;
       }
-      s_pre[kSubQ] = acc;
+      __syncthreads();
     }
-    __syncthreads();
     const uint32_t L = s_pre[kSubQ];
     if (L == 0) break;
-    if (single) {
+    bool pdone[2] = {false, false};
+    if (single) {  // stop as soon as both passes are done (uniform decision)
       const int tp0 = (int)(t - descs[0].t_start);
-      if (pass_done(descs[0], ctrs[0], 0, tp0) && pass_done(descs[0], ctrs[0], 1, tp0)) break;
+      pdone[0] = pass_done(descs[0], ctrs[0], 0, tp0);
+      pdone[1] = pass_done(descs[0], ctrs[0], 1, tp0);
+      if (pdone[0] && pdone[1]) break;
+    }
+    // per-iteration trace (diagnostic): [0] start, [1] last workgroup done,
+    // [2] workgroup 0 done, [3] all arrived, [4] folded, [5] queue length
+    uint64_t *tr = (trace && (int)t < qstats_len) ? trace + 8 * (uint64_t)t : nullptr;
+    if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
+      tr[0] = wall_clock64();
+      tr[5] = L;
     }
     uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
     uint32_t *qout_len = &es->qlen[(t + 1) % 3][kq];
@@ -688,14 +747,16 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
           ent = ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
         }
         const uint32_t tile = (uint32_t)ent;
-        const int p = (int)((ent >> 32) & 0xFFFF);
+        const int p = SINGLE ? 0 : (int)((ent >> 32) & 0xFFFF);
         const int m = (int)((ent >> 48) & 1);
         const bool first = (ent >> 49) & 1;
         const bool check = (ent >> 50) & 1;
         const PairDesc &d = descs[p];
         PairCtr &c = ctrs[p];
         const uint32_t tp = t - d.t_start;  // pair-local iteration
-        if (live && pass_done(d, c, m, (int)tp)) live = false;  // hello.cu:1748-1759
+        if (!SINGLE && live && entry_gen(ent) != (d.gen & kGenMask)) live = false;  // stale
+        if (live && (SINGLE ? pdone[m] : pass_done(d, c, m, (int)tp)))  // hello.cu:1748-1759
+          live = false;
         if (live && tp + 1 >= (uint32_t)d.hist_len) {
           c.status = -8;
           live = false;
@@ -822,19 +883,27 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
             }
             if (r == 6) {  // stays queued while not full
               want = true;
-              pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0);
+              pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0, d.gen);
             }
             if (r < 6 && nb_dir != kNoTile && nst == 0) {  // activate idle neighbours
               const bool face_on = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
                                    : r == 3 ? oyh : r == 4 ? ozl : ozh;
               if (face_on && atomicCAS(&d.state[m][nb_dir], 0u, tp + 2) == 0u) {
                 want = true;
-                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1);
+                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, d.gen);
               }
             }
           }
         }
         push_entries(want, pent, lq, Qout, qout_len, lane);
+      }
+    }
+    if (tr) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const uint64_t now = wall_clock64();
+        atomicMax((unsigned long long *)&tr[1], (unsigned long long)now);
+        if (blockIdx.x == 0) tr[2] = now;
       }
     }
     flush_local_queue(lq, Qout, qout_len);
@@ -845,27 +914,33 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
     // the histogram and the covered counts, decide each pass's stop
     // (hello.cu:1748-1759), reset the sub-queue counters that iteration t+1 fills
     auto fold = [&]() {
-      for (int i = threadIdx.x; i < 2 * npairs; i += blockDim.x) {
+      // kHitSlots lanes per (pair, mask): each loads one cumulative slot, the
+      // group sums them; the increase over the last fold is this iteration's
+      // histogram bin
+      const int s = threadIdx.x % kHitSlots;
+      for (int base = 0; base < 2 * npairs; base += kWgThreads / kHitSlots) {
+        const int i = base + threadIdx.x / kHitSlots;
         const int pp = i >> 1, mm = i & 1;
-        const PairDesc &dd = descs[pp];
-        if (tdone < dd.t_start) continue;
-        const int tpp = (int)(tdone - dd.t_start);
-        PairCtr &cc = ctrs[pp];
-        uint32_t v = 0;
-#pragma unroll
-        for (int s = 0; s < kHitSlots; ++s) {
-          const uint32_t h = ld_sc(&cc.hits[mm][s]);
-          if (h) {
-            v += h;
-            st_sc(&cc.hits[mm][s], 0u);
-          }
+        bool act = i < 2 * npairs;
+        int tpp = 0;
+        if (act) {
+          const uint32_t ts = descs[pp].t_start;
+          act = tdone >= ts;
+          tpp = (int)(tdone - ts);
+          if (act && ld_sc(&ctrs[pp].done_from[mm]) <= tpp) act = false;
         }
-        if (v == 0) continue;
-        if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = v;
+        unsigned long long v = act ? ld_sc(&ctrs[pp].hits[mm][s]) : 0ull;
+#pragma unroll
+        for (int off = kHitSlots / 2; off >= 1; off >>= 1) v += shfl64(v, (threadIdx.x & 63) ^ off);
+        if (!act || s != 0) continue;
+        PairCtr &cc = ctrs[pp];
         const unsigned long long before = cc.cov[mm];
-        cc.cov[mm] = before + v;
+        if (v == before) continue;
+        const PairDesc &dd = descs[pp];
+        if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = (uint32_t)(v - before);
+        cc.cov[mm] = v;
         const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
-        if (before < cut && before + v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
+        if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
       }
       if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 1) % 3][threadIdx.x], 0u);
       if (threadIdx.x == 0) {
@@ -874,89 +949,87 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
         if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
       }
     };
-    if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold)) break;
+    if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 3], s_pre)) break;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) st_sc(&es->t, t);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st_sc(&es->t, t);
+    st_sc(&es->qnext, s_pre[kSubQ]);
+  }
 }
 
 // ---------------------------------------------------------------- finalize
 // HD, sums and counts per pair from the first-hit histograms, with the robust
 // cut: the pass of mask m stops at the first t where the covered count reaches
-// ceil(r * total) (hello.cu:1748-1759, 1887-1895).
+// ceil(r * total) (hello.cu:1748-1759, 1887-1895).  Each thread owns a
+// contiguous block of the histogram; one block scan finds the stop.
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, T *sh, Op op) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kWgThreads / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] = op(sh[threadIdx.x], sh[threadIdx.x + off]);
+    __syncthreads();
+  }
+  const T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 __global__ __launch_bounds__(kWgThreads) void finalize_kernel(const PairDesc *__restrict__ descs,
                                                               const PairCtr *__restrict__ ctrs,
                                                               const Engine *es, PairOut *outs,
                                                               int p0, int npairs) {
-  __shared__ unsigned long long s_pref[kWgThreads];
+  __shared__ unsigned long long sh[kWgThreads];
   __shared__ int s_stop;
   const int p = p0 + blockIdx.x;
   if (p >= p0 + npairs) return;
   const PairDesc &d = descs[p];
   const PairCtr &c = ctrs[p];
+  const int per = (d.hist_len + kWgThreads - 1) / kWgThreads;
+  const int t0 = max(1, (int)threadIdx.x * per), t1 = min(d.hist_len, ((int)threadIdx.x + 1) * per);
   int hd[2] = {0, 0};
   unsigned long long sum[2] = {0, 0}, cnt[2] = {0, 0};
   bool reached[2] = {true, true};
+  auto add = [](unsigned long long a, unsigned long long b) { return a + b; };
+  auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
   for (int m = 0; m < 2; ++m) {
     const unsigned long long tgt = robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn);
-    if (tgt == 0) continue;
-    reached[m] = false;
-    unsigned long long carry = 0, lsum = 0, lcnt = 0;
-    int lhd = 0;
-    for (int t0 = 1; t0 < d.hist_len; t0 += kWgThreads) {
-      const int t = t0 + threadIdx.x;
-      const unsigned long long h = t < d.hist_len ? d.hist[m][t] : 0;
-      s_pref[threadIdx.x] = h;
+    if (tgt == 0) continue;  // uniform
+    const uint32_t *h = d.hist[m];
+    unsigned long long local = 0;
+    for (int t = t0; t < t1; ++t) local += h[t];
+    // exclusive scan of the per-thread block totals
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < kWgThreads; off <<= 1) {
+      const unsigned long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
       __syncthreads();
-      for (int off = 1; off < kWgThreads; off <<= 1) {  // inclusive scan
-        const unsigned long long v = threadIdx.x >= off ? s_pref[threadIdx.x - off] : 0;
-        __syncthreads();
-        s_pref[threadIdx.x] += v;
-        __syncthreads();
-      }
-      const unsigned long long pre = carry + s_pref[threadIdx.x];
-      if (threadIdx.x == 0) s_stop = 0x7FFFFFFF;
+      sh[threadIdx.x] += v;
       __syncthreads();
-      if (t < d.hist_len && pre >= tgt) atomicMin(&s_stop, t);
-      __syncthreads();
-      const int stop = s_stop;
-      if (t < d.hist_len && t <= stop) {
-        lsum += (unsigned long long)t * h;
-        lcnt += h;
-        if (h) lhd = t;
-      }
-      carry += s_pref[kWgThreads - 1];
-      __syncthreads();
-      if (stop != 0x7FFFFFFF) {
-        reached[m] = true;
+    }
+    unsigned long long run = sh[threadIdx.x] - local;
+    if (threadIdx.x == 0) s_stop = 0x7FFFFFFF;
+    __syncthreads();
+    for (int t = t0; t < t1; ++t) {
+      run += h[t];
+      if (run >= tgt) {
+        atomicMin(&s_stop, t);
         break;
       }
     }
-    // reduce lsum, lcnt, max lhd over the block
-    s_pref[threadIdx.x] = lsum;
     __syncthreads();
-    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) s_pref[threadIdx.x] += s_pref[threadIdx.x + off];
-      __syncthreads();
+    const int stop = s_stop;
+    reached[m] = stop != 0x7FFFFFFF;
+    unsigned long long ls = 0, lc = 0, lh = 0;
+    for (int t = t0; t < t1 && t <= stop; ++t) {
+      const unsigned long long v = h[t];
+      ls += (unsigned long long)t * v;
+      lc += v;
+      if (v) lh = (unsigned long long)t;
     }
-    sum[m] = s_pref[0];
-    __syncthreads();
-    s_pref[threadIdx.x] = lcnt;
-    __syncthreads();
-    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off) s_pref[threadIdx.x] += s_pref[threadIdx.x + off];
-      __syncthreads();
-    }
-    cnt[m] = s_pref[0];
-    __syncthreads();
-    s_pref[threadIdx.x] = (unsigned long long)lhd;
-    __syncthreads();
-    for (int off = kWgThreads / 2; off > 0; off >>= 1) {
-      if (threadIdx.x < off)
-        s_pref[threadIdx.x] = max(s_pref[threadIdx.x], s_pref[threadIdx.x + off]);
-      __syncthreads();
-    }
-    hd[m] = (int)s_pref[0];
-    __syncthreads();
+    sum[m] = block_reduce(ls, sh, add);
+    cnt[m] = block_reduce(lc, sh, add);
+    hd[m] = (int)block_reduce(lh, sh, mx);
   }
   if (threadIdx.x == 0) {
     PairOut o;
@@ -964,12 +1037,17 @@ __global__ __launch_bounds__(kWgThreads) void finalize_kernel(const PairDesc *__
     o.hd = max(hd[0], hd[1]);
     o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
     o.n_gold = c.n_gold, o.n_segm = c.n_segm, o.n_fp = c.n_fp, o.n_fn = c.n_fn;
+    const int tl = (int)(es->t - d.t_start);  // pair-local iterations so far
+    const bool d0 = robust_cut(d.robust, c.n_fp) == 0 || c.done_from[0] <= tl;
+    const bool d1 = robust_cut(d.robust, c.n_fn) == 0 || c.done_from[1] <= tl;
+    o.done = (d0 && d1) ? 1 : 0;
+    o.pad = 0;
     int st = c.status;
     if ((c.n_fp && !c.n_gold) || (c.n_fn && !c.n_segm))
       st = -2;  // HDM_E_EMPTY_MASK
     else if (es->error)
       st = -5;  // HDM_E_TIMEOUT
-    else if (st == 0 && (!reached[0] || !reached[1]))
+    else if (st == 0 && o.done && (!reached[0] || !reached[1]))
       st = -8;  // HDM_E_STUCK
     o.status = st;
     outs[p] = o;
@@ -994,17 +1072,27 @@ hipError_t launch_meta(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
 template <int TXB, int TY, int TZ>
 hipError_t launch_dilate(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c, int np,
                          Engine *es, uint64_t *q, uint32_t qcap, int max_iters, int chunk,
-                         uint64_t spin, uint32_t *qs, int qsl, int flags) {
-  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, c, np, es, q,
-                     qcap, max_iters, chunk, spin, qs, qsl, flags);
+                         uint64_t spin, uint32_t *qs, int qsl, int flags, uint64_t *trace) {
+  if (np == 1)
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true>), grid, dim3(kWgThreads), 0, s, d, c, np,
+                       es, q, qcap, max_iters, chunk, spin, qs, qsl, flags, trace);
+  else
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false>), grid, dim3(kWgThreads), 0, s, d, c,
+                       np, es, q, qcap, max_iters, chunk, spin, qs, qsl, flags, trace);
   return hipGetLastError();
 }
 
 template <int TXB, int TY, int TZ>
 TileLaunch make_launch() {
-  return TileLaunch{&launch_pack<TXB, TY, TZ>, &launch_meta<TXB, TY, TZ>,
-                    &launch_dilate<TXB, TY, TZ>, (const void *)&dilate_kernel<TXB, TY, TZ>,
-                    (const void *)&pack_kernel<TXB, TY, TZ>, TXB, TY, TZ};
+  return TileLaunch{&launch_pack<TXB, TY, TZ>,
+                    &launch_meta<TXB, TY, TZ>,
+                    &launch_dilate<TXB, TY, TZ>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, false>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, true>,
+                    (const void *)&pack_kernel<TXB, TY, TZ>,
+                    TXB,
+                    TY,
+                    TZ};
 }
 
 }  // namespace hdm

@@ -21,8 +21,9 @@ struct TileLaunch {
   hipError_t (*meta)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
   hipError_t (*dilate)(dim3, hipStream_t, const PairDesc *, PairCtr *, int, Engine *, uint64_t *,
-                       uint32_t, int, int, uint64_t, uint32_t *, int, int);
-  const void *dilate_fn;
+                       uint32_t, int, int, uint64_t, uint32_t *, int, int, uint64_t *);
+  const void *dilate_fn;         // several pairs
+  const void *dilate_single_fn;  // one pair
   const void *pack_fn;
   int txb, ty, tz;
 };

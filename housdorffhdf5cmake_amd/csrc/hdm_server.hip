@@ -1,10 +1,124 @@
-// Serving engine (continuous batching over the persistent dilation kernel).
+// Serving engine: continuous batching over the persistent dilation kernel.
+//
+// No counterpart in the reference (one pair per blocking call, hello.cu:2025).
+// Requests wait in a FIFO, are admitted into one of max_slots slots when a
+// slot is free and the HBM arena (the budget) has room for their workspace,
+// join the running engine at an iteration boundary (their descriptor carries
+// the engine iteration at which they start, so every pair keeps its own
+// iteration count), and retire as soon as both of their passes are done.
+// One step = admit, one slice of slice_iters engine iterations, retire.
+// Host-pointer inputs are copied on a separate copy stream; the engine stream
+// waits on an event per request, so copies overlap the dilation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
 
 #include "../../include/hdm.h"
+#include "hdm_host.h"
+
+using namespace hdm;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// first-fit allocator over [0, size) with 256-byte granules
+struct Arena {
+  uint64_t size = 0, used = 0;
+  std::map<uint64_t, uint64_t> free;  // offset -> length
+  void init(uint64_t n) {
+    size = n;
+    used = 0;
+    free.clear();
+    free[0] = n;
+  }
+  bool alloc(uint64_t n, uint64_t *off) {
+    n = align256(n);
+    for (auto it = free.begin(); it != free.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const uint64_t rest = it->second - n, at = it->first + n;
+        free.erase(it);
+        if (rest) free[at] = rest;
+        used += n;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    n = align256(n);
+    used -= n;
+    auto it = free.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free.erase(nx);
+    }
+    if (it != free.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free.erase(it);
+      }
+    }
+  }
+};
+
+struct Request {
+  uint64_t id = 0;
+  const void *gold = nullptr, *segm = nullptr;
+  int nx = 0, ny = 0, nz = 0, dtype = 0, label = 1, ptr_kind = HDM_PTR_DEVICE;
+  double t_submit = 0, t_admit = 0;
+  int slot = -1;
+  uint64_t off = 0, bytes = 0;  // arena block: [inputs if copied][workspace]
+  PairGeom geom{};
+  hipEvent_t copied = nullptr;
+};
+
+}  // namespace
+
+struct hdm_server {
+  hdm_server_opts o{};
+  Ctx *ctx = nullptr;
+  hipStream_t s = nullptr, s_copy = nullptr;
+  int tile = HDM_TILE_64x8x8;
+  const TileLaunch *tl = nullptr;
+  TileShape ts{};
+  LaunchCfg cfg{};
+  uint32_t qcap = 0;
+  void *common = nullptr;
+  GroupPtrs P{};
+  void *arena_mem = nullptr;
+  Arena arena;
+  std::deque<Request> pending;
+  std::vector<Request> slots;  // slot index -> request (id 0 = free)
+  std::vector<uint32_t> slot_gen;
+  std::vector<hipEvent_t> event_pool;
+  void *host = nullptr;  // pinned: outs + engine head
+  uint64_t next_id = 1;
+  uint32_t t_engine = 0;
+};
+
+static hipEvent_t take_event(hdm_server *s) {
+  if (!s->event_pool.empty()) {
+    hipEvent_t e = s->event_pool.back();
+    s->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+  return e;
+}
 
 extern "C" void hdm_server_opts_init(hdm_server_opts *o) {
   if (!o) return;
@@ -14,15 +128,260 @@ extern "C" void hdm_server_opts_init(hdm_server_opts *o) {
   o->slice_iters = 8;
   o->tile = HDM_TILE_64x8x8;
   o->robustness = 1.0;
+  o->max_nx = o->max_ny = o->max_nz = 512;
 }
-extern "C" int hdm_server_create(const hdm_server_opts *, hdm_server **) { return HDM_E_CAPACITY; }
-extern "C" int hdm_server_submit(hdm_server *, const void *, const void *, int, int, int, int,
-                                 int, uint64_t *) {
-  return HDM_E_INVALID;
+
+extern "C" double hdm_server_now(void) { return now_s(); }
+
+extern "C" void hdm_server_destroy(hdm_server *s) {
+  if (!s) return;
+  if (s->s) (void)hipStreamSynchronize(s->s);
+  if (s->s_copy) (void)hipStreamSynchronize(s->s_copy);
+  for (auto &r : s->slots)
+    if (r.copied) (void)hipEventDestroy(r.copied);
+  for (auto &r : s->pending)
+    if (r.copied) (void)hipEventDestroy(r.copied);
+  for (auto e : s->event_pool) (void)hipEventDestroy(e);
+  if (s->common) (void)hipFree(s->common);
+  if (s->arena_mem) (void)hipFree(s->arena_mem);
+  if (s->host) (void)hipHostFree(s->host);
+  if (s->s) (void)hipStreamDestroy(s->s);
+  if (s->s_copy) (void)hipStreamDestroy(s->s_copy);
+  (void)hipGetLastError();
+  delete s;
 }
-extern "C" int hdm_server_step(hdm_server *, hdm_completion *, int, int *) { return HDM_E_INVALID; }
-extern "C" int hdm_server_load(hdm_server *, int *, int *, uint64_t *) { return HDM_E_INVALID; }
-extern "C" double hdm_server_now(void) {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+
+extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
+  if (!out) return HDM_E_INVALID;
+  *out = nullptr;
+  hdm_server_opts o;
+  hdm_server_opts_init(&o);
+  if (oin) o = *oin;
+  if (o.max_slots < 2 || o.max_slots > 4096 || o.slice_iters < 1 || o.tile < 0 ||
+      o.tile >= HDM_TILE_COUNT || !(o.robustness >= 0.0 && o.robustness <= 1.0) ||
+      o.max_nx < 1 || o.max_ny < 1 || o.max_nz < 1 || o.hbm_budget < (1u << 20))
+    return HDM_E_INVALID;
+  Ctx *c = nullptr;
+  int rc = get_ctx(&c);
+  if (rc != HDM_OK) return rc;
+  hdm_server *s = new hdm_server();
+  s->o = o;
+  s->ctx = c;
+  s->tile = norm_tile(o.tile);
+  s->tl = hdm_tile_launch(s->tile);
+  s->ts = tile_shape(s->tile);
+  const PairGeom gmax = pair_geom(o.max_nx, o.max_ny, o.max_nz, s->ts);
+  const uint64_t qcap = 2ull * gmax.ntiles * (uint64_t)o.max_slots;
+  if (qcap > 0xFFFFFFFFull) {
+    delete s;
+    return HDM_E_INVALID;
+  }
+  s->qcap = (uint32_t)qcap;
+  const CommonLayout L = common_layout(o.max_slots, qcap, 1);
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  uint64_t arena = o.hbm_budget;
+  if (free_b && arena + L.total > (uint64_t)(free_b * 0.9)) arena = (uint64_t)(free_b * 0.9) - L.total;
+  if (hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&s->common, L.total) != hipSuccess || hipMalloc(&s->arena_mem, arena) != hipSuccess ||
+      hipHostMalloc(&s->host, sizeof(PairOut) * o.max_slots + 256, hipHostMallocDefault) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    hdm_server_destroy(s);
+    return HDM_E_NOMEM;
+  }
+  s->arena.init(arena);
+  s->P = carve_common((char *)s->common, L);
+  s->slots.resize(o.max_slots);
+  s->slot_gen.assign(o.max_slots, 0);
+  s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1);
+  // every slot starts unused: t_start beyond any engine iteration keeps it out
+  // of the per-iteration bookkeeping
+  std::vector<PairDesc> d(o.max_slots);
+  for (auto &x : d) {
+    std::memset(&x, 0, sizeof(x));
+    x.t_start = 0xFFFFFFFFu;
+  }
+  std::vector<PairCtr> cz(o.max_slots);
+  std::memset(cz.data(), 0, sizeof(PairCtr) * o.max_slots);
+  for (auto &x : cz) x.done_from[0] = x.done_from[1] = 0;
+  if (hipMemcpy(s->P.descs, d.data(), sizeof(PairDesc) * o.max_slots, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(s->P.ctrs, cz.data(), sizeof(PairCtr) * o.max_slots, hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    hdm_server_destroy(s);
+    return HDM_E_HIP;
+  }
+  // engine state: t = 0, empty queue
+  hipError_t e = hdm_launch_reset(s->s, s->P.descs, nullptr, nullptr, s->P.idx, s->P.ctrs, 0, 0,
+                                  s->P.engine, 1, s->qcap);
+  if (e != hipSuccess || hipStreamSynchronize(s->s) != hipSuccess) {
+    hdm_server_destroy(s);
+    return HDM_E_HIP;
+  }
+  *out = s;
+  return HDM_OK;
 }
-extern "C" void hdm_server_destroy(hdm_server *) {}
+
+static uint64_t request_bytes(const hdm_server *s, const Request &r, uint64_t *in_bytes) {
+  const uint64_t nvox = (uint64_t)r.nx * r.ny * r.nz;
+  const uint64_t esz = r.dtype == HDM_DTYPE_U8 ? 1 : 4;
+  *in_bytes = r.ptr_kind == HDM_PTR_HOST ? 2 * align256(nvox * esz) : 0;
+  return *in_bytes + r.geom.total;
+}
+
+extern "C" int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx,
+                                 int ny, int nz, int dtype, int label, int ptr_kind,
+                                 uint64_t *id_out) {
+  if (!s || !gold || !segm || nx < 1 || ny < 1 || nz < 1 ||
+      (dtype != HDM_DTYPE_U8 && dtype != HDM_DTYPE_I32) ||
+      (ptr_kind != HDM_PTR_DEVICE && ptr_kind != HDM_PTR_HOST))
+    return HDM_E_INVALID;
+  if (nx > s->o.max_nx || ny > s->o.max_ny || nz > s->o.max_nz) return HDM_E_CAPACITY;
+  Request r;
+  r.id = s->next_id++;
+  r.gold = gold, r.segm = segm;
+  r.nx = nx, r.ny = ny, r.nz = nz, r.dtype = dtype, r.label = label, r.ptr_kind = ptr_kind;
+  r.geom = pair_geom(nx, ny, nz, s->ts);
+  uint64_t inb = 0;
+  r.bytes = request_bytes(s, r, &inb);
+  if (align256(r.bytes) > s->arena.size) return HDM_E_BUDGET;
+  r.t_submit = now_s();
+  s->pending.push_back(r);
+  if (id_out) *id_out = r.id;
+  return HDM_OK;
+}
+
+// Admit one request into slot p: inputs (copied or resident), descriptor,
+// counters, pack, totals, metadata -- the new pair's tiles join the queue of
+// the engine's current iteration.
+static int admit(hdm_server *s, Request &r, int p) {
+  uint64_t inb = 0;
+  request_bytes(s, r, &inb);
+  char *base = (char *)s->arena_mem + r.off;
+  const void *gd = r.gold, *sd = r.segm;
+  if (r.ptr_kind == HDM_PTR_HOST) {
+    const uint64_t nb = (uint64_t)r.nx * r.ny * r.nz * (r.dtype == HDM_DTYPE_U8 ? 1 : 4);
+    r.copied = take_event(s);
+    if (!r.copied ||
+        hipMemcpyAsync(base, r.gold, nb, hipMemcpyHostToDevice, s->s_copy) != hipSuccess ||
+        hipMemcpyAsync(base + inb / 2, r.segm, nb, hipMemcpyHostToDevice, s->s_copy) !=
+            hipSuccess ||
+        hipEventRecord(r.copied, s->s_copy) != hipSuccess ||
+        hipStreamWaitEvent(s->s, r.copied, 0) != hipSuccess)
+      return HDM_E_HIP;
+    gd = base, sd = base + inb / 2;
+  }
+  PairDesc d{};
+  fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, s->t_engine);
+  carve_pair(d, r.geom, base + inb);
+  d.gen = (++s->slot_gen[p]) & kGenMask;  // drops queue entries of the slot's last owner
+  const PackIndex pi{0, 0};
+  hipError_t e = hdm_launch_reset(s->s, s->P.descs, &d, &pi, s->P.idx, s->P.ctrs, p, 1,
+                                  s->P.engine, 0, s->qcap);
+  if (e != hipSuccess) return map_hip(e);
+  const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1);
+  e = s->tl->pack(dim3(lc.pack_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
+                  r.geom.row_words);
+  if (e == hipSuccess) e = hdm_launch_totals(s->s, s->P.ctrs, p, 1);
+  if (e == hipSuccess)
+    e = s->tl->meta(dim3(lc.meta_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
+                    s->P.engine, s->P.q, s->qcap, r.geom.ntiles, p);
+  if (e != hipSuccess) return map_hip(e);
+  r.slot = p;
+  r.t_admit = now_s();
+  return HDM_OK;
+}
+
+extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done) {
+  if (!s || (!done && cap > 0) || !n_done) return HDM_E_INVALID;
+  *n_done = 0;
+  // admit in arrival order while a slot and arena room exist
+  int admitted = 0;
+  while (!s->pending.empty()) {
+    int p = -1;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+      if (s->slots[i].id == 0) {
+        p = i;
+        break;
+      }
+    if (p < 0) break;
+    Request &r = s->pending.front();
+    if (!s->arena.alloc(r.bytes, &r.off)) break;
+    const int rc = admit(s, r, p);
+    if (rc != HDM_OK) {
+      s->arena.release(r.off, r.bytes);
+      return rc;
+    }
+    s->slots[p] = r;
+    s->pending.pop_front();
+    ++admitted;
+  }
+  int busy = 0;
+  for (auto &r : s->slots) busy += r.id != 0;
+  if (busy == 0) return HDM_OK;
+  // one slice of the engine over every slot
+  const int nslots = (int)s->slots.size();
+  hipError_t e = s->tl->dilate(dim3(s->cfg.grid), s->s, s->P.descs, s->P.ctrs, nslots,
+                               s->P.engine, s->P.q, s->qcap, s->o.slice_iters, 1, s->cfg.spin,
+                               nullptr, 0, barrier_mode(), nullptr);
+  if (e == hipSuccess)
+    e = hdm_launch_finalize(s->s, s->P.descs, s->P.ctrs, s->P.engine, s->P.outs, 0, nslots);
+  char *h = (char *)s->host;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h, s->P.outs, sizeof(PairOut) * nslots, hipMemcpyDeviceToHost, s->s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h + sizeof(PairOut) * nslots, s->P.engine, kEngineHostBytes,
+                       hipMemcpyDeviceToHost, s->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+  if (e != hipSuccess) return map_hip(e);
+  Engine head{};
+  std::memcpy(&head, h + sizeof(PairOut) * nslots, kEngineHostBytes);
+  s->t_engine = head.t;
+  const double t_now = now_s();
+  for (int p = 0; p < nslots; ++p) {
+    Request &r = s->slots[p];
+    if (r.id == 0) continue;
+    PairOut o;
+    std::memcpy(&o, h + sizeof(PairOut) * p, sizeof(o));
+    int st = o.status;
+    bool finished = o.done || st != HDM_OK;
+    if (!finished && head.qnext == 0) {  // the queue drained with this pair unfinished
+      st = HDM_E_STUCK;
+      finished = true;
+    }
+    if (head.error) {
+      st = HDM_E_TIMEOUT;
+      finished = true;
+    }
+    if (!finished) continue;
+    if (*n_done < cap) {
+      hdm_completion &c = done[(*n_done)++];
+      std::memset(&c, 0, sizeof(c));
+      c.id = r.id;
+      c.status = st;
+      fill_result(&c.result, o, nullptr, r.geom.ntiles, std::max(o.hd_fp, o.hd_fn));
+      c.t_submit = r.t_submit;
+      c.t_admit = r.t_admit;
+      c.t_done = t_now;
+    } else {
+      continue;  // report it at the next step
+    }
+    if (r.copied) s->event_pool.push_back(r.copied);
+    s->arena.release(r.off, r.bytes);
+    r = Request();
+  }
+  (void)admitted;
+  return HDM_OK;
+}
+
+extern "C" int hdm_server_load(hdm_server *s, int *queued, int *in_flight, uint64_t *bytes) {
+  if (!s) return HDM_E_INVALID;
+  int busy = 0;
+  for (auto &r : s->slots) busy += r.id != 0;
+  if (queued) *queued = (int)s->pending.size();
+  if (in_flight) *in_flight = busy;
+  if (bytes) *bytes = s->arena.used;
+  return HDM_OK;
+}

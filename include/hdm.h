@@ -113,6 +113,14 @@ uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
  * stream: ms[0] pack (+ voxel totals), ms[1] tile metadata, ms[2] persistent
  * dilation, ms[3] finalize, ms[4] all kernels.  Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
+/* Per-iteration device trace of the last single-pair call on the current
+ * device that ran with opts->timing = 2 (observability; no counterpart in the
+ * reference, which has only printf): 8 words per iteration, wall-clock ticks
+ * [0] iteration start, [1] last workgroup done with its tiles, [2] workgroup 0
+ * done, [3] every workgroup arrived at the barrier, [4] counters folded,
+ * [5] queued (tile, mask) entries.  *tick_ns receives the tick length.
+ * Returns the number of iterations written (at most max_iters). */
+int hdm_last_trace(uint64_t *out, int max_iters, double *tick_ns);
 
 /* ---- single pair -------------------------------------------------------
  * Replaces: template<T> int getHausdorffDistance_CUDA_Generic(T* goldStandard,
@@ -192,11 +200,13 @@ typedef struct hdm_server hdm_server;
 
 typedef struct hdm_server_opts {
   uint64_t hbm_budget;   /* bytes of device memory for inputs + workspace   */
-  int32_t max_slots;     /* requests in flight at once                      */
+  int32_t max_slots;     /* requests in flight at once (<= 4096)            */
   int32_t slice_iters;   /* dilation iterations per persistent launch       */
   int32_t tile;          /* HDM_TILE_*                                      */
   int32_t grid_ctas;     /* 0 = occupancy-derived                           */
   double robustness;
+  int32_t max_nx, max_ny, max_nz; /* largest request dims (sizes the queue)  */
+  int32_t reserved;
 } hdm_server_opts;
 
 typedef struct hdm_completion {
@@ -209,10 +219,14 @@ typedef struct hdm_completion {
 
 void hdm_server_opts_init(hdm_server_opts *o);
 int hdm_server_create(const hdm_server_opts *o, hdm_server **out);
-/* Queue a request.  gold/segm are host pointers; the server copies them to
- * device memory it owns once the request is admitted. */
+/* Queue a request.  With ptr_kind HDM_PTR_DEVICE, gold/segm are resident
+ * device volumes (no copy); with HDM_PTR_HOST the server copies them on its
+ * copy stream once admitted (pinned host memory lets the copy overlap the
+ * engine).  The volumes must stay valid until the request completes.
+ * HDM_E_BUDGET if the request alone exceeds the HBM budget, HDM_E_CAPACITY
+ * if its dims exceed the server's max dims. */
 int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx, int ny,
-                      int nz, int dtype, int label, uint64_t *id_out);
+                      int nz, int dtype, int label, int ptr_kind, uint64_t *id_out);
 /* Drive the engine: admit, launch one slice, retire.  Fills up to cap
  * completions.  Returns HDM_OK. */
 int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done);

@@ -1,0 +1,163 @@
+"""Batched launches (hipGraph replays), the autotuner and the serving engine:
+every result must equal the same pair computed alone, bit for bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import housdorffhdf5cmake_amd as hdm
+from housdorffhdf5cmake_amd import serving
+from oracle import oracle
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+KEYS = ("hd", "sum", "count", "sum_fp", "cnt_fp", "sum_fn", "cnt_fn", "n_fp", "n_fn", "n_gold",
+        "n_segm", "hd_fp", "hd_fn")
+
+
+def same(a, b):
+    return all(a[k] == b[k] for k in KEYS)
+
+
+def mixed_pairs(n, seed=3):
+    rng = np.random.default_rng(seed)
+    dims = [(64, 64, 64), (128, 96, 64), (67, 45, 31), (256, 128, 64), (100, 100, 100)]
+    vols = []
+    for i in range(n):
+        nx, ny, nz = dims[int(rng.integers(0, len(dims)))]
+        sp = hdm.synth_spec("ellipsoids", nx, ny, nz, 6, 4, 20, 2, 0.01, 40 + i)
+        vols.append(hdm.synth_device(sp))
+    torch.cuda.synchronize()
+    return vols
+
+
+def test_batch_equals_single_and_oracle():
+    vols = mixed_pairs(12)
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    for tile in (hdm.TILE_64x8x8, hdm.TILE_64x4x4, hdm.TILE_32x8x8):
+        b = hdm.Batch(vols, tile=tile, graph=False)
+        b.run()
+        out = b.sync()
+        b.close()
+        for o, a in zip(out, alone):
+            assert same(o, a)
+    for (g, s), a in zip(vols[:4], alone[:4]):
+        rc, want, _, _, _ = oracle.run(g.cpu().numpy(), s.cpu().numpy())
+        assert rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
+
+
+def test_graph_replays_are_identical_with_eager_work_between():
+    vols = mixed_pairs(8, seed=5)
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    b = hdm.Batch(vols, graph=True)
+    for rep in range(4):
+        b.run()
+        out = b.sync()
+        for o, a in zip(out, alone):
+            assert same(o, a), rep
+        # eager engine work and synchronizations between the replays
+        g, s = vols[rep]
+        assert same(hdm.hausdorff(g, s), alone[rep])
+        torch.cuda.synchronize()
+    assert b.last_ms() > 0
+    b.close()
+
+
+def test_batch_with_empty_mask_pair_reports_error_for_it():
+    vols = mixed_pairs(3, seed=9)
+    g0 = torch.zeros((20, 30, 40), dtype=torch.uint8, device="cuda")
+    s0 = g0.clone()
+    s0[5, 5, 5] = 1
+    b = hdm.Batch(vols + [(g0, s0)], graph=False)
+    b.run()
+    with pytest.raises(hdm.HDMError) as e:
+        b.sync()
+    assert e.value.code == -2
+    b.close()
+
+
+def test_autotune_table_consistent():
+    vols = mixed_pairs(6, seed=7)
+    table = hdm.autotune(vols, tiles=(hdm.TILE_64x8x8, hdm.TILE_64x8x4), chunks=(1, 2),
+                         grids=(0, 256), reps=2)
+    assert len(table) == 8
+    assert all(e["status"] == 0 for e in table)
+    ms = [e["ms"] for e in table]
+    assert ms == sorted(ms) and ms[0] > 0
+
+
+def test_server_mix_equals_alone():
+    vols = mixed_pairs(10, seed=13)
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    srv = serving.Server(hbm_budget=4 << 30, max_slots=4, slice_iters=2, max_dims=(256, 128, 100))
+    ids = {srv.submit(g, s): i for i, (g, s) in enumerate(vols)}
+    got = {}
+    for c in srv.drain(60):
+        got[ids[c["id"]]] = c
+    assert len(got) == len(vols)
+    for i, c in got.items():
+        assert c["status"] == 0 and same(c, alone[i]), i
+    # waves: new requests arrive while earlier ones are in flight
+    ids = {}
+    done = []
+    for i, (g, s) in enumerate(vols):
+        ids[srv.submit(g, s)] = i
+        done += srv.step()
+    done += srv.drain(60)
+    assert len(done) == len(vols)
+    for c in done:
+        assert c["status"] == 0 and same(c, alone[ids[c["id"]]])
+    srv.close()
+
+
+def test_server_errors_budget_and_capacity():
+    vols = mixed_pairs(3, seed=17)
+    g0 = torch.zeros((16, 16, 64), dtype=torch.uint8, device="cuda")
+    s0 = g0.clone()
+    s0[3, 3, 3] = 1
+    srv = serving.Server(hbm_budget=64 << 20, max_slots=2, slice_iters=4, max_dims=(256, 128, 100))
+    ids = {srv.submit(g, s): i for i, (g, s) in enumerate(vols)}
+    bad = srv.submit(g0, s0)
+    res = {c["id"]: c for c in srv.drain(60)}
+    assert res[bad]["status"] == -2
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    for rid, i in ids.items():
+        assert res[rid]["status"] == 0 and same(res[rid], alone[i])
+    big = torch.zeros((100, 128, 300), dtype=torch.uint8, device="cuda")
+    with pytest.raises(hdm.HDMError) as e:
+        srv.submit(big, big)
+    assert e.value.code == -9  # dims beyond the server's max
+    srv.close()
+    tiny = serving.Server(hbm_budget=2 << 20, max_slots=2, max_dims=(256, 256, 256))
+    g, s = mixed_pairs(1, seed=19)[0]
+    big = torch.zeros((200, 200, 200), dtype=torch.uint8, device="cuda")
+    with pytest.raises(hdm.HDMError) as e:
+        tiny.submit(big, big)
+    assert e.value.code == -6  # larger than the whole budget
+    tiny.close()
+
+
+def test_server_host_pointer_requests():
+    vols = mixed_pairs(4, seed=23)
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    host = [(g.cpu().numpy(), s.cpu().numpy()) for g, s in vols]
+    srv = serving.Server(hbm_budget=1 << 30, max_slots=3, slice_iters=3, max_dims=(256, 128, 100))
+    ids = {srv.submit(g, s): i for i, (g, s) in enumerate(host)}
+    res = srv.drain(60)
+    assert len(res) == 4
+    for c in res:
+        assert c["status"] == 0 and same(c, alone[ids[c["id"]]])
+    srv.close()
+
+
+def test_poisson_serving_short_run():
+    def verify(g, s, a):
+        rc, want, _, _, _ = oracle.run(g.cpu().numpy(), s.cpu().numpy())
+        return rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
+
+    r = serving.poisson_run(seconds=1.5, rate=100, budget_bytes=8 << 30, verify=verify,
+                            per_class=2, max_slots=16)
+    assert r["completed"] == r["requests"] > 0
+    assert r["identical_alone_vs_mix"] and r["bit_exact_vs_cpu"]
+    assert r["p99_ms"] >= r["p50_ms"] > 0
