@@ -1119,7 +1119,7 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
   hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(256), 0, s, descs, one, one_idx, idx,
                      single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || npairs < 1) return e;
   hipLaunchKernelGGL(hdm::clear_hist_kernel, dim3(4, std::min(npairs, 1024)), dim3(256), 0, s,
                      descs, p0, npairs);
   return hipGetLastError();

@@ -103,6 +103,12 @@ struct hdm_server {
   std::deque<Request> pending;
   std::vector<Request> slots;  // slot index -> request (id 0 = free)
   std::vector<uint32_t> slot_gen;
+  // A slot freed after slice k is reused only after slice k+1 ran: that slice
+  // drops the queue entries its last owner left (stale generation), so one
+  // slot never holds a pair's entries and a predecessor's at once and the
+  // queue stays within 2 * tiles(max dims) per slot.
+  std::vector<uint64_t> freed_after;
+  uint64_t slices = 0;
   std::vector<hipEvent_t> event_pool;
   void *host = nullptr;  // pinned: outs + engine head
   uint64_t next_id = 1;
@@ -195,6 +201,8 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   s->P = carve_common((char *)s->common, L);
   s->slots.resize(o.max_slots);
   s->slot_gen.assign(o.max_slots, 0);
+  s->freed_after.assign(o.max_slots, 0);
+  s->slices = 1;
   s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1);
   // every slot starts unused: t_start beyond any engine iteration keeps it out
   // of the per-iteration bookkeeping
@@ -299,12 +307,16 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   *n_done = 0;
   // admit in arrival order while a slot and arena room exist
   int admitted = 0;
+  bool quarantined = false;
   while (!s->pending.empty()) {
     int p = -1;
     for (int i = 0; i < (int)s->slots.size(); ++i)
       if (s->slots[i].id == 0) {
-        p = i;
-        break;
+        if (s->freed_after[i] < s->slices) {
+          p = i;
+          break;
+        }
+        quarantined = true;
       }
     if (p < 0) break;
     Request &r = s->pending.front();
@@ -320,9 +332,12 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   }
   int busy = 0;
   for (auto &r : s->slots) busy += r.id != 0;
-  if (busy == 0) return HDM_OK;
+  // with nothing in flight a slice still runs when requests wait on
+  // quarantined slots: it drops the stale entries and frees them
+  if (busy == 0 && !(quarantined && !s->pending.empty())) return HDM_OK;
   // one slice of the engine over every slot
   const int nslots = (int)s->slots.size();
+  ++s->slices;
   hipError_t e = s->tl->dilate(dim3(s->cfg.grid), s->s, s->P.descs, s->P.ctrs, nslots,
                                s->P.engine, s->P.q, s->qcap, s->o.slice_iters, 1, s->cfg.spin,
                                nullptr, 0, barrier_mode(), nullptr);
@@ -371,6 +386,7 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
     if (r.copied) s->event_pool.push_back(r.copied);
     s->arena.release(r.off, r.bytes);
     r = Request();
+    s->freed_after[p] = s->slices;
   }
   (void)admitted;
   return HDM_OK;
