@@ -185,6 +185,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain device-to-device copy of MIB MiB three times "
+                         "(the known-size kernel that scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine is single-device (a pair or a batch per "
@@ -210,6 +213,13 @@ def main():
         "phase_ms": {k: round(v, 4) for k, v in avg.items()},
     }
     line["roofline"] = roofline(res, avg, nvox)
+    if args.copy_probe > 0:
+        a = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if not args.no_batch:
         log("batched vs plain ...")
         line["batched"] = leg_batched(hdm, torch, args.batch_pairs, args.batch_reps,
