@@ -93,27 +93,105 @@ def roofline(res, avg, nvox, tile=1):
     return out
 
 
-def leg_cpu(hdm, G, S, res, seconds):
-    """The CPU oracle (same tiled algorithm, OpenMP over tiles) on this host."""
-    from oracle import oracle
-    g, s = G.cpu().numpy(), S.cpu().numpy()
-    threads = min(os.cpu_count() or 1, 16)
+def _oracle_times(oracle, g, s, threads, seconds, min_runs=5):
+    """One warm-up run, then runs until `seconds` have passed and at least
+    min_runs were timed; returns (median seconds, runs, total seconds, result)."""
+    rc, want, _, _, _ = oracle.run(g, s, threads=threads)
     times = []
     t_end = time.perf_counter() + seconds
-    want = None
-    while time.perf_counter() < t_end or len(times) < 3:
+    while time.perf_counter() < t_end or len(times) < min_runs:
         t0 = time.perf_counter()
         rc, want, _, _, _ = oracle.run(g, s, threads=threads)
         times.append(time.perf_counter() - t0)
-        if len(times) >= 200:
+        if len(times) >= 2000:
             break
-    exact = rc == 0 and all(int(want[k]) == int(res[k]) for k in
-                            ("hd", "sum", "count", "sum_fp", "cnt_fp", "sum_fn", "cnt_fn"))
-    med = statistics.median(times)
+    return statistics.median(times), len(times), sum(times), (rc, want)
+
+
+EXACT_KEYS = ("hd", "sum", "count", "sum_fp", "cnt_fp", "sum_fn", "cnt_fn", "n_fp", "n_fn")
+
+
+def _same(rcw, res):
+    rc, want = rcw
+    return rc == 0 and all(int(want[k]) == int(res[k]) for k in EXACT_KEYS)
+
+
+def leg_cpu(hdm, G, S, res, seconds, extra=()):
+    """The CPU oracle (the same tiled dilation, OpenMP over tiles) on this
+    host's cores, single-core and all-core, bit-exact against the GPU; `extra`
+    is a list of (name, gold, segm, gpu result) also checked bit for bit."""
+    from oracle import oracle
+    g, s = G.cpu().numpy(), S.cpu().numpy()
+    threads = min(os.cpu_count() or 1, 16)
+    med, runs, tot, rcw = _oracle_times(oracle, g, s, threads, seconds)
+    med1, runs1, tot1, rcw1 = _oracle_times(oracle, g, s, 1, seconds / 2, min_runs=3)
+    checks = {}
+    for name, eg, es, eres in extra:
+        rc, want, _, _, _ = oracle.run(eg.cpu().numpy(), es.cpu().numpy(), threads=threads)
+        checks[name] = checks.get(name, True) and _same((rc, want), eres)
     return {"value": round(2 * g.size / med, 1), "unit": "mask-voxels/s", "cores": threads,
             "kind": "port", "ms_median": round(med * 1e3, 3),
-            "sample": f"the same 512x512x256 pair, {len(times)} runs of the tiled oracle over "
-                      f"{sum(times):.1f} s, median", "bit_exact_vs_gpu": bool(exact)}
+            "single_core_ms_median": round(med1 * 1e3, 3),
+            "single_core_value": round(2 * g.size / med1, 1),
+            "sample": f"the same 512x512x256 pair: {runs} runs of the tiled oracle on {threads} "
+                      f"threads over {tot:.1f} s and {runs1} runs on 1 thread over {tot1:.1f} s, "
+                      f"medians after one warm-up each",
+            "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "bit_exact_vs_gpu": bool(_same(rcw, res) and _same(rcw1, res)),
+            "also_bit_exact": checks}
+
+
+def leg_small(hdm, torch, steps):
+    """64^3 sphere r 20, segm shifted (2,1,0) with 1% surface flips."""
+    sp = hdm.synth_spec("sphere", 64, 64, 64, 32, 32, 32, 20, 2, 1, 0, 0.01, 1)
+    G, S = hdm.synth_device(sp)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        res = hdm.hausdorff(G, S)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = hdm.hausdorff(G, S)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return G, S, res, {"workload": "sphere_64x64x64_r20_shift_2_1_0", "hd": res["hd"],
+                       "avg_hd": res["avg_hd"], "ms_per_pair": round(dt * 1e3, 4),
+                       "pairs_per_s": round(1 / dt, 1), "mask_voxels_per_s": round(2 * G.numel() / dt, 1)}
+
+
+def leg_long(hdm, torch, reps):
+    """512^3: one ellipsoid (80,60,50) in both masks plus a 5^3 cube 300 voxels
+    beyond it in segm only, about 300 iterations with a sparse queue."""
+    sp = hdm.synth_spec("long", 512, 512, 512, 100, 256, 256, 80, 60, 50, 300, 5)
+    G, S = hdm.synth_device(sp)
+    torch.cuda.synchronize()
+    res = hdm.hausdorff(G, S, timing=True)
+    torch.cuda.synchronize()
+    phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = hdm.hausdorff(G, S, timing=True)
+        for k, v in hdm.last_timings().items():
+            phases[k].append(v)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    avg = {k: sum(v) / len(v) for k, v in phases.items()}
+    hdm.hausdorff(G, S, timing=2)
+    tr = hdm.last_trace()
+    occ = [r["queued"] for r in tr]
+    it_us = [r["next_start_us"] for r in tr if r["next_start_us"]]
+    dil_bytes = float(res["queue_total"]) * TILE_BYTES[1]
+    return G, S, res, {
+        "workload": "long_512x512x512_ellipsoid_80_60_50_cube5_gap300", "hd": res["hd"],
+        "avg_hd": res["avg_hd"], "iterations": res["iterations"],
+        "ms_per_pair": round(dt * 1e3, 3), "mask_voxels_per_s": round(2 * G.numel() / dt, 1),
+        "phase_ms": {k: round(v, 4) for k, v in avg.items()},
+        "queue_total": res["queue_total"], "queue_max": res["queue_max"],
+        "queue_per_iteration": {"min": min(occ), "median": statistics.median(occ),
+                                "max": max(occ), "first8": occ[:8], "last8": occ[-8:]},
+        "us_per_iteration_median": round(statistics.median(it_us), 2) if it_us else None,
+        "dilate_algo_bytes": int(dil_bytes),
+        "dilate_gbs": round(dil_bytes / (avg["dilate"] * 1e-3) / 1e9, 1)}
 
 
 def leg_batched(hdm, torch, n_pairs, reps, tune):
@@ -167,9 +245,11 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
 
 
 def leg_serving(hdm, torch, seconds, rate, budget_gb):
+    """Returns the summary and the pool pairs with their results computed
+    alone (every served result equalled those; the CPU leg checks them)."""
     from housdorffhdf5cmake_amd import serving
     return serving.poisson_run(seconds=seconds, rate=rate, budget_bytes=int(budget_gb * (1 << 30)),
-                               verify=None)
+                               verify=None, return_pool=True)
 
 
 def main():
@@ -181,9 +261,12 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 64^3 and the long-iteration 512^3 lines")
+    ap.add_argument("--long-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device-to-device copy of MIB MiB three times "
@@ -220,15 +303,28 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    extra = []
+    if not args.no_extra:
+        log("small 64^3 ...")
+        g1, s1, r1, line["small_64"] = leg_small(hdm, torch, 200)
+        extra.append(("small_64", g1, s1, r1))
+        log("long-iteration 512^3 ...")
+        g3, s3, r3, line["long_512"] = leg_long(hdm, torch, args.long_reps)
+        extra.append(("long_512", g3, s3, r3))
     if not args.no_batch:
         log("batched vs plain ...")
         line["batched"] = leg_batched(hdm, torch, args.batch_pairs, args.batch_reps,
                                       not args.no_tune)
     if not args.no_serve:
         log("serving ...")
-        line["serving"] = leg_serving(hdm, torch, args.serve_seconds, args.serve_rate, 64.0)
+        line["serving"], pool = leg_serving(hdm, torch, args.serve_seconds, args.serve_rate,
+                                            64.0)
+        extra += [("serving_pool", g, s, a) for g, s, a in pool]
     log("cpu baseline ...")
-    line["cpu_baseline"] = leg_cpu(hdm, G, S, res, args.cpu_seconds)
+    line["cpu_baseline"] = leg_cpu(hdm, G, S, res, args.cpu_seconds, extra)
+    if "serving" in line:
+        line["serving"]["bit_exact_vs_cpu"] = line["cpu_baseline"]["also_bit_exact"].get(
+            "serving_pool")
     print(json.dumps(line), flush=True)
 
 

@@ -105,9 +105,10 @@ def make_pool(per_class=4, seed=500):
 
 
 def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
-                per_class=4, max_slots=64, slice_iters=8, seed=11):
+                per_class=4, max_slots=64, slice_iters=8, seed=11, return_pool=False):
     """verify (optional): callable(gold, segm, result) -> bool, an independent
-    check of each pool pair's result computed alone (the caller's checker)."""
+    check of each pool pair's result computed alone (the caller's checker).
+    With return_pool, returns (summary, [(gold, segm, result alone)])."""
     import torch
     pool = make_pool(per_class)
     torch.cuda.synchronize()
@@ -127,6 +128,7 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
     lat, waits, ok = [], [], True
     id2req = {}
     done_n, nvox_done, max_in_flight, i = 0, 0, 0, 0
+    batch_sizes = []
     t0 = now()
     t_last = t0
     while True:
@@ -135,6 +137,9 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
             _, g, s = pool[picks[i]]
             id2req[srv.submit(g, s)] = i
             i += 1
+        q, f, _ = srv.load()
+        if q + f:  # requests this step admits or keeps running (slots permitting)
+            batch_sizes.append(min(q + f, max_slots))
         for c in srv.step():
             r = id2req.pop(c["id"])
             a = alone[picks[r]]
@@ -153,7 +158,8 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
     srv.close()
     span = max(t_last - t0, 1e-9)
     lat_ms = np.array(lat) * 1e3
-    return {"requests": n, "completed": done_n, "seconds": seconds, "rate_per_s": rate,
+    bs = np.array(batch_sizes or [0])
+    summary = {"requests": n, "completed": done_n, "seconds": seconds, "rate_per_s": rate,
             "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "mean_ms": round(float(lat_ms.mean()), 3),
@@ -161,6 +167,11 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
             "pairs_per_s": round(done_n / span, 1),
             "mask_voxels_per_s": round(nvox_done / span, 1),
             "max_in_flight": max_in_flight, "max_slots": max_slots,
+            "steps_with_work": int(len(batch_sizes)), "batch_mean": round(float(bs.mean()), 2),
+            "batch_max": int(bs.max()), "steps_with_2plus": int((bs >= 2).sum()),
             "hbm_budget_gb": budget_bytes / (1 << 30),
             "identical_alone_vs_mix": bool(ok), "bit_exact_vs_cpu": exact_cpu,
             "pool": f"{per_class} pairs per volume class, device resident"}
+    if return_pool:
+        return summary, [(g, s, a) for (_, g, s), a in zip(pool, alone)]
+    return summary

@@ -137,7 +137,8 @@ int hdm_hausdorff_i32(const int32_t *gold, const int32_t *segm, int nx, int ny, 
 
 /* Replaces: int HausdorffDistance::computeDistance(Volume* img1, Volume* img2)
  * (forBench/HausdorffDistance.cuh:17, .cu:101-187) for bool volumes
- * (forBench/Volume.h:10).  A voxel is on iff its byte is non-zero.  Uses
+ * (forBench/Volume.h:10).  A voxel is on iff its byte is 1 (bool storage
+ * holds 0 or 1; the reference allocates the volumes with calloc).  Uses
  * 6-connectivity (the reference's CHEBYSHEV 1 18-neighbour element is a
  * different metric). */
 int hdm_hausdorff_bool(const uint8_t *img1, const uint8_t *img2, int nx, int ny, int nz,

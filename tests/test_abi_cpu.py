@@ -88,3 +88,33 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     o.robustness = 1.5
     assert L.hdm_hausdorff_u8(p, p, 4, 4, 4, 1, C.byref(o), C.byref(r)) == -1
     assert L.hdm_batch_create(None, 0, None, 0, None) == -1
+
+
+def test_cpp_compat_header_compiles_and_reports_errors(tmp_path):
+    """include/hdm_compat.hpp: the reference's template signature over the C
+    ABI.  Built with g++ and linked against libhdm.so; the null-pointer call
+    is rejected on the host (HDM_E_INVALID) without touching a device."""
+    src = tmp_path / "compat.cpp"
+    src.write_text(r'''
+#include "hdm_compat.hpp"
+#include <cstdio>
+int main() {
+  try {
+    hdm_compat::getHausdorffDistance_CUDA_Generic<uint8_t>(nullptr, nullptr, 8, 8, 8, 1.0f,
+                                                           false, (uint8_t)1, false);
+  } catch (const hdm_compat::error &e) {
+    std::printf("%d\n", e.code);
+    return e.code == HDM_E_INVALID ? 0 : 2;
+  }
+  return 1;
+}
+''')
+    pkg = os.path.join(ROOT, "housdorffhdf5cmake_amd")
+    exe = tmp_path / "compat"
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-o", str(exe), "-L", pkg, "-lhdm", f"-Wl,-rpath,{pkg}"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.strip() == str(-1), (r.stdout, r.stderr[-500:])
