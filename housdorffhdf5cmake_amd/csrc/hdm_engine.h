@@ -76,13 +76,18 @@ struct PairDesc {
 struct PairCtr {
   unsigned long long n_gold, n_segm, n_fp, n_fn;  // totals (written before the dilation)
   unsigned long long cov[2];                      // covered so far; barrier master only
+  // Sum of first-hit iterations of the covered voxels and the last iteration
+  // with a hit, per pass; the master keeps them with cov while it folds, so
+  // they cover exactly the histogram bins up to the pass's stop.
+  unsigned long long sum_d[2];
   int32_t lo[3], hi[3];  // tile bounding box of non-empty tiles (inclusive)
   int32_t status;
   // First pair-local iteration at which pass m has nothing left to do; written
   // by the barrier master between iterations, so every workgroup reads the
   // same value during an iteration.
   int32_t done_from[2];
-  int32_t pad[3];
+  uint32_t hd_last[2];
+  int32_t pad;
   unsigned long long hits[2][kHitSlots];        // cumulative hits, spread over slots
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
 };
@@ -95,8 +100,21 @@ struct PairOut {
   int32_t pad;
 };
 
+// The engine fields the host reads after a launch; the dilation kernel writes
+// a copy next to the pair results when it leaves.
+struct EngineHead {
+  uint32_t t;      // completed engine iterations
+  uint32_t error;  // 1: a barrier wait exceeded its limit
+  uint32_t qmax;
+  uint32_t nwg;
+  unsigned long long qtotal;
+  uint32_t qcap;
+  uint32_t qnext;  // entries queued for the next iteration when the kernel left
+  uint32_t pad1[8];
+};
+
 struct Engine {
-  // head, copied back to the host (kEngineHostBytes)
+  // head (EngineHead layout)
   uint32_t t;      // completed engine iterations
   uint32_t error;  // 1: a barrier wait exceeded its limit
   uint32_t qmax;
@@ -114,7 +132,8 @@ struct Engine {
   // the iteration the epoch starts.  Workgroup b polls replica b % kReleaseCopies.
   uint32_t rel[kReleaseCopies][kLineWords];
 };
-constexpr size_t kEngineHostBytes = 64;
+constexpr size_t kEngineHostBytes = sizeof(EngineHead);
+static_assert(sizeof(EngineHead) == 64, "engine head is one 64-byte block");
 
 // ceil(r * n) clamped to [0, n]; the robust cut (old/testMainPassesKernels.cu:81-82)
 __host__ __device__ inline unsigned long long robust_cut(double r, unsigned long long n) {

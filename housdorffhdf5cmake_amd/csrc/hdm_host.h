@@ -42,8 +42,10 @@ struct LaunchCfg {
   unsigned pack_blocks = 1, meta_blocks = 1;
   int grid = 1, chunk = 1;
   uint64_t spin = 0;
-  const hipEvent_t *ev = nullptr;  // 5 events or null
+  const hipEvent_t *ev = nullptr;  // 4 events (phase bounds) or null
   uint64_t *trace = nullptr;       // per-iteration device trace or null
+  PairOut *outs = nullptr;         // where the results go (null: the group's outs)
+  EngineHead *head = nullptr;      // where the engine head goes (null: nowhere)
 };
 
 struct GroupPtrs {
@@ -69,7 +71,7 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
                          const LaunchCfg &cfg, int qstats_len);
 LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
                      int grid_req, int chunk);
-void fill_result(hdm_result *out, const PairOut &o, const Engine *es, uint32_t tiles,
+void fill_result(hdm_result *out, const PairOut &o, const EngineHead *es, uint32_t tiles,
                  int iterations);
 
 }  // namespace hdm

@@ -1,8 +1,8 @@
 // Host driver of the Hausdorff dilation engine: the C ABI of include/hdm.h.
 //
 // One call of the single-pair entry points enqueues, on one stream:
-//   reset -> clear histograms -> pack -> tile metadata -> persistent dilation
-//   -> finalize, then one small device-to-host copy of the result.
+//   reset -> pack -> totals -> tile metadata -> persistent dilation, whose
+//   workgroup 0 writes the result into pinned host memory as it leaves.
 // There are no host round trips between the kernels (the reference reads its
 // counters back twice mid-pipeline to size allocations, hello.cu:952, 1012):
 // the workspace is sized up front from the dims, grows only, and is reused.
@@ -102,7 +102,10 @@ int get_ctx(Ctx **out) {
       rc = HDM_E_HIP;
     c->occ_pack[k] = nb;
   }
-  if (rc == HDM_OK && hipHostMalloc(&c->host_out, 4096, hipHostMallocDefault) != hipSuccess)
+  // the dilation kernel stores the result and the engine head here directly:
+  // coherent (fine-grained) host memory, so the stores are seen by the host
+  // once the kernel has completed
+  if (rc == HDM_OK && hipHostMalloc(&c->host_out, 4096, hipHostMallocCoherent) != hipSuccess)
     rc = HDM_E_HIP;
   for (int i = 0; rc == HDM_OK && i < 5; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = HDM_E_HIP;
@@ -175,13 +178,25 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   e = tl->meta(dim3(cfg.meta_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,
                total_tiles, 0);
   if (e != hipSuccess || (e = mark(2)) != hipSuccess) return e;
-  const int flags = barrier_mode() | (npairs == 1 ? 4 : 0);
-  e = tl->dilate(dim3(cfg.grid), s, P.descs, P.ctrs, npairs, P.engine, P.q, qcap, INT_MAX,
-                 cfg.chunk, cfg.spin, P.qstats, qstats_len, flags, cfg.trace);
-  if (e != hipSuccess || (e = mark(3)) != hipSuccess) return e;
-  e = hdm_launch_finalize(s, P.descs, P.ctrs, P.engine, P.outs, 0, npairs);
+  DilateArgs a{};
+  a.descs = P.descs;
+  a.ctrs = P.ctrs;
+  a.npairs = npairs;
+  a.max_iters = INT_MAX;
+  a.es = P.engine;
+  a.q = P.q;
+  a.qcap = qcap;
+  a.chunk = cfg.chunk;
+  a.spin = cfg.spin;
+  a.qstats = P.qstats;
+  a.qstats_len = qstats_len;
+  a.flags = barrier_mode();
+  a.trace = cfg.trace;
+  a.outs = cfg.outs ? cfg.outs : P.outs;
+  a.head = cfg.head;
+  e = tl->dilate(dim3(cfg.grid), s, a);
   if (e != hipSuccess) return e;
-  return mark(4);
+  return mark(3);
 }
 
 GroupPtrs carve_common(char *base, const CommonLayout &L) {
@@ -196,7 +211,7 @@ GroupPtrs carve_common(char *base, const CommonLayout &L) {
   return P;
 }
 
-void fill_result(hdm_result *out, const PairOut &o, const Engine *es, uint32_t tiles,
+void fill_result(hdm_result *out, const PairOut &o, const EngineHead *es, uint32_t tiles,
                  int iterations) {
   std::memset(out, 0, sizeof(*out));
   out->hd = o.hd, out->hd_fp = o.hd_fp, out->hd_fn = o.hd_fn;
@@ -324,13 +339,17 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     if (hipMemsetAsync(c->trace.p, 0, trace_bytes, s) != hipSuccess) return HDM_E_HIP;
     cfg.trace = (uint64_t *)c->trace.p;
   }
+  // the dilation kernel writes the result and the engine head straight into
+  // the pinned host block; a status the kernel never writes marks "not run"
+  char *ho = (char *)c->host_out;
+  PairOut *hout = (PairOut *)ho;
+  EngineHead *hhead = (EngineHead *)(ho + 256);
+  hout->status = 1;
+  cfg.outs = hout;
+  cfg.head = hhead;
   hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, qcap, cfg,
                                g.hist_len);
   if (e != hipSuccess) return map_hip(e);
-  char *ho = (char *)c->host_out;
-  if (hipMemcpyAsync(ho, P.outs, sizeof(PairOut), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemcpyAsync(ho + 256, P.engine, kEngineHostBytes, hipMemcpyDeviceToHost, s) != hipSuccess)
-    return HDM_E_HIP;
   if (map_out && opts.ptr_kind == HDM_PTR_HOST &&
       hipMemcpyAsync(map_out, d.map, nvox * 2, hipMemcpyDeviceToHost, s) != hipSuccess)
     return HDM_E_HIP;
@@ -342,29 +361,30 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
       return HDM_E_HIP;
   }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip(e);
+  PairOut o;
+  EngineHead es{};
+  std::memcpy(&o, hout, sizeof(o));
+  std::memcpy(&es, hhead, sizeof(es));
+  if (o.status == 1) return HDM_E_HIP;
   if (opts.timing) {
-    float ms[4] = {0, 0, 0, 0};
-    for (int i = 0; i < 4; ++i)
+    // phases: pack (+ reset), metadata, dilation (+ finalize); total
+    float ms[3] = {0, 0, 0};
+    for (int i = 0; i < 3; ++i)
       if (hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]) != hipSuccess) ms[i] = -1.f;
     float all = 0.f;
-    if (hipEventElapsedTime(&all, c->ev[0], c->ev[4]) != hipSuccess) all = -1.f;
-    for (int i = 0; i < 4; ++i) c->last_ms[i] = ms[i];
+    if (hipEventElapsedTime(&all, c->ev[0], c->ev[3]) != hipSuccess) all = -1.f;
+    for (int i = 0; i < 3; ++i) c->last_ms[i] = ms[i];
+    c->last_ms[3] = 0.f;
     c->last_ms[4] = all;
     (void)hipGetLastError();
   }
   if (opts.timing == 2) {
-    Engine head{};
-    std::memcpy(&head, ho + 256, kEngineHostBytes);
-    const size_t nrec = std::min<size_t>(head.t, (size_t)g.hist_len);
+    const size_t nrec = std::min<size_t>(es.t, (size_t)g.hist_len);
     c->last_trace.assign(nrec * 8, 0);
     if (nrec && hipMemcpy(c->last_trace.data(), c->trace.p, nrec * 8 * sizeof(uint64_t),
                           hipMemcpyDeviceToHost) != hipSuccess)
       return HDM_E_HIP;
   }
-  PairOut o;
-  Engine es{};
-  std::memcpy(&o, ho, sizeof(o));
-  std::memcpy(&es, ho + 256, kEngineHostBytes);
   fill_result(out, o, &es, g.ntiles, (int)es.t);
   if (o.status == HDM_OK && !o.done) return HDM_E_STUCK;  // queue drained first
   return o.status;
@@ -557,7 +577,7 @@ extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
       hipMemcpy((char *)b->host + sizeof(PairOut) * b->n, b->P.engine, kEngineHostBytes,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return HDM_E_HIP;
-  Engine es{};
+  EngineHead es{};
   std::memcpy(&es, (char *)b->host + sizeof(PairOut) * b->n, kEngineHostBytes);
   int rc = HDM_OK;
   for (int i = 0; i < b->n; ++i) {

@@ -13,8 +13,8 @@
 //                    firstMetaPrepareKernel, hello.cu:537-748, 762-877)
 //   dilate_kernel    persistent kernel: one grid barrier per iteration, a
 //                    queue of the tiles still changing (mainPassKernel,
-//                    hello.cu:1066-1896)
-//   finalize_kernel  HD, sums and counts per pair from the histograms
+//                    hello.cu:1066-1896); when it leaves, its workgroup 0
+//                    writes every pair's HD, sums and counts (finalize_pair)
 //
 // Same-address atomics serialize (tens of ns each on MI355X), so every hot
 // counter is spread: partial voxel counts over kSumSlots, per-iteration hits
@@ -125,17 +125,20 @@ __device__ __forceinline__ uint64_t *sub_queue(uint64_t *q, uint32_t qcap, uint3
 }
 
 // ---------------------------------------------------------------- reset
-__global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_idx,
-                             PackIndex *idx, int write_single, PairCtr *ctrs, int p0, int npairs,
-                             Engine *es, int reset_engine, uint32_t qcap) {
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int stride = gridDim.x * blockDim.x;
-  if (write_single && gid == 0) {
+// Blocks stride over the pairs: each zeroes a pair's counters and histograms
+// with all its threads.  Block 0 also resets the engine, and stores the
+// descriptor of a single-pair call (passed by value, so this launch reads the
+// histogram pointers from the argument, not from the descriptor it writes).
+__global__ __launch_bounds__(kWgThreads) void reset_kernel(
+    PairDesc *descs, PairDesc single, PackIndex single_idx, PackIndex *idx, int write_single,
+    PairCtr *ctrs, int p0, int npairs, Engine *es, int reset_engine, uint32_t qcap) {
+  const int tid = threadIdx.x;
+  if (write_single && blockIdx.x == 0 && tid == 0) {
     descs[p0] = single;
     idx[p0] = single_idx;
   }
-  if (reset_engine) {
-    if (gid == 0) {
+  if (reset_engine && blockIdx.x == 0) {
+    if (tid == 0) {
       es->t = 0;
       es->error = 0;
       es->qmax = 0;
@@ -143,37 +146,31 @@ __global__ void reset_kernel(PairDesc *descs, PairDesc single, PackIndex single_
       es->qcap = qcap;
       es->qnext = 0;
     }
-    for (int i = gid; i < kLineWords; i += stride) {
+    for (int i = tid; i < kLineWords; i += blockDim.x) {
       es->gen[i] = 0;
       es->qlen[0][i] = es->qlen[1][i] = es->qlen[2][i] = 0;
     }
-    for (int i = gid; i < kMaxGrid; i += stride) es->arrive[i] = 0;
-    for (int i = gid; i < kReleaseCopies * kLineWords; i += stride) es->rel[0][i] = 0;
+    for (int i = tid; i < kMaxGrid; i += blockDim.x) es->arrive[i] = 0;
+    for (int i = tid; i < kReleaseCopies * kLineWords; i += blockDim.x) es->rel[0][i] = 0;
   }
-  for (int p = p0 + gid; p < p0 + npairs; p += stride) {
+  for (int p = p0 + blockIdx.x; p < p0 + npairs; p += gridDim.x) {
     PairCtr &c = ctrs[p];
-    c.n_gold = c.n_segm = c.n_fp = c.n_fn = 0;
-    c.cov[0] = c.cov[1] = 0;
-    c.lo[0] = c.lo[1] = c.lo[2] = 0x7FFFFFFF;
-    c.hi[0] = c.hi[1] = c.hi[2] = -1;
-    c.status = 0;
-    c.done_from[0] = c.done_from[1] = 0x7FFFFFFF;
-    for (int m = 0; m < 2; ++m)
-      for (int s = 0; s < kHitSlots; ++s) c.hits[m][s] = 0;
-    for (int f = 0; f < 4; ++f)
-      for (int s = 0; s < kSumSlots; ++s) c.sums[f][s] = 0;
-  }
-}
-
-// zero the histograms (needs the descriptors, so after reset_kernel)
-__global__ void clear_hist_kernel(const PairDesc *__restrict__ descs, int p0, int npairs) {
-  for (int p = p0 + blockIdx.y; p < p0 + npairs; p += gridDim.y) {
-    const PairDesc &d = descs[p];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.hist_len;
-         i += gridDim.x * blockDim.x) {
-      d.hist[0][i] = 0;
-      d.hist[1][i] = 0;
+    if (tid == 0) {
+      c.n_gold = c.n_segm = c.n_fp = c.n_fn = 0;
+      c.cov[0] = c.cov[1] = 0;
+      c.sum_d[0] = c.sum_d[1] = 0;
+      c.lo[0] = c.lo[1] = c.lo[2] = 0x7FFFFFFF;
+      c.hi[0] = c.hi[1] = c.hi[2] = -1;
+      c.status = 0;
+      c.done_from[0] = c.done_from[1] = 0x7FFFFFFF;
+      c.hd_last[0] = c.hd_last[1] = 0;
     }
+    for (int i = tid; i < 2 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
+    for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) (&c.sums[0][0])[i] = 0;
+    uint32_t *h0 = write_single ? single.hist[0] : descs[p].hist[0];
+    uint32_t *h1 = write_single ? single.hist[1] : descs[p].hist[1];
+    const int len = write_single ? single.hist_len : descs[p].hist_len;
+    for (int i = tid; i < len; i += blockDim.x) h0[i] = 0, h1[i] = 0;
   }
 }
 
@@ -636,9 +633,10 @@ __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_
     }
     for (int k = 0; k <= kSubQ; ++k) s_pre[k] = ld_sc(&line[1 + k]);
   }
+  // one thread reads the error flag, so the whole workgroup takes one decision
+  if (threadIdx.x == 0 && ld_sc(&es->error)) *s_ok = 0;
   __syncthreads();
   if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  if (ld_sc(&es->error)) return false;
   return *s_ok != 0;
 }
 
@@ -673,14 +671,64 @@ __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
   }
 }
 
+// ---------------------------------------------------------------- finalize
+// Result of pair d at engine iteration t_engine.  The barrier master kept, per
+// pass, the covered count, the sum of first-hit iterations and the last
+// iteration with a hit while it folded the histogram bins in order, and it
+// stops folding a pass at the bin where the covered count reaches the cut
+// ceil(r * total) (hello.cu:1748-1759, 1887-1895); so HD, sum and count are
+// read, not rescanned.
+__device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t_engine,
+                                 bool error) {
+  PairOut o;
+  const unsigned long long n_fp = ld_sc(&c.n_fp), n_fn = ld_sc(&c.n_fn);
+  const unsigned long long n_gold = ld_sc(&c.n_gold), n_segm = ld_sc(&c.n_segm);
+  const int tl = (int)(t_engine - d.t_start);  // pair-local iterations so far
+  bool done = true, reached = true;
+  unsigned long long sum[2], cnt[2];
+  int hd[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const unsigned long long cut = robust_cut(d.robust, m == 0 ? n_fp : n_fn);
+    sum[m] = cut ? ld_sc(&c.sum_d[m]) : 0ull;
+    cnt[m] = cut ? ld_sc(&c.cov[m]) : 0ull;
+    hd[m] = cut ? (int)ld_sc(&c.hd_last[m]) : 0;
+    done = done && (cut == 0 || ld_sc(&c.done_from[m]) <= tl);
+    reached = reached && cnt[m] >= cut;
+  }
+  o.hd_fp = hd[0], o.hd_fn = hd[1];
+  o.hd = max(hd[0], hd[1]);
+  o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
+  o.n_gold = n_gold, o.n_segm = n_segm, o.n_fp = n_fp, o.n_fn = n_fn;
+  o.done = done ? 1 : 0;
+  o.pad = 0;
+  int st = ld_sc(&c.status);
+  if ((n_fp && !n_gold) || (n_fn && !n_segm))
+    st = -2;  // HDM_E_EMPTY_MASK
+  else if (error)
+    st = -5;  // HDM_E_TIMEOUT
+  else if (st == 0 && done && !reached)
+    st = -8;  // HDM_E_STUCK
+  o.status = st;
+  return o;
+}
+
 // ---------------------------------------------------------------- dilate
 // SINGLE: the launch holds one pair, so its descriptor and its per-iteration
 // pass state are loop invariants instead of per-entry loads.
 template <int TXB, int TY, int TZ, bool SINGLE>
-__global__ __launch_bounds__(kWgThreads) void dilate_kernel(
-    const PairDesc *__restrict__ descs, PairCtr *ctrs, int npairs, Engine *es, uint64_t *q,
-    uint32_t qcap, int max_iters, int chunk, uint64_t spin_limit, uint32_t *qstats,
-    int qstats_len, int flags, uint64_t *trace) {
+__global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
+  const PairDesc *__restrict__ descs = a.descs;
+  PairCtr *ctrs = a.ctrs;
+  const int npairs = a.npairs;
+  Engine *es = a.es;
+  uint64_t *q = a.q;
+  const uint32_t qcap = a.qcap;
+  const int max_iters = a.max_iters, chunk = a.chunk;
+  const uint64_t spin_limit = a.spin;
+  uint32_t *qstats = a.qstats;
+  const int qstats_len = a.qstats_len, flags = a.flags;
+  uint64_t *trace = a.trace;
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;
@@ -934,11 +982,13 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
         for (int off = kHitSlots / 2; off >= 1; off >>= 1) v += shfl64(v, (threadIdx.x & 63) ^ off);
         if (!act || s != 0) continue;
         PairCtr &cc = ctrs[pp];
-        const unsigned long long before = cc.cov[mm];
+        const unsigned long long before = ld_sc(&cc.cov[mm]);
         if (v == before) continue;
         const PairDesc &dd = descs[pp];
         if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = (uint32_t)(v - before);
-        cc.cov[mm] = v;
+        st_sc(&cc.cov[mm], v);
+        st_sc(&cc.sum_d[mm], ld_sc(&cc.sum_d[mm]) + (v - before) * (unsigned long long)(tpp + 1));
+        st_sc(&cc.hd_last[mm], (uint32_t)(tpp + 1));
         const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
         if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
       }
@@ -951,106 +1001,29 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(
     };
     if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 3], s_pre)) break;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st_sc(&es->t, t);
-    st_sc(&es->qnext, s_pre[kSubQ]);
-  }
-}
-
-// ---------------------------------------------------------------- finalize
-// HD, sums and counts per pair from the first-hit histograms, with the robust
-// cut: the pass of mask m stops at the first t where the covered count reaches
-// ceil(r * total) (hello.cu:1748-1759, 1887-1895).  Each thread owns a
-// contiguous block of the histogram; one block scan finds the stop.
-template <typename T, typename Op>
-__device__ __forceinline__ T block_reduce(T v, T *sh, Op op) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = kWgThreads / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sh[threadIdx.x] = op(sh[threadIdx.x], sh[threadIdx.x + off]);
-    __syncthreads();
-  }
-  const T r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-__global__ __launch_bounds__(kWgThreads) void finalize_kernel(const PairDesc *__restrict__ descs,
-                                                              const PairCtr *__restrict__ ctrs,
-                                                              const Engine *es, PairOut *outs,
-                                                              int p0, int npairs) {
-  __shared__ unsigned long long sh[kWgThreads];
-  __shared__ int s_stop;
-  const int p = p0 + blockIdx.x;
-  if (p >= p0 + npairs) return;
-  const PairDesc &d = descs[p];
-  const PairCtr &c = ctrs[p];
-  const int per = (d.hist_len + kWgThreads - 1) / kWgThreads;
-  const int t0 = max(1, (int)threadIdx.x * per), t1 = min(d.hist_len, ((int)threadIdx.x + 1) * per);
-  int hd[2] = {0, 0};
-  unsigned long long sum[2] = {0, 0}, cnt[2] = {0, 0};
-  bool reached[2] = {true, true};
-  auto add = [](unsigned long long a, unsigned long long b) { return a + b; };
-  auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
-  for (int m = 0; m < 2; ++m) {
-    const unsigned long long tgt = robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn);
-    if (tgt == 0) continue;  // uniform
-    const uint32_t *h = d.hist[m];
-    unsigned long long local = 0;
-    for (int t = t0; t < t1; ++t) local += h[t];
-    // exclusive scan of the per-thread block totals
-    sh[threadIdx.x] = local;
-    __syncthreads();
-    for (int off = 1; off < kWgThreads; off <<= 1) {
-      const unsigned long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += v;
-      __syncthreads();
-    }
-    unsigned long long run = sh[threadIdx.x] - local;
-    if (threadIdx.x == 0) s_stop = 0x7FFFFFFF;
-    __syncthreads();
-    for (int t = t0; t < t1; ++t) {
-      run += h[t];
-      if (run >= tgt) {
-        atomicMin(&s_stop, t);
-        break;
+  // workgroup 0 holds every fold (it is the barrier master): it writes the
+  // results of all pairs and the engine head
+  if (blockIdx.x == 0) {
+    const bool err = ld_sc(&es->error) != 0;
+    for (int p = threadIdx.x; p < npairs; p += blockDim.x)
+      a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
+    if (threadIdx.x == 0) {
+      st_sc(&es->t, t);
+      st_sc(&es->qnext, s_pre[kSubQ]);
+      if (a.head) {
+        EngineHead h;
+        h.t = t;
+        h.error = err ? 1u : 0u;
+        h.qmax = es->qmax;
+        h.nwg = gridDim.x;
+        h.qtotal = es->qtotal;
+        h.qcap = es->qcap;
+        h.qnext = s_pre[kSubQ];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) h.pad1[i] = 0;
+        *a.head = h;
       }
     }
-    __syncthreads();
-    const int stop = s_stop;
-    reached[m] = stop != 0x7FFFFFFF;
-    unsigned long long ls = 0, lc = 0, lh = 0;
-    for (int t = t0; t < t1 && t <= stop; ++t) {
-      const unsigned long long v = h[t];
-      ls += (unsigned long long)t * v;
-      lc += v;
-      if (v) lh = (unsigned long long)t;
-    }
-    sum[m] = block_reduce(ls, sh, add);
-    cnt[m] = block_reduce(lc, sh, add);
-    hd[m] = (int)block_reduce(lh, sh, mx);
-  }
-  if (threadIdx.x == 0) {
-    PairOut o;
-    o.hd_fp = hd[0], o.hd_fn = hd[1];
-    o.hd = max(hd[0], hd[1]);
-    o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
-    o.n_gold = c.n_gold, o.n_segm = c.n_segm, o.n_fp = c.n_fp, o.n_fn = c.n_fn;
-    const int tl = (int)(es->t - d.t_start);  // pair-local iterations so far
-    const bool d0 = robust_cut(d.robust, c.n_fp) == 0 || c.done_from[0] <= tl;
-    const bool d1 = robust_cut(d.robust, c.n_fn) == 0 || c.done_from[1] <= tl;
-    o.done = (d0 && d1) ? 1 : 0;
-    o.pad = 0;
-    int st = c.status;
-    if ((c.n_fp && !c.n_gold) || (c.n_fn && !c.n_segm))
-      st = -2;  // HDM_E_EMPTY_MASK
-    else if (es->error)
-      st = -5;  // HDM_E_TIMEOUT
-    else if (st == 0 && o.done && (!reached[0] || !reached[1]))
-      st = -8;  // HDM_E_STUCK
-    o.status = st;
-    outs[p] = o;
   }
 }
 
@@ -1070,15 +1043,11 @@ hipError_t launch_meta(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
   return hipGetLastError();
 }
 template <int TXB, int TY, int TZ>
-hipError_t launch_dilate(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c, int np,
-                         Engine *es, uint64_t *q, uint32_t qcap, int max_iters, int chunk,
-                         uint64_t spin, uint32_t *qs, int qsl, int flags, uint64_t *trace) {
-  if (np == 1)
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true>), grid, dim3(kWgThreads), 0, s, d, c, np,
-                       es, q, qcap, max_iters, chunk, spin, qs, qsl, flags, trace);
+hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
+  if (a.npairs == 1)
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true>), grid, dim3(kWgThreads), 0, s, a);
   else
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false>), grid, dim3(kWgThreads), 0, s, d, c,
-                       np, es, q, qcap, max_iters, chunk, spin, qs, qsl, flags, trace);
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false>), grid, dim3(kWgThreads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1115,13 +1084,9 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
   hdm::PackIndex one_idx{};
   if (single) one = *single;
   if (single_idx) one_idx = *single_idx;
-  const int blocks = std::max(1, std::min(64, (npairs + 255) / 256));
-  hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(256), 0, s, descs, one, one_idx, idx,
-                     single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || npairs < 1) return e;
-  hipLaunchKernelGGL(hdm::clear_hist_kernel, dim3(4, std::min(npairs, 1024)), dim3(256), 0, s,
-                     descs, p0, npairs);
+  const int blocks = std::max(1, std::min(npairs, 1024));
+  hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(hdm::kWgThreads), 0, s, descs, one,
+                     one_idx, idx, single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
   return hipGetLastError();
 }
 
@@ -1130,10 +1095,3 @@ extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p
   return hipGetLastError();
 }
 
-extern "C" hipError_t hdm_launch_finalize(hipStream_t s, const hdm::PairDesc *descs,
-                                          const hdm::PairCtr *ctrs, const hdm::Engine *es,
-                                          hdm::PairOut *outs, int p0, int npairs) {
-  hipLaunchKernelGGL(hdm::finalize_kernel, dim3(npairs), dim3(hdm::kWgThreads), 0, s, descs,
-                     ctrs, es, outs, p0, npairs);
-  return hipGetLastError();
-}

@@ -15,13 +15,33 @@ struct PackIndex {
   uint64_t tile_base;
 };
 
+// Arguments of one persistent dilation launch.  When the launch leaves, its
+// workgroup 0 writes every pair's result to outs and the engine head to head
+// (either may be host-visible pinned memory; head may be null).
+struct DilateArgs {
+  const PairDesc *descs;
+  PairCtr *ctrs;
+  int npairs;
+  int max_iters;  // iterations this launch may run (a serving slice)
+  Engine *es;
+  uint64_t *q;
+  uint32_t qcap;
+  int chunk;       // queue entries per wave step, in tiles per wave
+  uint64_t spin;   // bound of every barrier wait, in wall-clock ticks
+  uint32_t *qstats;
+  int qstats_len;
+  int flags;       // 1: fence the barrier (release/acquire at agent scope)
+  uint64_t *trace; // per-iteration timestamps (diagnostic) or null
+  PairOut *outs;
+  EngineHead *head;
+};
+
 struct TileLaunch {
   hipError_t (*pack)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      uint64_t);
   hipError_t (*meta)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
-  hipError_t (*dilate)(dim3, hipStream_t, const PairDesc *, PairCtr *, int, Engine *, uint64_t *,
-                       uint32_t, int, int, uint64_t, uint32_t *, int, int, uint64_t *);
+  hipError_t (*dilate)(dim3, hipStream_t, const DilateArgs &);
   const void *dilate_fn;         // several pairs
   const void *dilate_single_fn;  // one pair
   const void *pack_fn;
@@ -149,6 +169,3 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
                                        hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
                                        int reset_engine, uint32_t qcap);
 extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs);
-extern "C" hipError_t hdm_launch_finalize(hipStream_t s, const hdm::PairDesc *descs,
-                                          const hdm::PairCtr *ctrs, const hdm::Engine *es,
-                                          hdm::PairOut *outs, int p0, int npairs);

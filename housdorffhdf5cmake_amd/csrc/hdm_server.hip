@@ -191,7 +191,8 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   if (hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&s->common, L.total) != hipSuccess || hipMalloc(&s->arena_mem, arena) != hipSuccess ||
-      hipHostMalloc(&s->host, sizeof(PairOut) * o.max_slots + 256, hipHostMallocDefault) !=
+      hipHostMalloc(&s->host, align256(sizeof(PairOut) * o.max_slots) + 256,
+                    hipHostMallocCoherent) !=
           hipSuccess) {
     (void)hipGetLastError();
     hdm_server_destroy(s);
@@ -338,21 +339,30 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   // one slice of the engine over every slot
   const int nslots = (int)s->slots.size();
   ++s->slices;
-  hipError_t e = s->tl->dilate(dim3(s->cfg.grid), s->s, s->P.descs, s->P.ctrs, nslots,
-                               s->P.engine, s->P.q, s->qcap, s->o.slice_iters, 1, s->cfg.spin,
-                               nullptr, 0, barrier_mode(), nullptr);
-  if (e == hipSuccess)
-    e = hdm_launch_finalize(s->s, s->P.descs, s->P.ctrs, s->P.engine, s->P.outs, 0, nslots);
+  // the slice's workgroup 0 writes every slot's result and the engine head
+  // into the pinned host block as the launch leaves
   char *h = (char *)s->host;
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h, s->P.outs, sizeof(PairOut) * nslots, hipMemcpyDeviceToHost, s->s);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(h + sizeof(PairOut) * nslots, s->P.engine, kEngineHostBytes,
-                       hipMemcpyDeviceToHost, s->s);
+  EngineHead *hhead = (EngineHead *)(h + align256(sizeof(PairOut) * nslots));
+  hhead->nwg = 0;
+  DilateArgs a{};
+  a.descs = s->P.descs;
+  a.ctrs = s->P.ctrs;
+  a.npairs = nslots;
+  a.max_iters = s->o.slice_iters;
+  a.es = s->P.engine;
+  a.q = s->P.q;
+  a.qcap = s->qcap;
+  a.chunk = 1;
+  a.spin = s->cfg.spin;
+  a.flags = barrier_mode();
+  a.outs = (PairOut *)h;
+  a.head = hhead;
+  hipError_t e = s->tl->dilate(dim3(s->cfg.grid), s->s, a);
   if (e == hipSuccess) e = hipStreamSynchronize(s->s);
   if (e != hipSuccess) return map_hip(e);
-  Engine head{};
-  std::memcpy(&head, h + sizeof(PairOut) * nslots, kEngineHostBytes);
+  EngineHead head{};
+  std::memcpy(&head, hhead, sizeof(head));
+  if (head.nwg == 0) return HDM_E_HIP;  // the launch did not complete
   s->t_engine = head.t;
   const double t_now = now_s();
   for (int p = 0; p < nslots; ++p) {
