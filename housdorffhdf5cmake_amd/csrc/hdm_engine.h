@@ -69,6 +69,9 @@ struct PairDesc {
   uint64_t *xf_orig[2];
   uint64_t *xf_buf[2][2];
   uint32_t *state[2];
+  // per tile, low byte gold / high byte segm: bit 0 non-empty, 1 full, 2..7
+  // voxels on the -x, +x, -y, +y, -z, +z face (written by the pack pass)
+  uint16_t *tflags;
   uint32_t *hist[2];
   uint16_t *map;
 };
@@ -90,6 +93,7 @@ struct PairCtr {
   int32_t pad;
   unsigned long long hits[2][kHitSlots];        // cumulative hits, spread over slots
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
+  int32_t box[6][kSumSlots];  // pack partials of lo[0..2] (min) and hi[0..2] (max)
 };
 
 struct PairOut {
@@ -124,14 +128,31 @@ struct Engine {
   uint32_t qnext;  // entries queued for the next iteration when the kernel left
   uint32_t pad1[8];
   // device only
-  uint32_t gen[kLineWords];            // unused (kept for layout)
-  uint32_t qlen[3][kLineWords];        // [t % 3][sub-queue]: read t%3, fill (t+1)%3
+  // [t % 4][sub-queue]: iteration t reads set t%4 and fills (t+1)%4; the
+  // barrier master zeroes (t+2)%4 at the end of t, a full barrier before use
+  uint32_t qlen[4][kLineWords];
   uint32_t arrive[kMaxGrid];           // last epoch each workgroup arrived at
   // Release lines, kReleaseCopies replicas so polling spreads over channels:
   // word 0 the epoch, words 1..kSubQ+1 the prefix offsets of the sub-queues of
   // the iteration the epoch starts.  Workgroup b polls replica b % kReleaseCopies.
   uint32_t rel[kReleaseCopies][kLineWords];
+  // Single-pair barrier: every word carries its epoch in the top 24 bits
+  // (tag()), so a reader that sees the right tag has the value that goes with
+  // it and no ordering between words is needed.  arr[wg]: this iteration's
+  // hits of pass 0 and 1 and the entries the workgroup pushed; rel2[copy]:
+  // word 0 flags (bit m: pass m done, bit 2: error), words 1..kSubQ+1 prefix
+  // offsets of the next queue.
+  unsigned long long arr[kMaxGrid][4];
+  unsigned long long rel2[kReleaseCopies][kLineWords];
 };
+constexpr int kTagShift = 40;
+constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;
+__host__ __device__ inline unsigned long long tag(uint32_t epoch, unsigned long long v) {
+  return ((unsigned long long)(epoch & 0xFFFFFFu) << kTagShift) | (v & kTagValue);
+}
+__host__ __device__ inline bool tagged(unsigned long long w, uint32_t epoch) {
+  return (uint32_t)(w >> kTagShift) == (epoch & 0xFFFFFFu);
+}
 constexpr size_t kEngineHostBytes = sizeof(EngineHead);
 static_assert(sizeof(EngineHead) == 64, "engine head is one 64-byte block");
 

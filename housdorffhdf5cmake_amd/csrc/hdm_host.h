@@ -39,7 +39,7 @@ struct Ctx {
 
 // launch shapes and optional phase events of one enqueue
 struct LaunchCfg {
-  unsigned pack_blocks = 1, meta_blocks = 1;
+  unsigned pack_blocks = 1, seed_blocks = 1;
   int grid = 1, chunk = 1;
   uint64_t spin = 0;
   const hipEvent_t *ev = nullptr;  // 4 events (phase bounds) or null

@@ -142,19 +142,21 @@ int barrier_mode() {
 
 int norm_tile(int tile) { return (tile <= 0 || tile >= HDM_TILE_COUNT) ? HDM_TILE_64x8x8 : tile; }
 
-// Launch shapes.  Pack: exactly the resident waves, each walking one
-// contiguous range of row words (one round, no tail).  Metadata: grid-stride
-// over tiles with at most 1024 workgroups so the queue flushes (one per
-// workgroup) stay few per sub-queue counter.  Dilation: occupancy-bounded.
+// Launch shapes.  Pack: at most the resident waves, each walking one
+// contiguous range of tiles, two wave steps each when the grid allows.  Seed:
+// one thread per tile.  Dilation: occupancy-bounded.
 LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
                      int grid_req, int chunk) {
   LaunchCfg L;
+  (void)total_rw;
   const uint64_t rows = (uint64_t)tile_shape(tile).rows();
-  const uint64_t pack_want = (total_rw + kWgThreads - 1) / kWgThreads;
+  const uint64_t tpw = (uint64_t)kWave / rows;  // tiles per wave step of the pack
+  const uint64_t pack_waves = (total_tiles + 2 * tpw - 1) / (2 * tpw);  // two steps per wave
+  const uint64_t pack_want = (pack_waves + kWavesPerWg - 1) / kWavesPerWg;
   const uint64_t pack_res = (uint64_t)std::max(1, c->occ_pack[tile]) * c->cus;
   L.pack_blocks = (unsigned)std::max<uint64_t>(1, std::min(pack_want, pack_res));
-  const uint64_t meta_want = (total_tiles * rows + kWgThreads - 1) / kWgThreads;
-  L.meta_blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(meta_want, 1024));
+  const uint64_t seed_want = (total_tiles + kWgThreads - 1) / kWgThreads;
+  L.seed_blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(seed_want, 8192));
   L.grid = resident_grid(c, tile, grid_req);
   L.chunk = std::max(1, chunk);
   L.spin = spin_ticks(c);
@@ -171,11 +173,12 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   hipError_t e = hdm_launch_reset(s, P.descs, single, single_idx, P.idx, P.ctrs, 0, npairs,
                                   P.engine, 1, qcap);
   if (e != hipSuccess || (e = mark(0)) != hipSuccess) return e;
-  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_rw);
+  (void)total_rw;
+  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_tiles);
   if (e != hipSuccess) return e;
-  e = hdm_launch_totals(s, P.ctrs, 0, npairs);
-  if (e != hipSuccess || (e = mark(1)) != hipSuccess) return e;
-  e = tl->meta(dim3(cfg.meta_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,
+  if (npairs > 1 && (e = hdm_launch_totals(s, P.ctrs, 0, npairs)) != hipSuccess) return e;
+  if ((e = mark(1)) != hipSuccess) return e;
+  e = tl->seed(dim3(cfg.seed_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,
                total_tiles, 0);
   if (e != hipSuccess || (e = mark(2)) != hipSuccess) return e;
   DilateArgs a{};

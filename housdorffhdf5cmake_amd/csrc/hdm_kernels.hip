@@ -3,14 +3,14 @@
 // by the latency of the per-iteration grid barrier.
 //
 //   reset_kernel     clears per-pair counters, engine state, histograms
-//   pack_kernel      dense labels -> packed tile-major bits, partial voxel
-//                    counts and the tile bounding box, in one read of the input
-//                    (replaces getMinMaxes and the packing half of
+//   pack_kernel      dense labels -> packed tile-major bits, x-face words and
+//                    tile flags, partial voxel counts and the tile bounding
+//                    box, in one read of the input (replaces getMinMaxes and
 //                    boolPrepareKernel, hello.cu:396-526, 537-748)
 //   totals_kernel    folds the partial counts into |gold|, |segm|, |FP|, |FN|
-//   meta_kernel      per-tile full/empty class, face words and the initial
-//                    work queue (boolPrepareKernel metadata and
-//                    firstMetaPrepareKernel, hello.cu:537-748, 762-877)
+//                    (batches; a single pair's seed pass does it itself)
+//   seed_kernel      tile states and the first work queue from the tile flags
+//                    (firstMetaPrepareKernel, hello.cu:762-877)
 //   dilate_kernel    persistent kernel: one grid barrier per iteration, a
 //                    queue of the tiles still changing (mainPassKernel,
 //                    hello.cu:1066-1896); when it leaves, its workgroup 0
@@ -137,8 +137,9 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     descs[p0] = single;
     idx[p0] = single_idx;
   }
-  if (reset_engine && blockIdx.x == 0) {
-    if (tid == 0) {
+  if (reset_engine) {  // spread over the grid's blocks
+    const int g = blockIdx.x * blockDim.x + tid, gs = gridDim.x * blockDim.x;
+    if (g == 0) {
       es->t = 0;
       es->error = 0;
       es->qmax = 0;
@@ -146,12 +147,11 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
       es->qcap = qcap;
       es->qnext = 0;
     }
-    for (int i = tid; i < kLineWords; i += blockDim.x) {
-      es->gen[i] = 0;
-      es->qlen[0][i] = es->qlen[1][i] = es->qlen[2][i] = 0;
-    }
-    for (int i = tid; i < kMaxGrid; i += blockDim.x) es->arrive[i] = 0;
-    for (int i = tid; i < kReleaseCopies * kLineWords; i += blockDim.x) es->rel[0][i] = 0;
+    for (int i = g; i < 4 * kLineWords; i += gs) es->qlen[0][i] = 0;
+    for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
+    for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel[0][i] = 0;
+    for (int i = g; i < kMaxGrid * 4; i += gs) es->arr[0][i] = 0;
+    for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel2[0][i] = 0;
   }
   for (int p = p0 + blockIdx.x; p < p0 + npairs; p += gridDim.x) {
     PairCtr &c = ctrs[p];
@@ -167,6 +167,8 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     }
     for (int i = tid; i < 2 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
     for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) (&c.sums[0][0])[i] = 0;
+    for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
+      (&c.box[0][0])[i] = i < 3 * kSumSlots ? 0x7FFFFFFF : -1;
     uint32_t *h0 = write_single ? single.hist[0] : descs[p].hist[0];
     uint32_t *h1 = write_single ? single.hist[1] : descs[p].hist[1];
     const int len = write_single ? single.hist_len : descs[p].hist_len;
@@ -212,7 +214,7 @@ __device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol,
       const u32x4 *q = (const u32x4 *)p;
 #pragma unroll
       for (int k = 0; k < TXB / 16; ++k)
-        bits |= (Row)eq_bits_u8x16(__builtin_nontemporal_load(q + k), lab * 0x01010101u)
+        bits |= (Row)eq_bits_u8x16(q[k], lab * 0x01010101u)
                 << (16 * k);
     } else {
       for (int k = 0; k < count; ++k) bits |= (Row)(p[k] == lab) << k;
@@ -224,7 +226,7 @@ __device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol,
       const i32x4 *q = (const i32x4 *)p;
 #pragma unroll
       for (int k = 0; k < TXB / 4; ++k) {
-        const i32x4 v = __builtin_nontemporal_load(q + k);
+        const i32x4 v = q[k];
         bits |= (Row)(v.x == lab) << (4 * k);
         bits |= (Row)(v.y == lab) << (4 * k + 1);
         bits |= (Row)(v.z == lab) << (4 * k + 2);
@@ -253,9 +255,9 @@ __device__ __forceinline__ void acc_publish(PairCtr &c, const PackAcc &a, int sl
     if (a.c[f]) atomicAdd(&c.sums[f][slot], (unsigned long long)a.c[f]);
   if (a.mx[0] >= 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {  // publish only improvements: most waves skip
-      if (a.mn[k] < ld_sc(&c.lo[k])) atomicMin(&c.lo[k], a.mn[k]);
-      if (a.mx[k] > ld_sc(&c.hi[k])) atomicMax(&c.hi[k], a.mx[k]);
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&c.box[k][slot], a.mn[k]);
+      atomicMax(&c.box[3 + k][slot], a.mx[k]);
     }
   }
 }
@@ -270,112 +272,145 @@ __device__ __forceinline__ void acc_flush(PairCtr &c, PackAcc &a, int lane, int 
   acc_clear(a);
 }
 
-// One lane per TXB-voxel row word, waves walk contiguous word ranges so the
-// pair index changes rarely; statistics accumulate in registers and are
-// published once per (wave, pair) into spread slots.
+// One wave per tile step (R lanes per tile, one lane per row): waves walk
+// contiguous tile ranges so the pair index changes rarely.  A lane compares
+// its row's TXB labels with the label (hello.cu:453 / 615), stores the packed
+// word (the R words of a tile are contiguous, so the store is coalesced), and
+// the tile's ballots give its x-face words and its flags for the seed pass:
+// non-empty, full, and which of its six faces hold voxels.  Voxel counts and
+// the box of non-empty tiles (getMinMaxes, hello.cu:396-526) accumulate per
+// lane and are published once per (wave, pair) into spread slots.
 template <int TXB, int TY, int TZ>
 __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs,
-                                                          uint64_t total_words) {
+                                                          uint64_t total_tiles) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
+  constexpr int TPW = kWave / R;  // tiles per wave step
+  constexpr int U = 2;            // wave steps whose loads are all issued first
   const int lane = threadIdx.x & 63;
+  const int sub = lane / R, r = lane % R;
+  const int yl = r % TY, zl = r / TY;
   const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
   const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
-  constexpr int U = 2;  // row words per lane per step, all loads issued first
-  const uint64_t per = ((total_words + nw - 1) / nw + 64 * U - 1) & ~uint64_t(64 * U - 1);
-  const uint64_t w0 = gw * per, w1 = min(total_words, w0 + per);
+  constexpr uint64_t kStep = (uint64_t)TPW * U;
+  const uint64_t per = ((total_tiles + nw - 1) / nw + kStep - 1) / kStep * kStep;
+  const uint64_t w0 = gw * per, w1 = min(total_tiles, w0 + per);
   const int slot = (int)(gw % kSumSlots);
   PackAcc acc;
   acc_clear(acc);
   int ap = -1;  // pair the accumulator belongs to (wave-uniform)
-  for (uint64_t base = w0; base < w1; base += 64 * U) {
-    bool inU[U];
-    int pU[U], wxU[U], yU[U], zU[U];
-    uint64_t wU[U], vbU[U];
+  for (uint64_t base = w0; base < w1; base += kStep) {
+    bool inU[U], okU[U];
+    int pU[U], txU[U], tyU[U], tzU[U], cntU[U];
+    uint32_t tileU[U];
+    uint64_t vbU[U];
     Row aU[U], bU[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const uint64_t g = base + 64 * u + lane;
+      const uint64_t g = base + (uint64_t)u * TPW + sub;
       inU[u] = g < w1;
       pU[u] = 0;
       if (npairs > 1)
-        pU[u] = find_pair(npairs, inU[u] ? g : w1 - 1, [&](int i) { return idx[i].rw_base; });
+        pU[u] = find_pair(npairs, inU[u] ? g : w1 - 1, [&](int i) { return idx[i].tile_base; });
       aU[u] = bU[u] = 0;
+      okU[u] = false;
+      tileU[u] = 0;
+      txU[u] = tyU[u] = tzU[u] = 0;
+      cntU[u] = 0;
+      vbU[u] = 0;
       if (inU[u]) {
         const PairDesc &d = descs[pU[u]];
-        const uint64_t w = g - idx[pU[u]].rw_base;
-        wU[u] = w;
-        wxU[u] = (int)(w % (uint64_t)d.wxn);
-        const uint64_t rest = w / (uint64_t)d.wxn;
-        yU[u] = (int)(rest % (uint64_t)d.ny);
-        zU[u] = (int)(rest / (uint64_t)d.ny);
-        vbU[u] = (uint64_t)wxU[u] * TXB + (uint64_t)d.nx * ((uint64_t)yU[u] + (uint64_t)d.ny * zU[u]);
-        const int count = min(TXB, d.nx - wxU[u] * TXB);
-        aU[u] = load_row_bits<Row>(d, d.gold, vbU[u], count);
-        bU[u] = load_row_bits<Row>(d, d.segm, vbU[u], count);
+        const uint32_t tile = (uint32_t)(g - idx[pU[u]].tile_base);
+        tileU[u] = tile;
+        txU[u] = (int)(tile % (uint32_t)d.gx);
+        tyU[u] = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+        tzU[u] = (int)(tile / ((uint32_t)d.gx * d.gy));
+        const int y = tyU[u] * TY + yl, z = tzU[u] * TZ + zl;
+        okU[u] = y < d.ny && z < d.nz;
+        cntU[u] = min(TXB, d.nx - txU[u] * TXB);
+        vbU[u] = (uint64_t)txU[u] * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+        if (okU[u]) {
+          aU[u] = load_row_bits<Row>(d, d.gold, vbU[u], cntU[u]);
+          bU[u] = load_row_bits<Row>(d, d.segm, vbU[u], cntU[u]);
+        }
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-    const int p = pU[u];
-    const int lead = __builtin_amdgcn_readfirstlane(p);
-    if (ap != lead) {
-      if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
-      ap = lead;
-    }
-    if (!inU[u]) continue;
-    const PairDesc &d = descs[p];
-    const uint64_t w = wU[u];
-    const int wx = wxU[u], y = yU[u], z = zU[u];
-    const uint64_t vbase = vbU[u];
-    const int count = min(TXB, d.nx - wx * TXB);
-    const Row a = aU[u];
-    const Row b = bU[u];
-    const uint32_t tile =
-        (uint32_t)wx + (uint32_t)d.gx * ((uint32_t)(y / TY) + (uint32_t)d.gy * (uint32_t)(z / TZ));
-    const uint64_t o = (uint64_t)tile * R + (uint64_t)(y % TY) + (uint64_t)TY * (z % TZ);
-    ((Row *)d.orig[0])[o] = a;
-    ((Row *)d.orig[1])[o] = b;
-    if (w < d.ntiles) {  // idle state for every tile before the metadata pass
-      d.state[0][w] = 0;
-      d.state[1][w] = 0;
-    }
-    if (d.map) {
-      uint16_t *mp = d.map + vbase;
-      for (int k = 0; k < count; ++k) mp[k] = 0;
-    }
-    // counts of hello.cu:624-625 plus the non-empty tile box (hello.cu:396-526)
-    const uint32_t c0 = popc_row(a), c1 = popc_row(b);
-    const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
-    const bool ne = (a | b) != 0;
-    const int ty_ = y / TY, tz_ = z / TZ;
-    if (p == ap) {
-      acc.c[0] += c0, acc.c[1] += c1, acc.c[2] += c2, acc.c[3] += c3;
-      if (ne) {
-        acc.mn[0] = min(acc.mn[0], wx), acc.mx[0] = max(acc.mx[0], wx);
-        acc.mn[1] = min(acc.mn[1], ty_), acc.mx[1] = max(acc.mx[1], ty_);
-        acc.mn[2] = min(acc.mn[2], tz_), acc.mx[2] = max(acc.mx[2], tz_);
+      const int p = pU[u];
+      const int lead = __builtin_amdgcn_readfirstlane(p);
+      if (ap != lead) {
+        if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+        ap = lead;
       }
-    } else {  // a lane past a pair boundary publishes its own word
-      PairCtr &cp = ctrs[p];
-      if (c0) atomicAdd(&cp.sums[0][slot], (unsigned long long)c0);
-      if (c1) atomicAdd(&cp.sums[1][slot], (unsigned long long)c1);
-      if (c2) atomicAdd(&cp.sums[2][slot], (unsigned long long)c2);
-      if (c3) atomicAdd(&cp.sums[3][slot], (unsigned long long)c3);
-      if (ne) {
-        atomicMin(&cp.lo[0], wx), atomicMax(&cp.hi[0], wx);
-        atomicMin(&cp.lo[1], ty_), atomicMax(&cp.hi[1], ty_);
-        atomicMin(&cp.lo[2], tz_), atomicMax(&cp.hi[2], tz_);
+      const bool in = inU[u], rowok = okU[u];
+      const PairDesc &d = descs[p];
+      const uint32_t tile = tileU[u];
+      const Row a = aU[u], b = bU[u];
+      const uint64_t o = (uint64_t)tile * R + r;
+      if (rowok) {
+        ((Row *)d.orig[0])[o] = a;
+        ((Row *)d.orig[1])[o] = b;
       }
-    }
+      const Row vb = valid_bits<Row>(txU[u], d.nx);
+      uint32_t flags = 0;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const Row w = m == 0 ? a : b;
+        const uint64_t fxl = group_bits<R>(__ballot(rowok && (w & 1)), lane);
+        const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w >> (TXB - 1)) & 1)), lane);
+        const bool any = group_bits<R>(__ballot(w != 0), lane) != 0;
+        const bool full = group_bits<R>(__ballot(rowok && w != vb), lane) == 0;
+        const bool fyl = group_bits<R>(__ballot(yl == 0 && w != 0), lane) != 0;
+        const bool fyh = group_bits<R>(__ballot(yl == TY - 1 && w != 0), lane) != 0;
+        const bool fzl = group_bits<R>(__ballot(zl == 0 && w != 0), lane) != 0;
+        const bool fzh = group_bits<R>(__ballot(zl == TZ - 1 && w != 0), lane) != 0;
+        if (in && r == 0) {
+          d.xf_orig[m][2 * (uint64_t)tile] = fxl;
+          d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
+        }
+        flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)(fxl != 0) << 2 |
+                  (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
+                  (uint32_t)fzl << 6 | (uint32_t)fzh << 7)
+                 << (8 * m);
+      }
+      if (in && r == 0) d.tflags[tile] = (uint16_t)flags;
+      if (d.map && rowok) {
+        uint16_t *mp = d.map + vbU[u];
+        for (int k = 0; k < cntU[u]; ++k) mp[k] = 0;
+      }
+      // counts of hello.cu:624-625 plus the non-empty tile box
+      const uint32_t c0 = popc_row(a), c1 = popc_row(b);
+      const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
+      const bool ne = (a | b) != 0;
+      if (p == ap) {
+        acc.c[0] += c0, acc.c[1] += c1, acc.c[2] += c2, acc.c[3] += c3;
+        if (ne) {
+          acc.mn[0] = min(acc.mn[0], txU[u]), acc.mx[0] = max(acc.mx[0], txU[u]);
+          acc.mn[1] = min(acc.mn[1], tyU[u]), acc.mx[1] = max(acc.mx[1], tyU[u]);
+          acc.mn[2] = min(acc.mn[2], tzU[u]), acc.mx[2] = max(acc.mx[2], tzU[u]);
+        }
+      } else if (in) {  // a lane past a pair boundary publishes its own row
+        PairCtr &cp = ctrs[p];
+        if (c0) atomicAdd(&cp.sums[0][slot], (unsigned long long)c0);
+        if (c1) atomicAdd(&cp.sums[1][slot], (unsigned long long)c1);
+        if (c2) atomicAdd(&cp.sums[2][slot], (unsigned long long)c2);
+        if (c3) atomicAdd(&cp.sums[3][slot], (unsigned long long)c3);
+        if (ne) {
+          atomicMin(&cp.box[0][slot], txU[u]), atomicMax(&cp.box[3][slot], txU[u]);
+          atomicMin(&cp.box[1][slot], tyU[u]), atomicMax(&cp.box[4][slot], tyU[u]);
+          atomicMin(&cp.box[2][slot], tzU[u]), atomicMax(&cp.box[5][slot], tzU[u]);
+        }
+      }
     }
   }
   if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
 }
 
-// |gold|, |segm|, |FP|, |FN| per pair from the spread partials (one wave per pair)
+// |gold|, |segm|, |FP|, |FN| per pair from the spread partials (one wave per
+// pair); the single-pair seed pass folds them itself
 __global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
   const int p = p0 + blockIdx.x;
   if (p >= p0 + npairs) return;
@@ -389,22 +424,33 @@ __global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
     for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
   }
   if (lane == 0) c.n_gold = v[0], c.n_segm = v[1], c.n_fp = v[2], c.n_fn = v[3];
+  int bx[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+    bx[k] = k < 3 ? wave_min(c.box[k][lane], lane) : wave_max(c.box[k][lane], lane);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c.lo[k] = bx[k], c.hi[k] = bx[3 + k];
+  }
 }
 
 // ---------------------------------------------------------------- helpers
 struct Box {
   int lo[3], hi[3];
 };
-__device__ __forceinline__ Box pair_box(const PairDesc &d, const PairCtr &c) {
+__device__ __forceinline__ Box pair_box(const PairDesc &d, const int *lo, const int *hi) {
   Box b;
   if (d.crop) {
-    b.lo[0] = c.lo[0], b.lo[1] = c.lo[1], b.lo[2] = c.lo[2];
-    b.hi[0] = c.hi[0], b.hi[1] = c.hi[1], b.hi[2] = c.hi[2];
+    b.lo[0] = lo[0], b.lo[1] = lo[1], b.lo[2] = lo[2];
+    b.hi[0] = hi[0], b.hi[1] = hi[1], b.hi[2] = hi[2];
   } else {
     b.lo[0] = b.lo[1] = b.lo[2] = 0;
     b.hi[0] = d.gx - 1, b.hi[1] = d.gy - 1, b.hi[2] = d.gz - 1;
   }
   return b;
+}
+__device__ __forceinline__ Box pair_box(const PairDesc &d, const PairCtr &c) {
+  return pair_box(d, c.lo, c.hi);
 }
 
 // neighbour tile of `tile` in direction dir (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z),
@@ -428,8 +474,10 @@ struct LocalQueue {
   uint32_t n, base;
 };
 
+// pushed (LDS, may be null) counts the entries that went straight to gq
 __device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue &lq,
-                                             uint64_t *gq, uint32_t *gq_len, int lane) {
+                                             uint64_t *gq, uint32_t *gq_len, int lane,
+                                             uint32_t *pushed = nullptr) {
   const uint64_t m = __ballot(want);
   if (m == 0) return;
   const uint32_t n = (uint32_t)__popcll(m);
@@ -440,6 +488,7 @@ __device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue
     for (;;) {
       if (cur + n > (uint32_t)kLocalQueue) {
         base = atomicAdd(gq_len, n);
+        if (pushed) atomicAdd(pushed, n);
         global = true;
         break;
       }
@@ -462,7 +511,9 @@ __device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue
   }
 }
 
-__device__ __forceinline__ void flush_local_queue(LocalQueue &lq, uint64_t *gq, uint32_t *gq_len) {
+// returns the number of entries flushed (the same in every thread)
+__device__ __forceinline__ uint32_t flush_local_queue(LocalQueue &lq, uint64_t *gq,
+                                                      uint32_t *gq_len) {
   __syncthreads();
   const uint32_t n = lq.n;
   if (n != 0) {
@@ -473,89 +524,107 @@ __device__ __forceinline__ void flush_local_queue(LocalQueue &lq, uint64_t *gq, 
   }
   __syncthreads();
   if (threadIdx.x == 0) lq.n = 0;
+  return n;
 }
 
-// ---------------------------------------------------------------- meta
+// ---------------------------------------------------------------- seed
+// One thread per tile decides, for both masks, the tile's state and whether
+// it enters the first queue, from its own flags and those of its six
+// neighbours (no atomics): a non-empty full tile is kFull and stays out; a
+// non-empty tile that is not full is queued; an empty tile is queued when a
+// non-empty neighbour has voxels on the face toward it (hello.cu:1649-1663).
+// Only tiles inside the pair's box of non-empty tiles take part
+// (hello.cu:940-973), and a pass whose cut is zero queues nothing.  Every
+// entry is first = 1 (input from orig) and check = 1 (count first hits).
+// With one pair the launch also folds the pack partials into the totals.
 template <int TXB, int TY, int TZ>
-__global__ __launch_bounds__(kWgThreads) void meta_kernel(const PairDesc *__restrict__ descs,
+__global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs, Engine *es,
                                                           uint64_t *q, uint32_t qcap,
                                                           uint64_t total_tiles, int p0) {
-  using Row = RowT<TXB>;
-  constexpr int R = TY * TZ;
-  constexpr int TPB = kWgThreads / R;  // tiles per workgroup step
   __shared__ LocalQueue lq;
-  if (threadIdx.x == 0) lq.n = 0;
-  __syncthreads();
+  __shared__ unsigned long long s_tot[4];
+  __shared__ int s_box[6];
   const int lane = threadIdx.x & 63;
-  const int r = lane % R;
-  const int yl = r % TY, zl = r / TY;
+  if (threadIdx.x == 0) lq.n = 0;
+  if (npairs == 1 && threadIdx.x < 64) {
+    PairCtr &c = ctrs[0];
+    unsigned long long v[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      v[f] = c.sums[f][lane];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+    }
+    int bx[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+      bx[k] = k < 3 ? wave_min(c.box[k][lane], lane) : wave_max(c.box[k][lane], lane);
+    if (lane == 0) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_box[k] = bx[k];
+      if (blockIdx.x == 0) {
+        c.n_gold = v[0], c.n_segm = v[1], c.n_fp = v[2], c.n_fn = v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) c.lo[k] = bx[k], c.hi[k] = bx[3 + k];
+      }
+    }
+  }
+  __syncthreads();
   const uint32_t t0 = es->t;
   const uint32_t kq = blockIdx.x % kSubQ;
   uint64_t *gq = sub_queue(q, qcap, t0 & 1, kq);
-  uint32_t *gq_len = &es->qlen[t0 % 3][kq];
-  for (uint64_t tb = (uint64_t)blockIdx.x * TPB; tb < total_tiles; tb += (uint64_t)gridDim.x * TPB) {
-    const uint64_t gtile = tb + threadIdx.x / R;
-    const bool in = gtile < total_tiles;
+  uint32_t *gq_len = &es->qlen[t0 % 4][kq];
+  // bit of the neighbour's flags that marks its face toward this tile, per
+  // direction (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z)
+  constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
+  for (uint64_t tb = (uint64_t)blockIdx.x * kWgThreads; tb < total_tiles;
+       tb += (uint64_t)gridDim.x * kWgThreads) {
+    const uint64_t g = tb + threadIdx.x;
+    const bool in = g < total_tiles;
     int p = 0;
     if (npairs > 1)
-      p = find_pair(npairs, in ? gtile : total_tiles - 1, [&](int i) { return idx[i].tile_base; });
+      p = find_pair(npairs, in ? g : total_tiles - 1, [&](int i) { return idx[i].tile_base; });
     const PairDesc &d = descs[p];
     const PairCtr &c = ctrs[p];
-    const uint32_t tile = in ? (uint32_t)(gtile - idx[p].tile_base) : 0;
+    const uint32_t tile = in ? (uint32_t)(g - idx[p].tile_base) : 0;
     const int tx = (int)(tile % (uint32_t)d.gx);
     const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
     const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
-    const int y = ty * TY + yl, z = tz * TZ + zl;
-    const bool rowok = in && y < d.ny && z < d.nz;
-    const Row vb = valid_bits<Row>(tx, d.nx);
-    Row w[2] = {0, 0};
-    if (rowok) {
-      w[0] = ((const Row *)d.orig[0])[(uint64_t)tile * R + r];
-      w[1] = ((const Row *)d.orig[1])[(uint64_t)tile * R + r];
-    }
-    const Box b = pair_box(d, c);
+    const Box b = npairs == 1 ? pair_box(d, s_box, s_box + 3) : pair_box(d, c);
     const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
                         ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
+    uint32_t own = 0, nbf[6] = {0, 0, 0, 0, 0, 0};
+    if (in_box) {
+      own = d.tflags[tile];
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) {
+        const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, dir);
+        if (nb != kNoTile) nbf[dir] = d.tflags[nb];
+      }
+    }
+    const unsigned long long nfp = npairs == 1 ? s_tot[2] : c.n_fp;
+    const unsigned long long nfn = npairs == 1 ? s_tot[3] : c.n_fn;
+    bool want[2];
+    uint64_t ent[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const uint64_t fxl = group_bits<R>(__ballot(rowok && (w[m] & 1)), lane);
-      const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w[m] >> (TXB - 1)) & 1)), lane);
-      const bool any = group_bits<R>(__ballot(w[m] != 0), lane) != 0;
-      const bool full = group_bits<R>(__ballot(rowok && w[m] != vb), lane) == 0;
-      if (in && r == 0) {
-        d.xf_orig[m][2 * (uint64_t)tile] = fxl;
-        d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
-      }
-      const bool fyl = group_bits<R>(__ballot(rowok && yl == 0 && w[m] != 0), lane) != 0;
-      const bool fyh = group_bits<R>(__ballot(rowok && yl == TY - 1 && w[m] != 0), lane) != 0;
-      const bool fzl = group_bits<R>(__ballot(rowok && zl == 0 && w[m] != 0), lane) != 0;
-      const bool fzh = group_bits<R>(__ballot(rowok && zl == TZ - 1 && w[m] != 0), lane) != 0;
-      // a pass whose cut is zero queues nothing
-      const bool needed = robust_cut(d.robust, m == 0 ? c.n_fp : c.n_fn) != 0;
-      // lane 6: the tile itself; lanes 0..5: its face neighbours (hello.cu:1649-1663)
-      bool want = false;
-      uint64_t ent = 0;
-      if (in_box && any) {
-        if (r == 6) {
-          const uint32_t nv = full ? kFull : 1u;
-          if (atomicCAS(&d.state[m][tile], 0u, nv) == 0u && !full && needed) {
-            want = true;
-            ent = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1, d.gen);
-          }
-        } else if (r < 6 && needed) {
-          const bool face = r == 0 ? fxl != 0 : r == 1 ? fxh != 0 : r == 2 ? fyl : r == 3 ? fyh
-                                                                    : r == 4 ? fzl : fzh;
-          const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, r);
-          if (face && nb != kNoTile && atomicCAS(&d.state[m][nb], 0u, 1u) == 0u) {
-            want = true;
-            ent = make_entry(nb, (uint32_t)(p0 + p), (uint32_t)m, 1, 1, d.gen);
-          }
-        }
-      }
-      push_entries(want, ent, lq, gq, gq_len, lane);
+      const uint32_t f = (own >> (8 * m)) & 0xFF;
+      const bool any = f & 1, full = (f >> 1) & 1;
+      bool act = false;
+#pragma unroll
+      for (int dir = 0; dir < 6; ++dir) act |= ((nbf[dir] >> (8 * m + kFacing[dir])) & 1) != 0;
+      const bool needed = robust_cut(d.robust, m == 0 ? nfp : nfn) != 0;
+      const uint32_t st = !in_box ? 0u : (any && full) ? kFull : (any || (act && needed)) ? 1u : 0u;
+      if (in) d.state[m][tile] = st;
+      want[m] = in_box && needed && (any ? !full : act);
+      ent[m] = make_entry(tile, (uint32_t)(p0 + p), (uint32_t)m, 1, 1, d.gen);
     }
+    push_entries(want[0], ent[0], lq, gq, gq_len, lane);
+    push_entries(want[1], ent[1], lq, gq, gq_len, lane);
   }
   flush_local_queue(lq, gq, gq_len);
 }
@@ -637,6 +706,149 @@ __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_
   if (threadIdx.x == 0 && ld_sc(&es->error)) *s_ok = 0;
   __syncthreads();
   if (fence) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return *s_ok != 0;
+}
+
+// Bookkeeping of the single-pair barrier master, kept in registers of thread 0
+// of workgroup 0 for the whole launch (loaded from and stored back to PairCtr).
+struct PassState {
+  unsigned long long cut[2], cov[2], sum[2];
+  uint32_t hd[2];
+  int done_from[2];
+};
+
+// Grid barrier of a single-pair launch.  It has the same structure as
+// grid_barrier, but every word that crosses workgroups carries its epoch
+// (tag()), so neither side needs a second round trip:
+//   arrival  after s_waitcnt 0 and a workgroup barrier, thread 0 stores
+//            tag(epoch, hits of pass 0 / pass 1 / entries pushed);
+//   master   polls the records (thread i watches workgroups i, i+256, ...,
+//            which all flush into sub-queue i % kSubQ), sums them, folds the
+//            hits into the histogram and its PassState, decides the passes'
+//            stop (hello.cu:1748-1759), zeroes the queue counters of the
+//            iteration after next, and stores tag(epoch, flags) and
+//            tag(epoch, prefix offsets of the next queue) into every release
+//            replica;
+//   others   poll their replica until all words carry the epoch.
+// Waits are bounded as in grid_barrier.  s_flags receives the pass-done and
+// error flags, s_pre the prefix offsets.  Returns false on a timeout.
+__device__ __forceinline__ bool grid_barrier_single(
+    Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, PassState &ps,
+    const PairDesc &d, uint32_t tdone, uint32_t L, uint32_t *qstats, int qstats_len,
+    uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *s_ok = 1;
+    st_sc(&es->arr[blockIdx.x][0], tag(epoch, s_cnt[0]));
+    st_sc(&es->arr[blockIdx.x][1], tag(epoch, s_cnt[1]));
+    st_sc(&es->arr[blockIdx.x][2], tag(epoch, s_cnt[2]));
+    s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;  // the next iteration's counts start here
+  }
+  constexpr int kWords = kSubQ + 2;  // flags + kSubQ + 1 prefix offsets
+  if (blockIdx.x == 0) {
+    unsigned long long h0 = 0, h1 = 0, np = 0;
+    int ok = 1;
+    const uint64_t t0 = wall_clock64();
+    for (uint32_t i = threadIdx.x; i < gridDim.x && ok; i += blockDim.x) {
+      for (;;) {
+        const unsigned long long w0 = ld_sc(&es->arr[i][0]);
+        const unsigned long long w1 = ld_sc(&es->arr[i][1]);
+        const unsigned long long w2 = ld_sc(&es->arr[i][2]);
+        if (tagged(w0, epoch) && tagged(w1, epoch) && tagged(w2, epoch)) {
+          h0 += w0 & kTagValue, h1 += w1 & kTagValue, np += w2 & kTagValue;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    // sums: hits per pass over all workgroups, pushes per sub-queue
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < kSubQ) s_red[8 + threadIdx.x] = 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      h0 += shfl64(h0, lane ^ off);
+      h1 += shfl64(h1, lane ^ off);
+    }
+    __syncthreads();
+    if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1;
+    if (!ok) *s_ok = 0;
+    atomicAdd(&s_red[8 + threadIdx.x % kSubQ], np);
+    __syncthreads();
+    if (tr && threadIdx.x == 0) tr[3] = wall_clock64();
+    const bool good = *s_ok != 0;
+    if (threadIdx.x == 0) {
+      uint32_t flags = 0;
+      if (good) {
+        const unsigned long long hits[2] = {s_red[0] + s_red[1] + s_red[2] + s_red[3],
+                                            s_red[4] + s_red[5] + s_red[6] + s_red[7]};
+        const int tp = (int)(tdone - d.t_start);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          if (ps.cut[m] == 0 || ps.done_from[m] <= tp || hits[m] == 0) continue;
+          if (tp + 1 < d.hist_len) d.hist[m][tp + 1] = (uint32_t)hits[m];
+          ps.cov[m] += hits[m];
+          ps.sum[m] += hits[m] * (unsigned long long)(tp + 1);
+          ps.hd[m] = (uint32_t)(tp + 1);
+          if (ps.cov[m] >= ps.cut[m]) ps.done_from[m] = tp + 1;
+        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+          if (ps.cut[m] == 0 || ps.done_from[m] <= tp + 1) flags |= 1u << m;
+        es->qtotal += L;
+        es->qmax = max(es->qmax, L);
+        if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
+      } else {
+        flags = 4u;
+        atomicOr(&es->error, 1u);
+      }
+      s_flags[0] = flags;
+      uint32_t acc = 0;
+      for (int k = 0; k < kSubQ; ++k) {
+        s_pre[k] = acc;
+        acc += (uint32_t)s_red[8 + k];
+      }
+      s_pre[kSubQ] = acc;
+    }
+    // the counters that iteration epoch+1 fills ((t+2) % 4 with t = epoch)
+    if (threadIdx.x < kSubQ) st_sc(&es->qlen[(epoch + 2) % 4][threadIdx.x], 0u);
+    __syncthreads();
+    if (tr && threadIdx.x == 0) tr[4] = wall_clock64();
+    for (int j = threadIdx.x; j < kReleaseCopies * kWords; j += blockDim.x) {
+      const int w = j % kWords;
+      st_sc(&es->rel2[j / kWords][w], tag(epoch, w == 0 ? s_flags[0] : s_pre[w - 1]));
+    }
+  } else {
+    __shared__ unsigned long long s_rel[kWords];
+    if (threadIdx.x < kWords) {
+      const unsigned long long *line = es->rel2[blockIdx.x % kReleaseCopies];
+      const uint64_t t0 = wall_clock64();
+      unsigned long long w;
+      for (;;) {
+        w = ld_sc(&line[threadIdx.x]);
+        if (tagged(w, epoch)) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+          atomicOr(&es->error, 1u);
+          w = 4u;  // flags word or not, the workgroup leaves
+          *s_ok = 0;
+          break;
+        }
+      }
+      s_rel[threadIdx.x] = w & kTagValue;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      s_flags[0] = (uint32_t)s_rel[0];
+      for (int k = 0; k <= kSubQ; ++k) s_pre[k] = (uint32_t)s_rel[1 + k];
+      if (s_flags[0] & 4u) *s_ok = 0;
+    }
+  }
+  __syncthreads();
   return *s_ok != 0;
 }
 
@@ -736,6 +948,11 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   __shared__ HitTable ht;
   __shared__ uint32_t s_pre[kSubQ + 1];
   __shared__ int s_ok;
+  // single pair: this iteration's hits per pass and entries pushed, the
+  // released flags, and the master's reduction space
+  __shared__ uint32_t s_cnt[3];
+  __shared__ uint32_t s_flags[1];
+  __shared__ unsigned long long s_red[8 + kSubQ];
   const bool fence = (flags & 1) != 0;
   constexpr bool single = SINGLE;
   const int lane = threadIdx.x & 63;
@@ -750,6 +967,18 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   const uint32_t kq = blockIdx.x % kSubQ;
   if (threadIdx.x == 0) lq.n = 0;
   if (threadIdx.x < kHitTable) ht.key[threadIdx.x] = ~0u, ht.cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+  PassState ps{};
+  if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's registers
+    const PairCtr &c0 = ctrs[0];
+    for (int m = 0; m < 2; ++m) {
+      ps.cut[m] = robust_cut(descs[0].robust, ld_sc(m == 0 ? &c0.n_fp : &c0.n_fn));
+      ps.cov[m] = ld_sc(&c0.cov[m]);
+      ps.sum[m] = ld_sc(&c0.sum_d[m]);
+      ps.hd[m] = ld_sc(&c0.hd_last[m]);
+      ps.done_from[m] = ld_sc(&c0.done_from[m]);
+    }
+  }
   __syncthreads();
   uint32_t t = ld_sc(&es->t);
   for (int it = 0; it < max_iters; ++it) {
@@ -760,7 +989,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         uint32_t acc = 0;
         for (int k = 0; k < kSubQ; ++k) {
           s_pre[k] = acc;
-          acc += ld_sc(&es->qlen[t % 3][k]);
+          acc += ld_sc(&es->qlen[t % 4][k]);
         }
         s_pre[kSubQ] = acc;
       }
@@ -770,9 +999,14 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     if (L == 0) break;
     bool pdone[2] = {false, false};
     if (single) {  // stop as soon as both passes are done (uniform decision)
-      const int tp0 = (int)(t - descs[0].t_start);
-      pdone[0] = pass_done(descs[0], ctrs[0], 0, tp0);
-      pdone[1] = pass_done(descs[0], ctrs[0], 1, tp0);
+      if (it == 0) {
+        const int tp0 = (int)(t - descs[0].t_start);
+        pdone[0] = pass_done(descs[0], ctrs[0], 0, tp0);
+        pdone[1] = pass_done(descs[0], ctrs[0], 1, tp0);
+      } else {  // delivered with the release
+        pdone[0] = (s_flags[0] & 1u) != 0;
+        pdone[1] = (s_flags[0] & 2u) != 0;
+      }
       if (pdone[0] && pdone[1]) break;
     }
     // per-iteration trace (diagnostic): [0] start, [1] last workgroup done,
@@ -783,7 +1017,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       tr[5] = L;
     }
     uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
-    uint32_t *qout_len = &es->qlen[(t + 1) % 3][kq];
+    uint32_t *qout_len = &es->qlen[(t + 1) % 4][kq];
     for (uint32_t base = gwave * step; base < L; base += nwave * step) {
       for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
         const uint32_t ei = base + k + sub;
@@ -910,7 +1144,12 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
             d.map[vrow + kb] = (uint16_t)(tp + 1);
           }
         }
-        if (live && r == 0 && cnt) hit_add(ht, ctrs, (uint32_t)p * 2 + m, cnt);
+        if (live && r == 0 && cnt) {
+          if (SINGLE)
+            atomicAdd(&s_cnt[m], cnt);
+          else
+            hit_add(ht, ctrs, (uint32_t)p * 2 + m, cnt);
+        }
         // -- outputs
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
@@ -943,7 +1182,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
             }
           }
         }
-        push_entries(want, pent, lq, Qout, qout_len, lane);
+        push_entries(want, pent, lq, Qout, qout_len, lane, SINGLE ? &s_cnt[2] : nullptr);
       }
     }
     if (tr) {
@@ -954,8 +1193,12 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         if (blockIdx.x == 0) tr[2] = now;
       }
     }
-    flush_local_queue(lq, Qout, qout_len);
-    hit_flush(ht, ctrs);
+    const uint32_t nflushed = flush_local_queue(lq, Qout, qout_len);
+    if (SINGLE) {
+      if (threadIdx.x == 0) s_cnt[2] += nflushed;
+    } else {
+      hit_flush(ht, ctrs);
+    }
     const uint32_t tdone = t;  // iteration that just ran
     ++t;
     // master bookkeeping between iterations: fold the spread hit counters into
@@ -992,14 +1235,29 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
         if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
       }
-      if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 1) % 3][threadIdx.x], 0u);
+      if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 2) % 4][threadIdx.x], 0u);
       if (threadIdx.x == 0) {
         es->qtotal += L;
         es->qmax = max(es->qmax, L);
         if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
       }
     };
-    if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 3], s_pre)) break;
+    if (SINGLE) {
+      if (!grid_barrier_single(es, t, spin_limit, &s_ok, s_cnt, ps, descs[0], tdone, L, qstats,
+                               qstats_len, tr, s_pre, s_flags, s_red))
+        break;
+    } else if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 4], s_pre)) {
+      break;
+    }
+  }
+  if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's state, for finalize
+    PairCtr &c0 = ctrs[0];
+    for (int m = 0; m < 2; ++m) {
+      st_sc(&c0.cov[m], ps.cov[m]);
+      st_sc(&c0.sum_d[m], ps.sum[m]);
+      st_sc(&c0.hd_last[m], ps.hd[m]);
+      st_sc(&c0.done_from[m], ps.done_from[m]);
+    }
   }
   // workgroup 0 holds every fold (it is the barrier master): it writes the
   // results of all pairs and the engine head
@@ -1035,10 +1293,10 @@ hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
   return hipGetLastError();
 }
 template <int TXB, int TY, int TZ>
-hipError_t launch_meta(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
+hipError_t launch_seed(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
                        PairCtr *c, int np, Engine *es, uint64_t *q, uint32_t qcap, uint64_t tt,
                        int p0) {
-  hipLaunchKernelGGL((meta_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, es,
+  hipLaunchKernelGGL((seed_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, es,
                      q, qcap, tt, p0);
   return hipGetLastError();
 }
@@ -1054,7 +1312,7 @@ hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
 template <int TXB, int TY, int TZ>
 TileLaunch make_launch() {
   return TileLaunch{&launch_pack<TXB, TY, TZ>,
-                    &launch_meta<TXB, TY, TZ>,
+                    &launch_seed<TXB, TY, TZ>,
                     &launch_dilate<TXB, TY, TZ>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, false>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true>,
@@ -1084,7 +1342,8 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
   hdm::PackIndex one_idx{};
   if (single) one = *single;
   if (single_idx) one_idx = *single_idx;
-  const int blocks = std::max(1, std::min(npairs, 1024));
+  // at least 16 blocks: the engine reset (about 140 KiB) is spread over them
+  const int blocks = std::max(16, std::min(npairs, 1024));
   hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(hdm::kWgThreads), 0, s, descs, one,
                      one_idx, idx, single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
   return hipGetLastError();

@@ -39,7 +39,7 @@ struct DilateArgs {
 struct TileLaunch {
   hipError_t (*pack)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      uint64_t);
-  hipError_t (*meta)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
+  hipError_t (*seed)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
   hipError_t (*dilate)(dim3, hipStream_t, const DilateArgs &);
   const void *dilate_fn;         // several pairs
@@ -71,7 +71,7 @@ struct PairGeom {
   uint64_t words;      // ntiles * rows
   uint64_t row_words;  // wxn * ny * nz, the pack grid
   int hist_len;
-  uint64_t buf_bytes, xf_bytes, st_bytes, hist_bytes, total;
+  uint64_t buf_bytes, xf_bytes, st_bytes, tf_bytes, hist_bytes, total;
 };
 
 inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
@@ -88,8 +88,9 @@ inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
   g.buf_bytes = align256(g.words * (ts.txb / 8));
   g.xf_bytes = align256((uint64_t)g.ntiles * 16);
   g.st_bytes = align256((uint64_t)g.ntiles * 4);
+  g.tf_bytes = align256((uint64_t)g.ntiles * 2);
   g.hist_bytes = align256((uint64_t)g.hist_len * 4);
-  g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + 2 * g.hist_bytes;
+  g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + g.tf_bytes + 2 * g.hist_bytes;
   return g;
 }
 
@@ -111,6 +112,7 @@ inline void carve_pair(PairDesc &d, const PairGeom &g, char *base) {
     for (int s = 0; s < 2; ++s) d.xf_buf[m][s] = (uint64_t *)take(g.xf_bytes);
   d.state[0] = (uint32_t *)take(g.st_bytes);
   d.state[1] = (uint32_t *)take(g.st_bytes);
+  d.tflags = (uint16_t *)take(g.tf_bytes);
   d.hist[0] = (uint32_t *)take(g.hist_bytes);
   d.hist[1] = (uint32_t *)take(g.hist_bytes);
 }

@@ -292,10 +292,9 @@ static int admit(hdm_server *s, Request &r, int p) {
   if (e != hipSuccess) return map_hip(e);
   const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1);
   e = s->tl->pack(dim3(lc.pack_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
-                  r.geom.row_words);
-  if (e == hipSuccess) e = hdm_launch_totals(s->s, s->P.ctrs, p, 1);
-  if (e == hipSuccess)
-    e = s->tl->meta(dim3(lc.meta_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
+                  r.geom.ntiles);
+  if (e == hipSuccess)  // one pair: the seed pass folds the totals itself
+    e = s->tl->seed(dim3(lc.seed_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
                     s->P.engine, s->P.q, s->qcap, r.geom.ntiles, p);
   if (e != hipSuccess) return map_hip(e);
   r.slot = p;
