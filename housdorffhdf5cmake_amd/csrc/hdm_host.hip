@@ -122,9 +122,9 @@ int get_ctx(Ctx **out) {
 int resident_grid(const Ctx *c, int tile, int requested) {
   const int maxres = std::min(c->occ[tile] * c->cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
-  // 2 workgroups per CU: dense iterations want many waves, sparse ones a
-  // short barrier; the autotuner picks per workload
-  return std::min(maxres, 2 * c->cus);
+  // every resident workgroup: dense iterations want many waves, and the
+  // barrier polls arrivals in batches, so a sparse queue pays little for them
+  return maxres;
 }
 
 uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.0); }

@@ -298,6 +298,8 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
   const uint64_t per = ((total_tiles + nw - 1) / nw + kStep - 1) / kStep * kStep;
   const uint64_t w0 = gw * per, w1 = min(total_tiles, w0 + per);
   const int slot = (int)(gw % kSumSlots);
+  // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
+  __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
   PackAcc acc;
   acc_clear(acc);
   int ap = -1;  // pair the accumulator belongs to (wave-uniform)
@@ -331,7 +333,56 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
         okU[u] = y < d.ny && z < d.nz;
         cntU[u] = min(TXB, d.nx - txU[u] * TXB);
         vbU[u] = (uint64_t)txU[u] * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
-        if (okU[u]) {
+      }
+    }
+    // Two x-adjacent interior 64x8x8 tiles of a uint8 pair: their rows are
+    // 128 contiguous bytes, so each load instruction reads 8 whole 128-byte
+    // lines (lane L: row y+L/8 of z-slice k, bytes 16*(L%8) ..), and the 16-bit
+    // compare pieces go through LDS to the lane that owns the row.
+    bool fast = false;
+    if constexpr (TXB == 64 && TY == 8 && TZ == 8) {
+      const PairDesc &d = descs[pU[0]];
+      fast = inU[0] && inU[1] && pU[0] == pU[1] && d.dtype == 0 && txU[1] == txU[0] + 1 &&
+             tyU[1] == tyU[0] && tzU[1] == tzU[0] && (txU[1] + 1) * 64 <= d.nx &&
+             (tyU[0] + 1) * 8 <= d.ny && (tzU[0] + 1) * 8 <= d.nz && (d.nx & 15) == 0 &&
+             (((uintptr_t)d.gold | (uintptr_t)d.segm) & 15) == 0;
+      fast = __builtin_amdgcn_readfirstlane((int)fast) != 0;  // one tile pair per wave
+      if (fast) {
+        const uint64_t nx = (uint64_t)d.nx, nxy = nx * (uint64_t)d.ny;
+        const uint64_t b0 = (uint64_t)txU[0] * 64 + nx * ((uint64_t)tyU[0] * 8) +
+                            nxy * ((uint64_t)tzU[0] * 8) + nx * (uint64_t)(lane >> 3) +
+                            16u * (uint64_t)(lane & 7);
+        const uint8_t *gp = (const uint8_t *)d.gold + b0, *sp = (const uint8_t *)d.segm + b0;
+        u32x4 vg[8], vs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          vg[k] = *(const u32x4 *)(gp + nxy * k);
+          vs[k] = *(const u32x4 *)(sp + nxy * k);
+        }
+        const uint32_t lab4 = (uint32_t)(uint8_t)d.label * 0x01010101u;
+        uint16_t *pc = s_pieces[threadIdx.x >> 6];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {  // row 8k + L/8, piece L%8
+          pc[(8 * k + (lane >> 3)) * 8 + (lane & 7)] = (uint16_t)eq_bits_u8x16(vg[k], lab4);
+          pc[512 + (8 * k + (lane >> 3)) * 8 + (lane & 7)] = (uint16_t)eq_bits_u8x16(vs[k], lab4);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint64_t *row = (const uint64_t *)(pc + lane * 8);
+        const uint64_t *rows = (const uint64_t *)(pc + 512 + lane * 8);
+        aU[0] = (Row)row[0], aU[1] = (Row)row[1];
+        bU[0] = (Row)rows[0], bU[1] = (Row)rows[1];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();  // the pieces are read before the next step writes
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    if (!fast) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (inU[u] && okU[u]) {
+          const PairDesc &d = descs[pU[u]];
           aU[u] = load_row_bits<Row>(d, d.gold, vbU[u], cntU[u]);
           bU[u] = load_row_bits<Row>(d, d.segm, vbU[u], cntU[u]);
         }
@@ -656,15 +707,27 @@ __device__ __forceinline__ bool grid_barrier(Engine *es, uint32_t epoch, uint64_
     __syncthreads();
     int ok = 1;
     const uint64_t t0 = wall_clock64();
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
-      while (ld_sc(&es->arrive[i]) != epoch) {
+    constexpr int kPoll = 4;  // flags polled together per thread (one round trip each poll)
+    for (uint32_t i0 = threadIdx.x; i0 < gridDim.x && ok; i0 += kPoll * blockDim.x) {
+      uint32_t pending = 0;
+#pragma unroll
+      for (int k = 0; k < kPoll; ++k)
+        if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
+      while (pending) {
+        uint32_t f[kPoll];
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k)
+          f[k] = ((pending >> k) & 1) ? ld_sc(&es->arrive[i0 + k * blockDim.x]) : epoch;
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k)
+          if (f[k] == epoch) pending &= ~(1u << k);
+        if (!pending) break;
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
           ok = 0;
           break;
         }
       }
-      if (!ok) break;
     }
     if (!ok) {
       atomicOr(&es->error, 1u);
@@ -750,15 +813,35 @@ __device__ __forceinline__ bool grid_barrier_single(
     unsigned long long h0 = 0, h1 = 0, np = 0;
     int ok = 1;
     const uint64_t t0 = wall_clock64();
-    for (uint32_t i = threadIdx.x; i < gridDim.x && ok; i += blockDim.x) {
-      for (;;) {
-        const unsigned long long w0 = ld_sc(&es->arr[i][0]);
-        const unsigned long long w1 = ld_sc(&es->arr[i][1]);
-        const unsigned long long w2 = ld_sc(&es->arr[i][2]);
-        if (tagged(w0, epoch) && tagged(w1, epoch) && tagged(w2, epoch)) {
-          h0 += w0 & kTagValue, h1 += w1 & kTagValue, np += w2 & kTagValue;
-          break;
+    // up to kPoll records per thread are polled together: all their loads are
+    // in flight at once, so a grid of kPoll * 256 workgroups costs one round
+    // trip per poll, not one per workgroup
+    constexpr int kPoll = 4;
+    for (uint32_t i0 = threadIdx.x; i0 < gridDim.x && ok; i0 += kPoll * blockDim.x) {
+      uint32_t pending = 0;
+#pragma unroll
+      for (int k = 0; k < kPoll; ++k)
+        if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
+      while (pending) {
+        unsigned long long w[kPoll][3];
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k) {
+          const uint32_t i = i0 + k * blockDim.x;
+          if ((pending >> k) & 1) {
+            w[k][0] = ld_sc(&es->arr[i][0]);
+            w[k][1] = ld_sc(&es->arr[i][1]);
+            w[k][2] = ld_sc(&es->arr[i][2]);
+          }
         }
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k) {
+          if (((pending >> k) & 1) && tagged(w[k][0], epoch) && tagged(w[k][1], epoch) &&
+              tagged(w[k][2], epoch)) {
+            h0 += w[k][0] & kTagValue, h1 += w[k][1] & kTagValue, np += w[k][2] & kTagValue;
+            pending &= ~(1u << k);
+          }
+        }
+        if (!pending) break;
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
           ok = 0;
