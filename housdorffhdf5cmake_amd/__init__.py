@@ -157,8 +157,14 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+_OPTS_DEFAULT = None
+
+
 def _current_stream_ptr():
     import torch
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:  # the same handle as current_stream().cuda_stream, without the object
+        return C.c_void_p(raw(torch.cuda.current_device()))
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
@@ -203,8 +209,11 @@ def _volume_args(gold, segm):
 
 def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, stream=None,
               ptr_kind=PTR_DEVICE, timing=False) -> Opts:
-    o = Opts()
-    lib().hdm_opts_init(C.byref(o))
+    global _OPTS_DEFAULT
+    if _OPTS_DEFAULT is None:
+        _OPTS_DEFAULT = Opts()
+        lib().hdm_opts_init(C.byref(_OPTS_DEFAULT))
+    o = Opts.from_buffer_copy(_OPTS_DEFAULT)
     o.robustness = float(robustness)
     o.tile = int(tile)
     o.grid_ctas = int(grid_ctas)

@@ -43,6 +43,7 @@ struct LaunchCfg {
   int grid = 1, chunk = 1;
   uint64_t spin = 0;
   const hipEvent_t *ev = nullptr;  // 4 events (phase bounds) or null
+  int ev_mode = 0;                 // opts.timing: 1 or 2 every phase, 3 the dilation only
   uint64_t *trace = nullptr;       // per-iteration device trace or null
   PairOut *outs = nullptr;         // where the results go (null: the group's outs)
   EngineHead *head = nullptr;      // where the engine head goes (null: nowhere)

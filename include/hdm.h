@@ -76,7 +76,8 @@ typedef struct hdm_opts {
   int32_t grid_ctas;  /* persistent-kernel workgroups; 0 = occupancy-derived           */
   int32_t chunk;      /* queue entries a wave takes per claim; 0 = default             */
   int32_t crop;       /* 1 (default) = restrict dilation to the tile bounding box      */
-  int32_t timing;     /* 1 = time the phases with HIP events (hdm_last_timings)        */
+  int32_t timing;     /* HIP-event timing (hdm_last_timings): 1 every phase, 2 also a
+                         per-iteration trace (hdm_last_trace), 3 the dilation only */
   void *stream;       /* hipStream_t or NULL                                           */
   uint16_t *map_out;  /* optional nx*ny*nz first-hit map: d(v) on FP u FN, 0 elsewhere */
   uint32_t *hist_out; /* optional 2*hist_len first-hit histograms (FP then FN), host   */
@@ -110,8 +111,11 @@ int hdm_device_count(void);
 uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
 /* Device milliseconds of the phases of the last single-pair call on the
  * current device that ran with opts->timing = 1, from HIP events on its
- * stream: ms[0] pack (+ voxel totals), ms[1] tile metadata, ms[2] persistent
- * dilation, ms[3] finalize, ms[4] all kernels.  Returns how many were written. */
+ * stream: ms[0] pack (+ reset), ms[1] tile states and first queue, ms[2]
+ * persistent dilation (which also finalizes), ms[3] 0, ms[4] all kernels.
+ * With opts->timing = 3 only the dilation is bracketed (ms[2]; the others
+ * are -1): its events are queued while the pack runs, so they cost no time.
+ * Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
 /* Per-iteration device trace of the last single-pair call on the current
  * device that ran with opts->timing = 2 (observability; no counterpart in the
