@@ -33,14 +33,34 @@ namespace hdm {
 template <int TXB>
 using RowT = std::conditional_t<TXB == 64, uint64_t, uint32_t>;
 
+// Every pointer these helpers take is global memory.  Pointers read out of a
+// descriptor are generic to the compiler, which would emit flat accesses
+// (waited for on both the vector and the LDS counters); the casts make them
+// global_load / global_store / global_atomic.
+#define GLB __attribute__((address_space(1)))
+
 // agent-coherent load / store (global_load / global_store with sc1)
 template <typename T>
 __device__ __forceinline__ T ld_sc(const T *p) {
-  return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, AGENT);
+  return __hip_atomic_load((GLB T *)p, __ATOMIC_RELAXED, AGENT);
 }
 template <typename T>
 __device__ __forceinline__ void st_sc(T *p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, AGENT);
+  __hip_atomic_store((GLB T *)p, v, __ATOMIC_RELAXED, AGENT);
+}
+// plain global load (data no launch in flight writes: packed originals)
+template <typename T>
+__device__ __forceinline__ T ldg(const T *p) {
+  return *(const GLB T *)p;
+}
+// compare-and-swap at agent scope; returns the value found
+__device__ __forceinline__ uint32_t cas_g(uint32_t *p, uint32_t expect, uint32_t v) {
+  __hip_atomic_compare_exchange_strong((GLB uint32_t *)p, &expect, v, __ATOMIC_RELAXED,
+                                       __ATOMIC_RELAXED, AGENT);
+  return expect;
+}
+__device__ __forceinline__ void or_g(uint32_t *p, uint32_t v) {
+  __hip_atomic_fetch_or((GLB uint32_t *)p, v, __ATOMIC_RELAXED, AGENT);
 }
 
 __device__ __forceinline__ uint32_t shfl32(uint32_t v, int src) {
@@ -504,10 +524,44 @@ __device__ __forceinline__ Box pair_box(const PairDesc &d, const PairCtr &c) {
   return pair_box(d, c.lo, c.hi);
 }
 
+// What a dilation step reads of its pair, with the box applied: for one pair
+// the launch keeps it in registers, for several a step loads it in one batch.
+struct PairView {
+  const void *orig[2];
+  void *buf[2][2];
+  const uint64_t *xf_orig[2];
+  uint64_t *xf_buf[2][2];
+  uint32_t *state[2];
+  uint16_t *map;
+  int nx, ny, nz, gx, gy, hist_len;
+  uint32_t t_start, gen;
+  Box b;
+  unsigned long long cut[2];  // robust cut per pass; 0: nothing to do
+};
+__device__ __forceinline__ PairView make_view(const PairDesc &d, const PairCtr &c) {
+  PairView v;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    v.orig[m] = d.orig[m];
+    v.buf[m][0] = d.buf[m][0], v.buf[m][1] = d.buf[m][1];
+    v.xf_orig[m] = d.xf_orig[m];
+    v.xf_buf[m][0] = d.xf_buf[m][0], v.xf_buf[m][1] = d.xf_buf[m][1];
+    v.state[m] = d.state[m];
+  }
+  v.map = d.map;
+  v.nx = d.nx, v.ny = d.ny, v.nz = d.nz, v.gx = d.gx, v.gy = d.gy, v.hist_len = d.hist_len;
+  v.t_start = d.t_start, v.gen = d.gen;
+  v.b = pair_box(d, c);
+  v.cut[0] = robust_cut(d.robust, c.n_fp);
+  v.cut[1] = robust_cut(d.robust, c.n_fn);
+  return v;
+}
+
 // neighbour tile of `tile` in direction dir (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z),
-// or kNoTile when it is outside the (cropped) grid
-__device__ __forceinline__ uint32_t neighbour(const PairDesc &d, const Box &b, uint32_t tile,
-                                              int tx, int ty, int tz, int dir) {
+// or kNoTile when it is outside the (cropped) grid; G has the grid dims gx, gy
+template <typename G>
+__device__ __forceinline__ uint32_t neighbour(const G &d, const Box &b, uint32_t tile, int tx,
+                                              int ty, int tz, int dir) {
   switch (dir) {
     case 0: return tx > b.lo[0] ? tile - 1 : kNoTile;
     case 1: return tx < b.hi[0] ? tile + 1 : kNoTile;
@@ -1051,6 +1105,8 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   if (threadIdx.x == 0) lq.n = 0;
   if (threadIdx.x < kHitTable) ht.key[threadIdx.x] = ~0u, ht.cnt[threadIdx.x] = 0;
   if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+  PairView v0{};  // one pair: its fields for the whole launch
+  if constexpr (SINGLE) v0 = make_view(descs[0], ctrs[0]);
   PassState ps{};
   if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's registers
     const PairCtr &c0 = ctrs[0];
@@ -1068,13 +1124,11 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     // queue of iteration t: sub-queue lengths -> prefix offsets (after the
     // first iteration the barrier delivers them with the release)
     if (it == 0) {
+      if (threadIdx.x < kSubQ) s_pre[threadIdx.x + 1] = ld_sc(&es->qlen[t % 4][threadIdx.x]);
+      __syncthreads();
       if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int k = 0; k < kSubQ; ++k) {
-          s_pre[k] = acc;
-          acc += ld_sc(&es->qlen[t % 4][k]);
-        }
-        s_pre[kSubQ] = acc;
+        s_pre[0] = 0;
+        for (int k = 1; k <= kSubQ; ++k) s_pre[k] += s_pre[k - 1];
       }
       __syncthreads();
     }
@@ -1116,64 +1170,81 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         const int m = (int)((ent >> 48) & 1);
         const bool first = (ent >> 49) & 1;
         const bool check = (ent >> 50) & 1;
-        const PairDesc &d = descs[p];
-        PairCtr &c = ctrs[p];
-        const uint32_t tp = t - d.t_start;  // pair-local iteration
-        if (!SINGLE && live && entry_gen(ent) != (d.gen & kGenMask)) live = false;  // stale
-        if (live && (SINGLE ? pdone[m] : pass_done(d, c, m, (int)tp)))  // hello.cu:1748-1759
-          live = false;
-        if (live && tp + 1 >= (uint32_t)d.hist_len) {
-          c.status = -8;
+        // the pair's fields: registers for the launch (one pair) or one batch
+        // of loads together with the pass's stop state (several pairs)
+        PairView v;
+        int dfrom = 0x7FFFFFFF;
+        if constexpr (SINGLE) {
+          v = v0;
+        } else {
+          v = make_view(descs[p], ctrs[p]);
+          dfrom = ld_sc(&ctrs[p].done_from[m]);
+        }
+        const uint32_t tp = t - v.t_start;  // pair-local iteration
+        if (!SINGLE && live && entry_gen(ent) != (v.gen & kGenMask)) live = false;  // stale
+        // a pass whose cut was reached stops (hello.cu:1748-1759)
+        const unsigned long long cut_m = m ? v.cut[1] : v.cut[0];
+        if (live && (SINGLE ? pdone[m] : (cut_m == 0 || dfrom <= (int)tp))) live = false;
+        if (live && tp + 1 >= (uint32_t)v.hist_len) {
+          ctrs[p].status = -8;
           live = false;
         }
-        const int tx = (int)(tile % (uint32_t)d.gx);
-        const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
-        const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+        const int tx = (int)(tile % (uint32_t)v.gx);
+        const int ty = (int)((tile / (uint32_t)v.gx) % (uint32_t)v.gy);
+        const int tz = (int)(tile / ((uint32_t)v.gx * v.gy));
         const int y = ty * TY + yl, z = tz * TZ + zl;
-        const bool exists = y < d.ny && z < d.nz;
+        const bool exists = y < v.ny && z < v.nz;
         const bool rowok = live && exists;
-        const Row vb = valid_bits<Row>(tx, d.nx);
-        const Box b = pair_box(d, c);
+        const Row vb = valid_bits<Row>(tx, v.nx);
         const int rd = (tp + 1) & 1;  // slot holding iteration tp's input
         const int wr = tp & 1;        // slot receiving iteration tp's output
-        const Row *orig_m = (const Row *)d.orig[m];
-        const Row *bufr = (const Row *)d.buf[m][rd];
+        // pointer selects, not indexing: indexing the view with run-time
+        // values would put it in scratch memory
+        const Row *orig_m = (const Row *)(m ? v.orig[1] : v.orig[0]);
+        const Row *orig_o = (const Row *)(m ? v.orig[0] : v.orig[1]);
+        Row *buf0 = (Row *)(m ? v.buf[1][0] : v.buf[0][0]);
+        Row *buf1 = (Row *)(m ? v.buf[1][1] : v.buf[0][1]);
+        const Row *bufr = rd ? buf1 : buf0;
+        Row *bufw = wr ? buf1 : buf0;
+        const uint64_t *xfo = m ? v.xf_orig[1] : v.xf_orig[0];
+        uint64_t *xf0 = m ? v.xf_buf[1][0] : v.xf_buf[0][0];
+        uint64_t *xf1 = m ? v.xf_buf[1][1] : v.xf_buf[0][1];
+        const uint64_t *xfr = rd ? xf1 : xf0;
+        uint64_t *xfw = wr ? xf1 : xf0;
+        uint32_t *st_m = m ? v.state[1] : v.state[0];
         const uint64_t ro = (uint64_t)tile * R + r;
+        // neighbour tiles: lanes r < 6 also read the state of neighbour r
+        const uint32_t nb_dir = (live && r < 6) ? neighbour(v, v.b, tile, tx, ty, tz, r) : kNoTile;
+        const uint32_t nb_xl = live ? neighbour(v, v.b, tile, tx, ty, tz, 0) : kNoTile;
+        const uint32_t nb_xh = live ? neighbour(v, v.b, tile, tx, ty, tz, 1) : kNoTile;
+        const int on_face[4] = {yl == 0, yl == TY - 1, zl == 0, zl == TZ - 1};
+        const int src_row[4] = {(TY - 1) + TY * zl, TY * zl, yl + TY * (TZ - 1), yl};
+        uint32_t nb_yz[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+          nb_yz[f] = (rowok && on_face[f]) ? neighbour(v, v.b, tile, tx, ty, tz, 2 + f) : kNoTile;
         // -- issue every load of this step before using any of them
         Row own = 0;
-        if (rowok) own = first ? orig_m[ro] : ld_sc(&bufr[ro]);
-        const Row other = (rowok && check) ? ((const Row *)d.orig[1 - m])[ro] : Row(0);
-        uint32_t nb_dir = kNoTile, nst = 0;
-        if (live && r < 6) {
-          nb_dir = neighbour(d, b, tile, tx, ty, tz, r);
-          if (nb_dir != kNoTile) nst = ld_sc(&d.state[m][nb_dir]);
-        }
-        const uint32_t nb_xl = live ? neighbour(d, b, tile, tx, ty, tz, 0) : kNoTile;
-        const uint32_t nb_xh = live ? neighbour(d, b, tile, tx, ty, tz, 1) : kNoTile;
+        if (rowok) own = first ? ldg(orig_m + ro) : ld_sc(bufr + ro);
+        const Row other = (rowok && check) ? ldg(orig_o + ro) : Row(0);
+        const uint32_t nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
         uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
         if (nb_xl != kNoTile) {
-          fxl_o = d.xf_orig[m][2 * (uint64_t)nb_xl + 1];
-          fxl_b = ld_sc(&d.xf_buf[m][rd][2 * (uint64_t)nb_xl + 1]);
+          fxl_o = ldg(xfo + 2 * (uint64_t)nb_xl + 1);
+          fxl_b = ld_sc(xfr + 2 * (uint64_t)nb_xl + 1);
         }
         if (nb_xh != kNoTile) {
-          fxh_o = d.xf_orig[m][2 * (uint64_t)nb_xh];
-          fxh_b = ld_sc(&d.xf_buf[m][rd][2 * (uint64_t)nb_xh]);
+          fxh_o = ldg(xfo + 2 * (uint64_t)nb_xh);
+          fxh_b = ld_sc(xfr + 2 * (uint64_t)nb_xh);
         }
         // y / z face rows from the neighbour tiles, both candidate sources
-        uint32_t nb_yz[4] = {kNoTile, kNoTile, kNoTile, kNoTile};
         Row fo[4] = {0, 0, 0, 0}, fb[4] = {0, 0, 0, 0};
-        if (rowok) {
-          const int on_face[4] = {yl == 0, yl == TY - 1, zl == 0, zl == TZ - 1};
-          const int src_row[4] = {(TY - 1) + TY * zl, TY * zl, yl + TY * (TZ - 1), yl};
 #pragma unroll
-          for (int f = 0; f < 4; ++f) {
-            if (!on_face[f]) continue;
-            nb_yz[f] = neighbour(d, b, tile, tx, ty, tz, 2 + f);
-            if (nb_yz[f] == kNoTile) continue;
-            const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
-            fo[f] = orig_m[o];
-            fb[f] = ld_sc(&bufr[o]);
-          }
+        for (int f = 0; f < 4; ++f) {
+          if (nb_yz[f] == kNoTile) continue;
+          const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
+          fo[f] = ldg(orig_m + o);
+          fb[f] = ld_sc(bufr + o);
         }
         // -- neighbour states -> which candidate holds iteration tp's input
         const uint32_t st_xl = shfl32(nst, gbase + 0), st_xh = shfl32(nst, gbase + 1);
@@ -1217,14 +1288,14 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         const Row nw = dl & (Row)~own & other;
         const uint32_t cnt = group_sum<R>((uint32_t)popc_row(nw), lane);
         const uint32_t rem = group_sum<R>((uint32_t)popc_row((Row)(other & ~dl)), lane);
-        if (nw && d.map) {
+        if (nw && v.map) {
           Row bits = nw;
           const uint64_t vrow =
-              (uint64_t)tx * TXB + (uint64_t)d.nx * ((uint64_t)y + (uint64_t)d.ny * z);
+              (uint64_t)tx * TXB + (uint64_t)v.nx * ((uint64_t)y + (uint64_t)v.ny * z);
           while (bits) {
             const int kb = (int)__builtin_ctzll((uint64_t)bits);
             bits &= bits - 1;
-            d.map[vrow + kb] = (uint16_t)(tp + 1);
+            v.map[vrow + kb] = (uint16_t)(tp + 1);
           }
         }
         if (live && r == 0 && cnt) {
@@ -1244,23 +1315,23 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         uint64_t pent = 0;
         if (live) {
           if (full) {
-            if (r == 0) atomicOr(&d.state[m][tile], kFull);
+            if (r == 0) or_g(st_m + tile, kFull);
           } else {
-            st_sc(&((Row *)d.buf[m][wr])[ro], dl);
+            st_sc(bufw + ro, dl);
             if (r == 0) {
-              st_sc(&d.xf_buf[m][wr][2 * (uint64_t)tile], oxl);
-              st_sc(&d.xf_buf[m][wr][2 * (uint64_t)tile + 1], oxh);
+              st_sc(xfw + 2 * (uint64_t)tile, oxl);
+              st_sc(xfw + 2 * (uint64_t)tile + 1, oxh);
             }
             if (r == 6) {  // stays queued while not full
               want = true;
-              pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0, d.gen);
+              pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0, v.gen);
             }
             if (r < 6 && nb_dir != kNoTile && nst == 0) {  // activate idle neighbours
               const bool face_on = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
                                    : r == 3 ? oyh : r == 4 ? ozl : ozh;
-              if (face_on && atomicCAS(&d.state[m][nb_dir], 0u, tp + 2) == 0u) {
+              if (face_on && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
                 want = true;
-                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, d.gen);
+                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
               }
             }
           }
