@@ -278,6 +278,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 64^3 and the long-iteration 512^3 lines")
+    ap.add_argument("--no-small", action="store_true", help="skip the 64^3 line")
     ap.add_argument("--long-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -316,10 +317,11 @@ def main():
         torch.cuda.synchronize()
         del a, b
     extra = []
-    if not args.no_extra:
+    if not args.no_extra and not args.no_small:
         log("small 64^3 ...")
         g1, s1, r1, line["small_64"] = leg_small(hdm, torch, 200)
         extra.append(("small_64", g1, s1, r1))
+    if not args.no_extra:
         log("long-iteration 512^3 ...")
         g3, s3, r3, line["long_512"] = leg_long(hdm, torch, args.long_reps)
         extra.append(("long_512", g3, s3, r3))

@@ -144,6 +144,9 @@ struct Engine {
   // offsets of the next queue.
   unsigned long long arr[kMaxGrid][4];
   unsigned long long rel2[kReleaseCopies][kLineWords];
+  // tag(epoch, first global entry) of each workgroup's free wave slots: the
+  // master's prefix over workgroups of (slots - entries the workgroup kept)
+  unsigned long long fslot[kMaxGrid];
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel[0][i] = 0;
     for (int i = g; i < kMaxGrid * 4; i += gs) es->arr[0][i] = 0;
+    for (int i = g; i < kMaxGrid; i += gs) es->fslot[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel2[0][i] = 0;
   }
   for (int p = p0 + blockIdx.x; p < p0 + npairs; p += gridDim.x) {
@@ -532,6 +533,7 @@ struct PairView {
   const uint64_t *xf_orig[2];
   uint64_t *xf_buf[2][2];
   uint32_t *state[2];
+  uint32_t *hist0, *hist1;
   uint16_t *map;
   int nx, ny, nz, gx, gy, hist_len;
   uint32_t t_start, gen;
@@ -548,6 +550,7 @@ __device__ __forceinline__ PairView make_view(const PairDesc &d, const PairCtr &
     v.xf_buf[m][0] = d.xf_buf[m][0], v.xf_buf[m][1] = d.xf_buf[m][1];
     v.state[m] = d.state[m];
   }
+  v.hist0 = d.hist[0], v.hist1 = d.hist[1];
   v.map = d.map;
   v.nx = d.nx, v.ny = d.ny, v.nz = d.nz, v.gx = d.gx, v.gy = d.gy, v.hist_len = d.hist_len;
   v.t_start = d.t_start, v.gen = d.gen;
@@ -614,6 +617,39 @@ __device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue
     else
       lq.e[pos] = ent;
   }
+}
+
+// Pushes of a single-pair launch: kept in the workgroup's own LDS queue st
+// (count *stn, capacity cap) for its next iteration, so no global queue
+// round trip; when it is full they go to the staging queue lq, which the
+// workgroup flushes into the global queue once at the end of the iteration.
+__device__ __forceinline__ void push_sticky(bool want, uint64_t ent, uint64_t *st, uint32_t *stn,
+                                            uint32_t cap, LocalQueue &lq, uint64_t *gq,
+                                            uint32_t *gq_len, int lane, uint32_t *pushed) {
+  const uint64_t m = __ballot(want);
+  if (m == 0) return;
+  const uint32_t n = (uint32_t)__popcll(m);
+  uint32_t base = 0;
+  bool kept = false;
+  if (lane == 0) {  // reserve only if the whole run fits
+    uint32_t cur = *stn;
+    while (cur + n <= cap) {
+      const uint32_t prev = atomicCAS(stn, cur, cur + n);
+      if (prev == cur) {
+        base = cur;
+        kept = true;
+        break;
+      }
+      cur = prev;
+    }
+  }
+  kept = __shfl((int)kept, 0) != 0;
+  if (!kept) {  // staged; straight to gq (counted in *pushed) when lq is full too
+    push_entries(want, ent, lq, gq, gq_len, lane, pushed);
+    return;
+  }
+  base = (uint32_t)__shfl((int)base, 0);
+  if (want) st[base + lanes_below(m)] = ent;
 }
 
 // returns the number of entries flushed (the same in every thread)
@@ -832,39 +868,51 @@ struct PassState {
   unsigned long long cut[2], cov[2], sum[2];
   uint32_t hd[2];
   int done_from[2];
+  uint32_t cur_total;   // entries of the iteration running (global + kept)
+  uint32_t next_total;  // entries queued for the next one
 };
 
 // Grid barrier of a single-pair launch.  It has the same structure as
 // grid_barrier, but every word that crosses workgroups carries its epoch
 // (tag()), so neither side needs a second round trip:
 //   arrival  after s_waitcnt 0 and a workgroup barrier, thread 0 stores
-//            tag(epoch, hits of pass 0 / pass 1 / entries pushed);
+//            tag(epoch, x) for x = this iteration's hits of pass 0 and 1, the
+//            entries it pushed to the global queue, and the entries it kept
+//            in its own LDS queue for the next iteration;
 //   master   polls the records (thread i watches workgroups i, i+256, ...,
 //            which all flush into sub-queue i % kSubQ), sums them, folds the
 //            hits into the histogram and its PassState, decides the passes'
 //            stop (hello.cu:1748-1759), zeroes the queue counters of the
 //            iteration after next, and stores tag(epoch, flags) and
-//            tag(epoch, prefix offsets of the next queue) into every release
-//            replica;
+//            tag(epoch, prefix offsets of the next global queue) into every
+//            release replica;
 //   others   poll their replica until all words carry the epoch.
-// Waits are bounded as in grid_barrier.  s_flags receives the pass-done and
-// error flags, s_pre the prefix offsets.  Returns false on a timeout.
+// Flags: bit m pass m done, bit 2 error, bit 3 entries remain (global or kept).
+// The master also turns the kept counts into each workgroup's first global
+// entry: a prefix over workgroups of their free wave slots (slots minus kept
+// entries), stored tagged in es->fslot[wg] and read by each workgroup in the
+// same poll as its release replica; s_fs[0] receives it, s_fs[1] the total of
+// free slots, so global entries go first to waves that hold no kept entry.
+// Waits are bounded as in grid_barrier.  s_cnt[0..3] holds the four counts
+// and is zeroed here, as is *s_zero (the kept count of the queue that becomes
+// the next iteration's push target).  Returns false on a timeout.
 __device__ __forceinline__ bool grid_barrier_single(
-    Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, PassState &ps,
-    const PairDesc &d, uint32_t tdone, uint32_t L, uint32_t *qstats, int qstats_len,
-    uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red) {
+    Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, uint32_t *s_zero,
+    PassState &ps, const PairView &d, uint32_t tdone, uint32_t *qstats, int qstats_len,
+    uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red, uint32_t slots,
+    uint8_t *s_kept, uint32_t *s_fs) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     *s_ok = 1;
-    st_sc(&es->arr[blockIdx.x][0], tag(epoch, s_cnt[0]));
-    st_sc(&es->arr[blockIdx.x][1], tag(epoch, s_cnt[1]));
-    st_sc(&es->arr[blockIdx.x][2], tag(epoch, s_cnt[2]));
-    s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;  // the next iteration's counts start here
+#pragma unroll
+    for (int j = 0; j < 4; ++j) st_sc(&es->arr[blockIdx.x][j], tag(epoch, s_cnt[j]));
+    s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;  // the next iteration's counts start here
+    *s_zero = 0;
   }
-  constexpr int kWords = kSubQ + 2;  // flags + kSubQ + 1 prefix offsets
+  constexpr int kWords = kSubQ + 3;  // flags, kSubQ + 1 prefix offsets, free slots
   if (blockIdx.x == 0) {
-    unsigned long long h0 = 0, h1 = 0, np = 0;
+    unsigned long long h0 = 0, h1 = 0, np = 0, nl = 0;
     int ok = 1;
     const uint64_t t0 = wall_clock64();
     // up to kPoll records per thread are polled together: all their loads are
@@ -877,21 +925,21 @@ __device__ __forceinline__ bool grid_barrier_single(
       for (int k = 0; k < kPoll; ++k)
         if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
       while (pending) {
-        unsigned long long w[kPoll][3];
+        unsigned long long w[kPoll][4];
 #pragma unroll
         for (int k = 0; k < kPoll; ++k) {
-          const uint32_t i = i0 + k * blockDim.x;
           if ((pending >> k) & 1) {
-            w[k][0] = ld_sc(&es->arr[i][0]);
-            w[k][1] = ld_sc(&es->arr[i][1]);
-            w[k][2] = ld_sc(&es->arr[i][2]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[k][j] = ld_sc(&es->arr[i0 + k * blockDim.x][j]);
           }
         }
 #pragma unroll
         for (int k = 0; k < kPoll; ++k) {
           if (((pending >> k) & 1) && tagged(w[k][0], epoch) && tagged(w[k][1], epoch) &&
-              tagged(w[k][2], epoch)) {
-            h0 += w[k][0] & kTagValue, h1 += w[k][1] & kTagValue, np += w[k][2] & kTagValue;
+              tagged(w[k][2], epoch) && tagged(w[k][3], epoch)) {
+            h0 += w[k][0] & kTagValue, h1 += w[k][1] & kTagValue;
+            np += w[k][2] & kTagValue, nl += w[k][3] & kTagValue;
+            s_kept[i0 + k * blockDim.x] = (uint8_t)(w[k][3] & kTagValue);
             pending &= ~(1u << k);
           }
         }
@@ -903,31 +951,67 @@ __device__ __forceinline__ bool grid_barrier_single(
         }
       }
     }
-    // sums: hits per pass over all workgroups, pushes per sub-queue
+    // sums: hits per pass and kept entries over all workgroups, global pushes
+    // per sub-queue
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < kSubQ) s_red[8 + threadIdx.x] = 0;
+    if (threadIdx.x < kSubQ) s_red[12 + threadIdx.x] = 0;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
       h0 += shfl64(h0, lane ^ off);
       h1 += shfl64(h1, lane ^ off);
+      nl += shfl64(nl, lane ^ off);
     }
     __syncthreads();
-    if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1;
+    if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1, s_red[8 + wave] = nl;
     if (!ok) *s_ok = 0;
-    atomicAdd(&s_red[8 + threadIdx.x % kSubQ], np);
+    atomicAdd(&s_red[12 + threadIdx.x % kSubQ], np);
     __syncthreads();
     if (tr && threadIdx.x == 0) tr[3] = wall_clock64();
     const bool good = *s_ok != 0;
+    {  // free slots: thread i owns workgroups [i*c, i*c + c); wave scan, then waves
+      const uint32_t nwg = gridDim.x, c = (nwg + blockDim.x - 1) / blockDim.x;
+      const uint32_t wb = threadIdx.x * c, we = min(nwg, wb + c);
+      uint32_t part = 0;
+      for (uint32_t w = wb; w < we; ++w) part += slots - min(slots, (uint32_t)s_kept[w]);
+      uint32_t x = part;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+      }
+      __shared__ uint32_t s_wsum[kWavesPerWg];
+      if (lane == 63) s_wsum[wave] = x;
+      __syncthreads();
+      uint32_t run = x - part, total = 0;
+#pragma unroll
+      for (int k = 0; k < kWavesPerWg; ++k) {
+        if (k < wave) run += s_wsum[k];
+        total += s_wsum[k];
+      }
+      for (uint32_t w = wb; w < we; ++w) {
+        st_sc(&es->fslot[w], tag(epoch, run));
+        run += slots - min(slots, (uint32_t)s_kept[w]);
+      }
+      if (threadIdx.x == 0) s_fs[0] = 0, s_fs[1] = total;  // workgroup 0's range starts at 0
+    }
     if (threadIdx.x == 0) {
       uint32_t flags = 0;
+      uint32_t acc = 0;
+      for (int k = 0; k < kSubQ; ++k) {
+        s_pre[k] = acc;
+        acc += (uint32_t)s_red[12 + k];
+      }
+      s_pre[kSubQ] = acc;
       if (good) {
         const unsigned long long hits[2] = {s_red[0] + s_red[1] + s_red[2] + s_red[3],
                                             s_red[4] + s_red[5] + s_red[6] + s_red[7]};
+        const uint32_t kept = (uint32_t)(s_red[8] + s_red[9] + s_red[10] + s_red[11]);
         const int tp = (int)(tdone - d.t_start);
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
           if (ps.cut[m] == 0 || ps.done_from[m] <= tp || hits[m] == 0) continue;
-          if (tp + 1 < d.hist_len) d.hist[m][tp + 1] = (uint32_t)hits[m];
+          uint32_t *h = m ? d.hist1 : d.hist0;
+          if (tp + 1 < d.hist_len) h[tp + 1] = (uint32_t)hits[m];
           ps.cov[m] += hits[m];
           ps.sum[m] += hits[m] * (unsigned long long)(tp + 1);
           ps.hd[m] = (uint32_t)(tp + 1);
@@ -936,20 +1020,17 @@ __device__ __forceinline__ bool grid_barrier_single(
 #pragma unroll
         for (int m = 0; m < 2; ++m)
           if (ps.cut[m] == 0 || ps.done_from[m] <= tp + 1) flags |= 1u << m;
-        es->qtotal += L;
-        es->qmax = max(es->qmax, L);
-        if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
+        es->qtotal += ps.cur_total;
+        es->qmax = max(es->qmax, ps.cur_total);
+        if (qstats && (int)tdone < qstats_len) qstats[tdone] = ps.cur_total;
+        ps.next_total = acc + kept;
+        ps.cur_total = ps.next_total;
+        if (ps.next_total) flags |= 8u;
       } else {
         flags = 4u;
         atomicOr(&es->error, 1u);
       }
       s_flags[0] = flags;
-      uint32_t acc = 0;
-      for (int k = 0; k < kSubQ; ++k) {
-        s_pre[k] = acc;
-        acc += (uint32_t)s_red[8 + k];
-      }
-      s_pre[kSubQ] = acc;
     }
     // the counters that iteration epoch+1 fills ((t+2) % 4 with t = epoch)
     if (threadIdx.x < kSubQ) st_sc(&es->qlen[(epoch + 2) % 4][threadIdx.x], 0u);
@@ -957,16 +1038,19 @@ __device__ __forceinline__ bool grid_barrier_single(
     if (tr && threadIdx.x == 0) tr[4] = wall_clock64();
     for (int j = threadIdx.x; j < kReleaseCopies * kWords; j += blockDim.x) {
       const int w = j % kWords;
-      st_sc(&es->rel2[j / kWords][w], tag(epoch, w == 0 ? s_flags[0] : s_pre[w - 1]));
+      const uint32_t val = w == 0 ? s_flags[0] : w <= kSubQ + 1 ? s_pre[w - 1] : s_fs[1];
+      st_sc(&es->rel2[j / kWords][w], tag(epoch, val));
     }
   } else {
-    __shared__ unsigned long long s_rel[kWords];
-    if (threadIdx.x < kWords) {
-      const unsigned long long *line = es->rel2[blockIdx.x % kReleaseCopies];
+    __shared__ unsigned long long s_rel[kWords + 1];
+    if (threadIdx.x <= kWords) {  // the replica's words, then this workgroup's first slot
+      const unsigned long long *src = threadIdx.x < kWords
+                                          ? &es->rel2[blockIdx.x % kReleaseCopies][threadIdx.x]
+                                          : &es->fslot[blockIdx.x];
       const uint64_t t0 = wall_clock64();
       unsigned long long w;
       for (;;) {
-        w = ld_sc(&line[threadIdx.x]);
+        w = ld_sc(src);
         if (tagged(w, epoch)) break;
         __builtin_amdgcn_s_sleep(1);
         if (wall_clock64() - t0 > limit) {
@@ -982,6 +1066,8 @@ __device__ __forceinline__ bool grid_barrier_single(
     if (threadIdx.x == 0) {
       s_flags[0] = (uint32_t)s_rel[0];
       for (int k = 0; k <= kSubQ; ++k) s_pre[k] = (uint32_t)s_rel[1 + k];
+      s_fs[1] = (uint32_t)s_rel[kSubQ + 2];
+      s_fs[0] = (uint32_t)s_rel[kWords];
       if (s_flags[0] & 4u) *s_ok = 0;
     }
   }
@@ -1085,11 +1171,22 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   __shared__ HitTable ht;
   __shared__ uint32_t s_pre[kSubQ + 1];
   __shared__ int s_ok;
-  // single pair: this iteration's hits per pass and entries pushed, the
-  // released flags, and the master's reduction space
-  __shared__ uint32_t s_cnt[3];
+  // single pair: this iteration's hits per pass, entries pushed to the global
+  // queue and entries kept; the released flags; the master's reduction space;
+  // the workgroup's own queue of the running and of the next iteration
+  __shared__ uint32_t s_cnt[4];
   __shared__ uint32_t s_flags[1];
-  __shared__ unsigned long long s_red[8 + kSubQ];
+  __shared__ unsigned long long s_red[12 + kSubQ];
+  // kept entries: at most one per wave slot (kSlots), taken by slots 0..;
+  // the other slots take global entries from the workgroup's range of the
+  // master's free-slot prefix (s_fs), so while the queue fits, no wave runs
+  // two steps in a row
+  constexpr uint32_t kSlots = kWavesPerWg * TPW;
+  constexpr uint32_t kStick = kSlots;
+  __shared__ uint64_t s_st[2][kStick];
+  __shared__ uint32_t s_stn[2];
+  __shared__ uint8_t s_kept[kMaxGrid];
+  __shared__ uint32_t s_fs[2];
   const bool fence = (flags & 1) != 0;
   constexpr bool single = SINGLE;
   const int lane = threadIdx.x & 63;
@@ -1104,7 +1201,12 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   const uint32_t kq = blockIdx.x % kSubQ;
   if (threadIdx.x == 0) lq.n = 0;
   if (threadIdx.x < kHitTable) ht.key[threadIdx.x] = ~0u, ht.cnt[threadIdx.x] = 0;
-  if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 2) s_stn[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {  // no kept entries yet: every slot is free
+    s_fs[0] = blockIdx.x * kSlots;
+    s_fs[1] = gridDim.x * kSlots;
+  }
   PairView v0{};  // one pair: its fields for the whole launch
   if constexpr (SINGLE) v0 = make_view(descs[0], ctrs[0]);
   PassState ps{};
@@ -1132,8 +1234,11 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       }
       __syncthreads();
     }
-    const uint32_t L = s_pre[kSubQ];
-    if (L == 0) break;
+    const uint32_t L = s_pre[kSubQ];  // entries in the global queue
+    // one pair: entries may also sit in the workgroups' own queues; the
+    // release says whether any remain anywhere (uniform decision)
+    if (SINGLE ? (it == 0 ? L == 0 : (s_flags[0] & 8u) == 0) : L == 0) break;
+    if (SINGLE && it == 0 && blockIdx.x == 0 && threadIdx.x == 0) ps.cur_total = L;
     bool pdone[2] = {false, false};
     if (single) {  // stop as soon as both passes are done (uniform decision)
       if (it == 0) {
@@ -1151,20 +1256,13 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     uint64_t *tr = (trace && (int)t < qstats_len) ? trace + 8 * (uint64_t)t : nullptr;
     if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
       tr[0] = wall_clock64();
-      tr[5] = L;
+      tr[5] = SINGLE ? ps.cur_total : L;
     }
     uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
     uint32_t *qout_len = &es->qlen[(t + 1) % 4][kq];
-    for (uint32_t base = gwave * step; base < L; base += nwave * step) {
-      for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
-        const uint32_t ei = base + k + sub;
-        bool live = ei < L;
-        uint64_t ent = 0;
-        if (live) {
-          int sq = 0;
-          while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
-          ent = ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
-        }
+    // one step of queue entry ent (live: a real entry) by the R lanes of its
+    // group; pushes the entries it produces
+    auto step_tile = [&](uint64_t ent, bool live) {
         const uint32_t tile = (uint32_t)ent;
         const int p = SINGLE ? 0 : (int)((ent >> 32) & 0xFFFF);
         const int m = (int)((ent >> 48) & 1);
@@ -1336,7 +1434,39 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
             }
           }
         }
-        push_entries(want, pent, lq, Qout, qout_len, lane, SINGLE ? &s_cnt[2] : nullptr);
+        if constexpr (SINGLE)
+          push_sticky(want, pent, s_st[(t + 1) & 1], &s_stn[(t + 1) & 1], kStick, lq, Qout,
+                      qout_len, lane, &s_cnt[2]);
+        else
+          push_entries(want, pent, lq, Qout, qout_len, lane);
+    };
+    // global entry ei of the queue of iteration t
+    auto global_entry = [&](uint32_t ei) {
+      int sq = 0;
+      while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
+      return ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
+    };
+    uint32_t gfirst = 0;  // global entries below this were taken in round 1
+    if constexpr (SINGLE) {
+      // round 1: slot s takes kept entry s, or else global entry
+      // s_fs[0] + (s - kept) of this workgroup's free range
+      const uint32_t nloc = s_stn[t & 1];
+      const uint32_t s = (uint32_t)wave * TPW + sub;
+      const uint32_t gi = s_fs[0] + (s - nloc);
+      const bool glive = s >= nloc && gi < L && gi < s_fs[1];
+      uint64_t ent = 0;
+      if (s < nloc) ent = s_st[t & 1][s];
+      if (glive) ent = global_entry(gi);
+      if (__ballot(s < nloc || glive)) step_tile(ent, s < nloc || glive);
+      gfirst = min(L, s_fs[1]);
+    }
+    // global entries beyond the free slots (one pair) or all of them
+    // (several pairs), round robin over the waves
+    for (uint32_t base = gfirst + gwave * step; base < L; base += nwave * step) {
+      for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
+        const uint32_t ei = base + k + sub;
+        const bool live = ei < L;
+        step_tile(live ? global_entry(ei) : 0ull, live);
       }
     }
     if (tr) {
@@ -1347,10 +1477,11 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         if (blockIdx.x == 0) tr[2] = now;
       }
     }
-    const uint32_t nflushed = flush_local_queue(lq, Qout, qout_len);
-    if (SINGLE) {
-      if (threadIdx.x == 0) s_cnt[2] += nflushed;
+    if constexpr (SINGLE) {  // staged overflow to the global queue; counts for the barrier
+      const uint32_t nflushed = flush_local_queue(lq, Qout, qout_len);
+      if (threadIdx.x == 0) s_cnt[2] += nflushed, s_cnt[3] = s_stn[(t + 1) & 1];
     } else {
+      flush_local_queue(lq, Qout, qout_len);
       hit_flush(ht, ctrs);
     }
     const uint32_t tdone = t;  // iteration that just ran
@@ -1397,8 +1528,9 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       }
     };
     if (SINGLE) {
-      if (!grid_barrier_single(es, t, spin_limit, &s_ok, s_cnt, ps, descs[0], tdone, L, qstats,
-                               qstats_len, tr, s_pre, s_flags, s_red))
+      if (!grid_barrier_single(es, t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps, v0, tdone,
+                               qstats, qstats_len, tr, s_pre, s_flags, s_red, kSlots, s_kept,
+                               s_fs))
         break;
     } else if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 4], s_pre)) {
       break;
@@ -1420,8 +1552,11 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     for (int p = threadIdx.x; p < npairs; p += blockDim.x)
       a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
     if (threadIdx.x == 0) {
+      // entries left for a next iteration (one pair: the master's count,
+      // which includes the workgroups' own queues)
+      const uint32_t qn = SINGLE ? ps.next_total : s_pre[kSubQ];
       st_sc(&es->t, t);
-      st_sc(&es->qnext, s_pre[kSubQ]);
+      st_sc(&es->qnext, qn);
       if (a.head) {
         EngineHead h;
         h.t = t;
@@ -1430,7 +1565,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         h.nwg = gridDim.x;
         h.qtotal = es->qtotal;
         h.qcap = es->qcap;
-        h.qnext = s_pre[kSubQ];
+        h.qnext = qn;
 #pragma unroll
         for (int i = 0; i < 8; ++i) h.pad1[i] = 0;
         *a.head = h;
