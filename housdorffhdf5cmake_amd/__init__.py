@@ -227,11 +227,15 @@ def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, s
 
 
 def last_timings() -> dict:
-    """Device ms of the phases of the last call made with timing=True."""
-    ms = (C.c_float * 5)()
-    n = lib().hdm_last_timings(ms, 5)
+    """Device ms of the phases of the last call made with timing=True; "fused"
+    is True when the call ran as one launch (phases then come from the
+    kernel's own clock, and timing=3 fills only "total")."""
+    ms = (C.c_float * 6)()
+    n = lib().hdm_last_timings(ms, 6)
     names = ("pack", "meta", "dilate", "finalize", "total")
-    return {names[i]: float(ms[i]) for i in range(n)}
+    out = {names[i]: float(ms[i]) for i in range(min(n, 5))}
+    out["fused"] = n > 5 and ms[5] == 1.0
+    return out
 
 
 def last_trace(max_iters: int = 4096):

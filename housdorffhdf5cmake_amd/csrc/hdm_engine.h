@@ -114,7 +114,9 @@ struct EngineHead {
   unsigned long long qtotal;
   uint32_t qcap;
   uint32_t qnext;  // entries queued for the next iteration when the kernel left
-  uint32_t pad1[8];
+  // fused single-pair launch: wall-clock ticks at its start and at the end of
+  // its pack, seed and dilation phases (0 otherwise)
+  unsigned long long phase[4];
 };
 
 struct Engine {

@@ -27,11 +27,16 @@ struct Ctx {
   int cus = 0;
   int occ[HDM_TILE_COUNT] = {};       // resident dilate workgroups per CU, per tile kind
   int occ_pack[HDM_TILE_COUNT] = {};  // resident pack workgroups per CU
+  int occ_fused[HDM_TILE_COUNT] = {};  // resident fused single-pair workgroups per CU
   double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
   hipEvent_t ev[5] = {};     // phase boundaries of a timed call
-  float last_ms[5] = {};
+  float last_ms[6] = {};     // pack, seed, dilation, 0, all; 1 when the call was one fused launch
+  // epoch base of the next fused launch's barrier tags; the tagged words of
+  // the workspace's engine are cleared whenever the base restarts at 0
+  uint32_t ebase = 0;
+  void *ebase_ws = nullptr;  // the workspace the base belongs to
   std::vector<uint64_t> last_trace;  // 8 words per iteration of a traced call
   DevMem trace;
   std::mutex mu;
@@ -65,6 +70,8 @@ int resident_grid(const Ctx *c, int tile, int requested);
 uint64_t spin_ticks(const Ctx *c);
 int norm_tile(int tile);
 int barrier_mode();
+bool fused_enabled();
+int fused_grid(const Ctx *c, int tile, int requested);
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,

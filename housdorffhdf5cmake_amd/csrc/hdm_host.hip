@@ -101,6 +101,12 @@ int get_ctx(Ctx **out) {
          nb < 1))
       rc = HDM_E_HIP;
     c->occ_pack[k] = nb;
+    if (rc == HDM_OK &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->fused_fn,
+                                                      kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ_fused[k] = nb;
   }
   // the dilation kernel stores the result and the engine head here directly:
   // coherent (fine-grained) host memory, so the stores are seen by the host
@@ -132,6 +138,22 @@ uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.
 // Grid barrier: 0 (default) flag array with agent-coherent data accesses and
 // no cache maintenance; 1 the same plus L2 write-back / invalidate at every
 // barrier (HDM_BARRIER_FENCE=1, a conservative setting for comparison).
+// A single pair runs as one fused launch unless HDM_FUSED=0 asks for the
+// four-launch path (reset, pack, seed, dilation), kept for batches and for
+// comparison.
+bool fused_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("HDM_FUSED");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
+int fused_grid(const Ctx *c, int tile, int requested) {
+  const int maxres = std::min(std::max(1, c->occ_fused[tile]) * c->cus, kMaxGrid);
+  return requested > 0 ? std::min(requested, maxres) : maxres;
+}
+
 int barrier_mode() {
   static int mode = [] {
     const char *e = std::getenv("HDM_BARRIER_FENCE");
@@ -353,8 +375,47 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   hout->status = 1;
   cfg.outs = hout;
   cfg.head = hhead;
-  hipError_t e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, qcap, cfg,
-                               g.hist_len);
+  hipError_t e = hipSuccess;
+  const bool fused = fused_enabled();
+  const uint32_t ebase = c->ebase;
+  if (fused) {
+    // one launch: reset, pack, seed and dilation.  Its barrier tags use
+    // epochs above every earlier launch's on this workspace; a new workspace
+    // or a restarting base gets its engine cleared first.
+    if (c->ebase_ws != c->ws.p || ebase > (1u << 23)) {
+      if (hipMemsetAsync(P.engine, 0, sizeof(Engine), s) != hipSuccess) return HDM_E_HIP;
+      c->ebase = 0;
+      c->ebase_ws = c->ws.p;
+    }
+    if (d.map && hipMemsetAsync(d.map, 0, nvox * 2, s) != hipSuccess) return HDM_E_HIP;
+    DilateArgs a{};
+    a.descs = P.descs;
+    a.ctrs = P.ctrs;
+    a.npairs = 1;
+    a.max_iters = INT_MAX;
+    a.es = P.engine;
+    a.q = P.q;
+    a.qcap = qcap;
+    a.chunk = cfg.chunk;
+    a.spin = cfg.spin;
+    a.qstats = P.qstats;
+    a.qstats_len = g.hist_len;
+    a.flags = barrier_mode();
+    a.trace = cfg.trace;
+    a.outs = hout;
+    a.head = hhead;
+    a.desc = d;
+    a.total_tiles = g.ntiles;
+    a.ebase = c->ebase;
+    if (cfg.ev) e = hipEventRecord(c->ev[0], s);
+    if (e == hipSuccess) e = tl->fused(dim3(fused_grid(c, tile, opts.grid_ctas)), s, a);
+    if (e == hipSuccess && cfg.ev) e = hipEventRecord(c->ev[3], s);
+    // until the result is read, assume the launch used every epoch it could
+    c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;
+  } else {
+    c->ebase_ws = nullptr;  // the reset kernel clears the tagged words
+    e = enqueue_group(s, tl, P, 1, &d, &pi, g.row_words, g.ntiles, qcap, cfg, g.hist_len);
+  }
   if (e != hipSuccess) return map_hip(e);
   if (map_out && opts.ptr_kind == HDM_PTR_HOST &&
       hipMemcpyAsync(map_out, d.map, nvox * 2, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -372,11 +433,31 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   std::memcpy(&o, hout, sizeof(o));
   std::memcpy(&es, hhead, sizeof(es));
   if (o.status == 1) return HDM_E_HIP;
-  if (opts.timing == 3) {  // the dilation only
+  if (fused) {
+    c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used
+    c->last_ms[5] = 1.f;
+    if (opts.timing) {
+      float all = -1.f;
+      if (hipEventElapsedTime(&all, c->ev[0], c->ev[3]) != hipSuccess) all = -1.f;
+      (void)hipGetLastError();
+      if (opts.timing == 3) {  // the launch as a whole
+        c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = c->last_ms[3] = -1.f;
+        c->last_ms[4] = all;
+      } else {  // phases from the launch's own clock stamps
+        const double tick_ms = 1.0 / c->clock_khz;
+        c->last_ms[0] = (float)((es.phase[1] - es.phase[0]) * tick_ms);
+        c->last_ms[1] = (float)((es.phase[2] - es.phase[1]) * tick_ms);
+        c->last_ms[2] = (float)((es.phase[3] - es.phase[2]) * tick_ms);
+        c->last_ms[3] = 0.f;
+        c->last_ms[4] = all;
+      }
+    }
+  } else if (opts.timing == 3) {  // the dilation only
     float ms = -1.f;
     if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) != hipSuccess) ms = -1.f;
     c->last_ms[0] = c->last_ms[1] = c->last_ms[3] = c->last_ms[4] = -1.f;
     c->last_ms[2] = ms;
+    c->last_ms[5] = 0.f;
     (void)hipGetLastError();
   } else if (opts.timing) {
     // phases: pack (+ reset), metadata, dilation (+ finalize); total
@@ -388,6 +469,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     for (int i = 0; i < 3; ++i) c->last_ms[i] = ms[i];
     c->last_ms[3] = 0.f;
     c->last_ms[4] = all;
+    c->last_ms[5] = 0.f;
     (void)hipGetLastError();
   }
   if (opts.timing == 2) {
@@ -693,7 +775,7 @@ extern "C" int hdm_last_timings(float *ms, int n) {
   Ctx *c = nullptr;
   if (get_ctx(&c) != HDM_OK) return 0;
   std::lock_guard<std::mutex> lk(c->mu);
-  const int k = std::min(n, 5);
+  const int k = std::min(n, 6);
   for (int i = 0; i < k; ++i) ms[i] = c->last_ms[i];
   return k;
 }

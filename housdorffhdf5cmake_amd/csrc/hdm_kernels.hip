@@ -301,11 +301,25 @@ __device__ __forceinline__ void acc_flush(PairCtr &c, PackAcc &a, int lane, int 
 // non-empty, full, and which of its six faces hold voxels.  Voxel counts and
 // the box of non-empty tiles (getMinMaxes, hello.cu:396-526) accumulate per
 // lane and are published once per (wave, pair) into spread slots.
-template <int TXB, int TY, int TZ>
-__global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
-                                                          const PackIndex *__restrict__ idx,
-                                                          PairCtr *ctrs, int npairs,
-                                                          uint64_t total_tiles) {
+// store that other workgroups of the same launch read later (COH: agent-
+// coherent, written through) or that only later launches read (plain)
+template <bool COH, typename T>
+__device__ __forceinline__ void put(T *p, T v) {
+  if constexpr (COH)
+    st_sc(p, v);
+  else
+    *p = v;
+}
+
+// The pack pass over all tiles of the launch's pairs, spread over the grid's
+// waves.  COH: the same launch reads the results in later phases (the fused
+// single-pair launch), so every store is written through; the first-hit map
+// is then cleared by the host instead.
+template <int TXB, int TY, int TZ, bool COH>
+__device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
+                                           const PackIndex *__restrict__ idx, PairCtr *ctrs,
+                                           int npairs, uint64_t total_tiles,
+                                           uint16_t (*s_pieces)[1024]) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -319,8 +333,6 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
   const uint64_t per = ((total_tiles + nw - 1) / nw + kStep - 1) / kStep * kStep;
   const uint64_t w0 = gw * per, w1 = min(total_tiles, w0 + per);
   const int slot = (int)(gw % kSumSlots);
-  // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
-  __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
   PackAcc acc;
   acc_clear(acc);
   int ap = -1;  // pair the accumulator belongs to (wave-uniform)
@@ -345,7 +357,7 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
       vbU[u] = 0;
       if (inU[u]) {
         const PairDesc &d = descs[pU[u]];
-        const uint32_t tile = (uint32_t)(g - idx[pU[u]].tile_base);
+        const uint32_t tile = (uint32_t)(g - (npairs > 1 ? idx[pU[u]].tile_base : 0));
         tileU[u] = tile;
         txU[u] = (int)(tile % (uint32_t)d.gx);
         tyU[u] = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
@@ -423,8 +435,8 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
       const Row a = aU[u], b = bU[u];
       const uint64_t o = (uint64_t)tile * R + r;
       if (rowok) {
-        ((Row *)d.orig[0])[o] = a;
-        ((Row *)d.orig[1])[o] = b;
+        put<COH>((Row *)d.orig[0] + o, a);
+        put<COH>((Row *)d.orig[1] + o, b);
       }
       const Row vb = valid_bits<Row>(txU[u], d.nx);
       uint32_t flags = 0;
@@ -440,16 +452,16 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
         const bool fzl = group_bits<R>(__ballot(zl == 0 && w != 0), lane) != 0;
         const bool fzh = group_bits<R>(__ballot(zl == TZ - 1 && w != 0), lane) != 0;
         if (in && r == 0) {
-          d.xf_orig[m][2 * (uint64_t)tile] = fxl;
-          d.xf_orig[m][2 * (uint64_t)tile + 1] = fxh;
+          put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile, fxl);
+          put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile + 1, fxh);
         }
         flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)(fxl != 0) << 2 |
                   (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
                   (uint32_t)fzl << 6 | (uint32_t)fzh << 7)
                  << (8 * m);
       }
-      if (in && r == 0) d.tflags[tile] = (uint16_t)flags;
-      if (d.map && rowok) {
+      if (in && r == 0) put<COH>(d.tflags + tile, (uint16_t)flags);
+      if (!COH && d.map && rowok) {
         uint16_t *mp = d.map + vbU[u];
         for (int k = 0; k < cntU[u]; ++k) mp[k] = 0;
       }
@@ -479,6 +491,16 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
     }
   }
   if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+}
+
+template <int TXB, int TY, int TZ>
+__global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
+                                                          const PackIndex *__restrict__ idx,
+                                                          PairCtr *ctrs, int npairs,
+                                                          uint64_t total_tiles) {
+  // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
+  __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
+  pack_tiles<TXB, TY, TZ, false>(descs, idx, ctrs, npairs, total_tiles, s_pieces);
 }
 
 // |gold|, |segm|, |FP|, |FN| per pair from the spread partials (one wave per
@@ -540,7 +562,9 @@ struct PairView {
   Box b;
   unsigned long long cut[2];  // robust cut per pass; 0: nothing to do
 };
-__device__ __forceinline__ PairView make_view(const PairDesc &d, const PairCtr &c) {
+// from the descriptor and explicit box and FP / FN counts
+__device__ __forceinline__ PairView make_view(const PairDesc &d, const int *lo, const int *hi,
+                                              unsigned long long n_fp, unsigned long long n_fn) {
   PairView v;
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
@@ -554,10 +578,13 @@ __device__ __forceinline__ PairView make_view(const PairDesc &d, const PairCtr &
   v.map = d.map;
   v.nx = d.nx, v.ny = d.ny, v.nz = d.nz, v.gx = d.gx, v.gy = d.gy, v.hist_len = d.hist_len;
   v.t_start = d.t_start, v.gen = d.gen;
-  v.b = pair_box(d, c);
-  v.cut[0] = robust_cut(d.robust, c.n_fp);
-  v.cut[1] = robust_cut(d.robust, c.n_fn);
+  v.b = pair_box(d, lo, hi);
+  v.cut[0] = robust_cut(d.robust, n_fp);
+  v.cut[1] = robust_cut(d.robust, n_fn);
   return v;
+}
+__device__ __forceinline__ PairView make_view(const PairDesc &d, const PairCtr &c) {
+  return make_view(d, c.lo, c.hi, c.n_fp, c.n_fn);
 }
 
 // neighbour tile of `tile` in direction dir (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z),
@@ -893,14 +920,15 @@ struct PassState {
 // entries), stored tagged in es->fslot[wg] and read by each workgroup in the
 // same poll as its release replica; s_fs[0] receives it, s_fs[1] the total of
 // free slots, so global entries go first to waves that hold no kept entry.
-// Waits are bounded as in grid_barrier.  s_cnt[0..3] holds the four counts
+// tdone is the iteration that ran (~0u before the first), qz the set of queue
+// counters to zero.  Waits are bounded as in grid_barrier.  s_cnt[0..3] holds the four counts
 // and is zeroed here, as is *s_zero (the kept count of the queue that becomes
 // the next iteration's push target).  Returns false on a timeout.
 __device__ __forceinline__ bool grid_barrier_single(
     Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, uint32_t *s_zero,
     PassState &ps, const PairView &d, uint32_t tdone, uint32_t *qstats, int qstats_len,
     uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red, uint32_t slots,
-    uint8_t *s_kept, uint32_t *s_fs) {
+    uint8_t *s_kept, uint32_t *s_fs, int qz) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1022,7 +1050,7 @@ __device__ __forceinline__ bool grid_barrier_single(
           if (ps.cut[m] == 0 || ps.done_from[m] <= tp + 1) flags |= 1u << m;
         es->qtotal += ps.cur_total;
         es->qmax = max(es->qmax, ps.cur_total);
-        if (qstats && (int)tdone < qstats_len) qstats[tdone] = ps.cur_total;
+        if (qstats && tdone < (uint32_t)qstats_len) qstats[tdone] = ps.cur_total;
         ps.next_total = acc + kept;
         ps.cur_total = ps.next_total;
         if (ps.next_total) flags |= 8u;
@@ -1032,8 +1060,8 @@ __device__ __forceinline__ bool grid_barrier_single(
       }
       s_flags[0] = flags;
     }
-    // the counters that iteration epoch+1 fills ((t+2) % 4 with t = epoch)
-    if (threadIdx.x < kSubQ) st_sc(&es->qlen[(epoch + 2) % 4][threadIdx.x], 0u);
+    // the counters set qz (the one the iteration after next fills)
+    if (threadIdx.x < kSubQ) st_sc(&es->qlen[qz][threadIdx.x], 0u);
     __syncthreads();
     if (tr && threadIdx.x == 0) tr[4] = wall_clock64();
     for (int j = threadIdx.x; j < kReleaseCopies * kWords; j += blockDim.x) {
@@ -1069,6 +1097,71 @@ __device__ __forceinline__ bool grid_barrier_single(
       s_fs[1] = (uint32_t)s_rel[kSubQ + 2];
       s_fs[0] = (uint32_t)s_rel[kWords];
       if (s_flags[0] & 4u) *s_ok = 0;
+    }
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// Grid barrier between the phases of the fused single-pair launch, carrying no
+// data: each workgroup drains its stores and stores tag(epoch, 0) into word 0
+// of its record; workgroup 0 waits for all of them and stores tag(epoch,
+// flags) into word 0 of every release replica (flags bit 2: a wait timed out).
+// Bounded like the other barriers; returns false on a timeout.
+__device__ __forceinline__ bool grid_sync_tagged(Engine *es, uint32_t epoch, uint64_t limit,
+                                                 int *s_ok) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *s_ok = 1;
+    st_sc(&es->arr[blockIdx.x][0], tag(epoch, 0));
+  }
+  if (blockIdx.x == 0) {
+    int ok = 1;
+    const uint64_t t0 = wall_clock64();
+    constexpr int kPoll = 4;
+    for (uint32_t i0 = threadIdx.x; i0 < gridDim.x && ok; i0 += kPoll * blockDim.x) {
+      uint32_t pending = 0;
+#pragma unroll
+      for (int k = 0; k < kPoll; ++k)
+        if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
+      while (pending) {
+        unsigned long long w[kPoll];
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k)
+          w[k] = ((pending >> k) & 1) ? ld_sc(&es->arr[i0 + k * blockDim.x][0]) : 0ull;
+#pragma unroll
+        for (int k = 0; k < kPoll; ++k)
+          if (((pending >> k) & 1) && tagged(w[k], epoch)) pending &= ~(1u << k);
+        if (!pending) break;
+        __builtin_amdgcn_s_sleep(1);
+        if (wall_clock64() - t0 > limit) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+    __syncthreads();
+    if (!ok) *s_ok = 0;
+    __syncthreads();
+    if (threadIdx.x == 0 && !*s_ok) atomicOr(&es->error, 1u);
+    if (threadIdx.x < kReleaseCopies)
+      st_sc(&es->rel2[threadIdx.x][0], tag(epoch, *s_ok ? 0u : 4u));
+  } else if (threadIdx.x == 0) {
+    const unsigned long long *w0 = &es->rel2[blockIdx.x % kReleaseCopies][0];
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+      const unsigned long long w = ld_sc(w0);
+      if (tagged(w, epoch)) {
+        if (w & 4u) *s_ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      if (wall_clock64() - t0 > limit) {
+        atomicOr(&es->error, 1u);
+        *s_ok = 0;
+        break;
+      }
     }
   }
   __syncthreads();
@@ -1151,8 +1244,13 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
 // ---------------------------------------------------------------- dilate
 // SINGLE: the launch holds one pair, so its descriptor and its per-iteration
 // pass state are loop invariants instead of per-entry loads.
-template <int TXB, int TY, int TZ, bool SINGLE>
+// FUSED (one pair): the launch first runs the reset, pack and seed passes
+// itself, separated by grid barriers, so one call is one launch; every store
+// that a later phase reads in another workgroup is written through, and those
+// reads are agent-coherent.
+template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false>
 __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
+  static_assert(!FUSED || SINGLE, "the fused launch holds one pair");
   const PairDesc *__restrict__ descs = a.descs;
   PairCtr *ctrs = a.ctrs;
   const int npairs = a.npairs;
@@ -1208,24 +1306,159 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     s_fs[1] = gridDim.x * kSlots;
   }
   PairView v0{};  // one pair: its fields for the whole launch
-  if constexpr (SINGLE) v0 = make_view(descs[0], ctrs[0]);
   PassState ps{};
-  if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's registers
-    const PairCtr &c0 = ctrs[0];
-    for (int m = 0; m < 2; ++m) {
-      ps.cut[m] = robust_cut(descs[0].robust, ld_sc(m == 0 ? &c0.n_fp : &c0.n_fn));
-      ps.cov[m] = ld_sc(&c0.cov[m]);
-      ps.sum[m] = ld_sc(&c0.sum_d[m]);
-      ps.hd[m] = ld_sc(&c0.hd_last[m]);
-      ps.done_from[m] = ld_sc(&c0.done_from[m]);
+  uint32_t t = 0;
+  bool run = true;
+  // epoch of a dilation barrier = eoff + the iteration that follows it; the
+  // fused launch's prologue uses ebase + 1 .. ebase + 3
+  const uint32_t eoff = FUSED ? a.ebase + 3 : 0u;
+  __shared__ unsigned long long s_phase[3];
+  if constexpr (FUSED) {
+    const PairDesc &d = a.desc;
+    PairCtr &c0 = ctrs[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[0] = wall_clock64();
+    // R: workgroup 0 zeroes the pair's counters and histograms and the
+    // engine's state (barrier words are tagged with ever larger epochs)
+    if (blockIdx.x == 0) {
+      const int tid = threadIdx.x;
+      if (tid == 0) {
+        st_sc(&c0.n_gold, 0ull), st_sc(&c0.n_segm, 0ull);
+        st_sc(&c0.n_fp, 0ull), st_sc(&c0.n_fn, 0ull);
+        for (int m = 0; m < 2; ++m) {
+          st_sc(&c0.cov[m], 0ull), st_sc(&c0.sum_d[m], 0ull);
+          st_sc(&c0.done_from[m], 0x7FFFFFFF), st_sc(&c0.hd_last[m], 0u);
+        }
+        st_sc(&c0.status, 0);
+        st_sc(&es->t, 0u), st_sc(&es->error, 0u), st_sc(&es->qmax, 0u);
+        st_sc(&es->qtotal, 0ull), st_sc(&es->qcap, qcap), st_sc(&es->qnext, 0u);
+      }
+      for (int i = tid; i < 2 * kHitSlots; i += blockDim.x) st_sc(&c0.hits[0][0] + i, 0ull);
+      for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&c0.sums[0][0] + i, 0ull);
+      for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
+        st_sc(&c0.box[0][0] + i, i < 3 * kSumSlots ? 0x7FFFFFFF : -1);
+      for (int i = tid; i < d.hist_len; i += blockDim.x)
+        st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
+      for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
     }
+    run = grid_sync_tagged(es, a.ebase + 1, spin_limit, &s_ok);
+    // P: pack, every store written through
+    if (run) {
+      __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
+      pack_tiles<TXB, TY, TZ, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces);
+      if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[1] = wall_clock64();
+      run = grid_sync_tagged(es, a.ebase + 2, spin_limit, &s_ok);
+    }
+    // S: totals and box from the pack partials, then tile states and the
+    // first queue, whose entries go to this workgroup's own queue first
+    if (run) {
+      __shared__ unsigned long long s_tot[4];
+      __shared__ int s_box[6];
+      if (threadIdx.x < 64) {
+        unsigned long long v[4];
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          v[f] = ld_sc(&c0.sums[f][lane]);
+#pragma unroll
+          for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+        }
+        int bx[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const int x = ld_sc(&c0.box[k][lane]);
+          bx[k] = k < 3 ? wave_min(x, lane) : wave_max(x, lane);
+        }
+        if (lane == 0) {
+#pragma unroll
+          for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) s_box[k] = bx[k];
+          if (blockIdx.x == 0) {
+            st_sc(&c0.n_gold, v[0]), st_sc(&c0.n_segm, v[1]);
+            st_sc(&c0.n_fp, v[2]), st_sc(&c0.n_fn, v[3]);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) st_sc(&c0.lo[k], bx[k]), st_sc(&c0.hi[k], bx[3 + k]);
+          }
+        }
+      }
+      __syncthreads();
+      v0 = make_view(d, s_box, s_box + 3, s_tot[2], s_tot[3]);
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        for (int m = 0; m < 2; ++m) {
+          ps.cut[m] = v0.cut[m];
+          ps.cov[m] = ps.sum[m] = 0;
+          ps.hd[m] = 0;
+          ps.done_from[m] = 0x7FFFFFFF;
+        }
+      }
+      // the rule of seed_kernel, with agent-coherent flag reads
+      uint64_t *Q0 = sub_queue(q, qcap, 0, kq);
+      uint32_t *Q0len = &es->qlen[0][kq];
+      constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
+      for (uint64_t tb = (uint64_t)blockIdx.x * kWgThreads; tb < a.total_tiles;
+           tb += (uint64_t)gridDim.x * kWgThreads) {
+        const uint64_t g = tb + threadIdx.x;
+        const bool in = g < a.total_tiles;
+        const uint32_t tile = in ? (uint32_t)g : 0u;
+        const int tx = (int)(tile % (uint32_t)v0.gx);
+        const int ty = (int)((tile / (uint32_t)v0.gx) % (uint32_t)v0.gy);
+        const int tz = (int)(tile / ((uint32_t)v0.gx * v0.gy));
+        const Box &b = v0.b;
+        const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
+                            ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
+        uint32_t own = 0, nbf[6] = {0, 0, 0, 0, 0, 0};
+        if (in_box) {
+          own = ld_sc(d.tflags + tile);
+#pragma unroll
+          for (int dir = 0; dir < 6; ++dir) {
+            const uint32_t nb = neighbour(v0, b, tile, tx, ty, tz, dir);
+            if (nb != kNoTile) nbf[dir] = ld_sc(d.tflags + nb);
+          }
+        }
+        bool want[2];
+        uint64_t ent[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+          const uint32_t f = (own >> (8 * m)) & 0xFF;
+          const bool any = f & 1, full = (f >> 1) & 1;
+          bool act = false;
+#pragma unroll
+          for (int dir = 0; dir < 6; ++dir) act |= ((nbf[dir] >> (8 * m + kFacing[dir])) & 1) != 0;
+          const bool needed = v0.cut[m] != 0;
+          const uint32_t st = !in_box ? 0u : (any && full) ? kFull : (any || (act && needed)) ? 1u : 0u;
+          if (in) st_sc((m ? v0.state[1] : v0.state[0]) + tile, st);
+          want[m] = in_box && needed && (any ? !full : act);
+          ent[m] = make_entry(tile, 0u, (uint32_t)m, 1, 1, v0.gen);
+        }
+        push_sticky(want[0], ent[0], s_st[0], &s_stn[0], kStick, lq, Q0, Q0len, lane, &s_cnt[2]);
+        push_sticky(want[1], ent[1], s_st[0], &s_stn[0], kStick, lq, Q0, Q0len, lane, &s_cnt[2]);
+      }
+      const uint32_t nflushed = flush_local_queue(lq, Q0, Q0len);
+      if (threadIdx.x == 0) s_cnt[2] += nflushed, s_cnt[3] = s_stn[0];
+      if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[2] = wall_clock64();
+      // hands iteration 0 its flags, global prefix and free slots
+      run = grid_barrier_single(es, a.ebase + 3, spin_limit, &s_ok, s_cnt, &s_stn[1], ps, v0,
+                                ~0u, qstats, qstats_len, nullptr, s_pre, s_flags, s_red, kSlots,
+                                s_kept, s_fs, 2);
+    }
+  } else {
+    if constexpr (SINGLE) v0 = make_view(descs[0], ctrs[0]);
+    if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's registers
+      const PairCtr &c0 = ctrs[0];
+      for (int m = 0; m < 2; ++m) {
+        ps.cut[m] = robust_cut(descs[0].robust, ld_sc(m == 0 ? &c0.n_fp : &c0.n_fn));
+        ps.cov[m] = ld_sc(&c0.cov[m]);
+        ps.sum[m] = ld_sc(&c0.sum_d[m]);
+        ps.hd[m] = ld_sc(&c0.hd_last[m]);
+        ps.done_from[m] = ld_sc(&c0.done_from[m]);
+      }
+    }
+    __syncthreads();
+    t = ld_sc(&es->t);
   }
-  __syncthreads();
-  uint32_t t = ld_sc(&es->t);
-  for (int it = 0; it < max_iters; ++it) {
+  for (int it = 0; run && it < max_iters; ++it) {
     // queue of iteration t: sub-queue lengths -> prefix offsets (after the
     // first iteration the barrier delivers them with the release)
-    if (it == 0) {
+    if (it == 0 && !FUSED) {
       if (threadIdx.x < kSubQ) s_pre[threadIdx.x + 1] = ld_sc(&es->qlen[t % 4][threadIdx.x]);
       __syncthreads();
       if (threadIdx.x == 0) {
@@ -1237,11 +1470,13 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
     const uint32_t L = s_pre[kSubQ];  // entries in the global queue
     // one pair: entries may also sit in the workgroups' own queues; the
     // release says whether any remain anywhere (uniform decision)
-    if (SINGLE ? (it == 0 ? L == 0 : (s_flags[0] & 8u) == 0) : L == 0) break;
-    if (SINGLE && it == 0 && blockIdx.x == 0 && threadIdx.x == 0) ps.cur_total = L;
+    // (the fused launch's prologue barrier released the first iteration too)
+    const bool first_read = it == 0 && !FUSED;
+    if (SINGLE ? (first_read ? L == 0 : (s_flags[0] & 8u) == 0) : L == 0) break;
+    if (SINGLE && first_read && blockIdx.x == 0 && threadIdx.x == 0) ps.cur_total = L;
     bool pdone[2] = {false, false};
     if (single) {  // stop as soon as both passes are done (uniform decision)
-      if (it == 0) {
+      if (first_read) {
         const int tp0 = (int)(t - descs[0].t_start);
         pdone[0] = pass_done(descs[0], ctrs[0], 0, tp0);
         pdone[1] = pass_done(descs[0], ctrs[0], 1, tp0);
@@ -1284,7 +1519,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         const unsigned long long cut_m = m ? v.cut[1] : v.cut[0];
         if (live && (SINGLE ? pdone[m] : (cut_m == 0 || dfrom <= (int)tp))) live = false;
         if (live && tp + 1 >= (uint32_t)v.hist_len) {
-          ctrs[p].status = -8;
+          st_sc(&ctrs[p].status, -8);
           live = false;
         }
         const int tx = (int)(tile % (uint32_t)v.gx);
@@ -1528,9 +1763,9 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       }
     };
     if (SINGLE) {
-      if (!grid_barrier_single(es, t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps, v0, tdone,
-                               qstats, qstats_len, tr, s_pre, s_flags, s_red, kSlots, s_kept,
-                               s_fs))
+      if (!grid_barrier_single(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
+                               v0, tdone, qstats, qstats_len, tr, s_pre, s_flags, s_red, kSlots,
+                               s_kept, s_fs, (t + 2) % 4))
         break;
     } else if (!grid_barrier(es, t, spin_limit, fence, &s_ok, fold, tr, es->qlen[t % 4], s_pre)) {
       break;
@@ -1549,8 +1784,12 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   // results of all pairs and the engine head
   if (blockIdx.x == 0) {
     const bool err = ld_sc(&es->error) != 0;
-    for (int p = threadIdx.x; p < npairs; p += blockDim.x)
-      a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
+    if constexpr (FUSED) {
+      if (threadIdx.x == 0) a.outs[0] = finalize_pair(a.desc, ctrs[0], t, err);
+    } else {
+      for (int p = threadIdx.x; p < npairs; p += blockDim.x)
+        a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
+    }
     if (threadIdx.x == 0) {
       // entries left for a next iteration (one pair: the master's count,
       // which includes the workgroups' own queues)
@@ -1566,8 +1805,10 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         h.qtotal = es->qtotal;
         h.qcap = es->qcap;
         h.qnext = qn;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) h.pad1[i] = 0;
+        h.phase[0] = FUSED ? s_phase[0] : 0ull;
+        h.phase[1] = FUSED ? s_phase[1] : 0ull;
+        h.phase[2] = FUSED ? s_phase[2] : 0ull;
+        h.phase[3] = FUSED ? wall_clock64() : 0ull;
         *a.head = h;
       }
     }
@@ -1599,12 +1840,20 @@ hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
 }
 
 template <int TXB, int TY, int TZ>
+hipError_t launch_fused(dim3 grid, hipStream_t s, const DilateArgs &a) {
+  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, true>), grid, dim3(kWgThreads), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int TXB, int TY, int TZ>
 TileLaunch make_launch() {
   return TileLaunch{&launch_pack<TXB, TY, TZ>,
                     &launch_seed<TXB, TY, TZ>,
                     &launch_dilate<TXB, TY, TZ>,
+                    &launch_fused<TXB, TY, TZ>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, false>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, true, true>,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     TXB,
                     TY,

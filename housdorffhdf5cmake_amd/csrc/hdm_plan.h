@@ -34,6 +34,14 @@ struct DilateArgs {
   uint64_t *trace; // per-iteration timestamps (diagnostic) or null
   PairOut *outs;
   EngineHead *head;
+  // the fused single-pair launch (reset, pack, seed and dilation in one
+  // kernel): the pair's descriptor, its tile count, and the epoch base of its
+  // barrier tags (every launch's epochs exceed the previous launch's, so
+  // tagged barrier words never need clearing)
+  PairDesc desc;
+  uint64_t total_tiles;
+  uint32_t ebase;
+  uint32_t pad;
 };
 
 struct TileLaunch {
@@ -42,8 +50,10 @@ struct TileLaunch {
   hipError_t (*seed)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
   hipError_t (*dilate)(dim3, hipStream_t, const DilateArgs &);
+  hipError_t (*fused)(dim3, hipStream_t, const DilateArgs &);  // one pair, one launch
   const void *dilate_fn;         // several pairs
   const void *dilate_single_fn;  // one pair
+  const void *fused_fn;          // one pair, reset + pack + seed + dilation
   const void *pack_fn;
   int txb, ty, tz;
 };

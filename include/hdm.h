@@ -77,7 +77,8 @@ typedef struct hdm_opts {
   int32_t chunk;      /* queue entries a wave takes per claim; 0 = default             */
   int32_t crop;       /* 1 (default) = restrict dilation to the tile bounding box      */
   int32_t timing;     /* HIP-event timing (hdm_last_timings): 1 every phase, 2 also a
-                         per-iteration trace (hdm_last_trace), 3 the dilation only */
+                         per-iteration trace (hdm_last_trace), 3 the dominant launch
+                         only (hdm_last_timings) */
   void *stream;       /* hipStream_t or NULL                                           */
   uint16_t *map_out;  /* optional nx*ny*nz first-hit map: d(v) on FP u FN, 0 elsewhere */
   uint32_t *hist_out; /* optional 2*hist_len first-hit histograms (FP then FN), host   */
@@ -110,11 +111,15 @@ int hdm_device_count(void);
 /* Device bytes one pair of the given dims needs (workspace, excluding inputs). */
 uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
 /* Device milliseconds of the phases of the last single-pair call on the
- * current device that ran with opts->timing = 1, from HIP events on its
- * stream: ms[0] pack (+ reset), ms[1] tile states and first queue, ms[2]
- * persistent dilation (which also finalizes), ms[3] 0, ms[4] all kernels.
- * With opts->timing = 3 only the dilation is bracketed (ms[2]; the others
- * are -1): its events are queued while the pack runs, so they cost no time.
+ * current device that ran with opts->timing = 1: ms[0] pack (+ reset), ms[1]
+ * tile states and first queue, ms[2] persistent dilation (which also
+ * finalizes), ms[3] 0, ms[4] all kernels; ms[5] is 1 when the call ran as one
+ * fused launch (the default for a single pair: reset, pack, seed and dilation
+ * in one persistent kernel, phases then timed by the kernel's own wall clock
+ * between its grid barriers) and 0 when it ran as separate kernels (phases
+ * timed by HIP events on the stream).  With opts->timing = 3 only the
+ * dominant launch is bracketed by events, the others are -1: ms[2] the
+ * dilation kernel of a separate-kernel call, ms[4] the whole fused launch.
  * Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
 /* Per-iteration device trace of the last single-pair call on the current
