@@ -41,7 +41,7 @@ class Opts(C.Structure):
     _fields_ = [("robustness", C.c_double), ("ptr_kind", C.c_int32), ("tile", C.c_int32),
                 ("grid_ctas", C.c_int32), ("chunk", C.c_int32), ("crop", C.c_int32),
                 ("timing", C.c_int32), ("stream", C.c_void_p), ("map_out", C.c_void_p),
-                ("hist_out", C.c_void_p), ("hist_len", C.c_int32), ("reserved1", C.c_int32)]
+                ("hist_out", C.c_void_p), ("hist_len", C.c_int32), ("connectivity", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -71,7 +71,7 @@ class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_slots", C.c_int32), ("slice_iters", C.c_int32),
                 ("tile", C.c_int32), ("grid_ctas", C.c_int32), ("robustness", C.c_double),
                 ("max_nx", C.c_int32), ("max_ny", C.c_int32), ("max_nz", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("connectivity", C.c_int32)]
 
 
 class Completion(C.Structure):
@@ -208,7 +208,7 @@ def _volume_args(gold, segm):
 
 
 def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, stream=None,
-              ptr_kind=PTR_DEVICE, timing=False) -> Opts:
+              ptr_kind=PTR_DEVICE, timing=False, connectivity=6) -> Opts:
     global _OPTS_DEFAULT
     if _OPTS_DEFAULT is None:
         _OPTS_DEFAULT = Opts()
@@ -221,6 +221,7 @@ def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, s
     o.crop = 1 if crop else 0
     o.ptr_kind = ptr_kind
     o.timing = int(timing)
+    o.connectivity = int(connectivity)
     if stream is not None:
         o.stream = stream
     return o
@@ -258,12 +259,14 @@ def last_trace(max_iters: int = 4096):
 
 def hausdorff(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO,
               grid_ctas: int = 0, chunk: int = 0, crop: bool = True, raise_on_error: bool = True,
-              hist_len: int = 0, timing: bool = False):
+              hist_len: int = 0, timing: bool = False, connectivity: int = 6):
     """HD and average HD of two volumes.  Returns a dict of hdm_result fields
-    (plus 'status', and 'hist_fp'/'hist_fn' when hist_len > 0)."""
+    (plus 'status', and 'hist_fp'/'hist_fn' when hist_len > 0).  connectivity:
+    the structuring element, 6 (the reference's cross), 18 (its forBench
+    engine's CHEBYSHEV element) or 26 (the cube)."""
     gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
     o = make_opts(robustness, tile, grid_ctas, chunk, crop,
-                  _current_stream_ptr() if pk == PTR_DEVICE else None, pk, timing)
+                  _current_stream_ptr() if pk == PTR_DEVICE else None, pk, timing, connectivity)
     hist = None
     if hist_len > 0:
         hist = np.zeros(2 * hist_len, np.uint32)
@@ -285,7 +288,8 @@ def hausdorff(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = T
     return d
 
 
-def first_hit_map(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO):
+def first_hit_map(gold, segm, label: int = 1, robustness: float = 1.0, tile: int = TILE_AUTO,
+                  connectivity: int = 6):
     """(result dict, uint16 map [z,y,x]) with d(v) on FP and FN voxels."""
     gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
     if dt != DTYPE_U8:
@@ -294,11 +298,12 @@ def first_hit_map(gold, segm, label: int = 1, robustness: float = 1.0, tile: int
         import torch
         mp = torch.empty((nz, ny, nx), dtype=torch.int16, device=gold.device)
         mptr = C.c_void_p(mp.data_ptr())
-        o = make_opts(robustness, tile, stream=_current_stream_ptr(), ptr_kind=pk)
+        o = make_opts(robustness, tile, stream=_current_stream_ptr(), ptr_kind=pk,
+                      connectivity=connectivity)
     else:
         mp = np.zeros((nz, ny, nx), np.uint16)
         mptr = C.c_void_p(mp.ctypes.data)
-        o = make_opts(robustness, tile, ptr_kind=pk)
+        o = make_opts(robustness, tile, ptr_kind=pk, connectivity=connectivity)
     res = Result()
     rc = lib().hdm_first_hit_map_u8(gp, sp, nx, ny, nz, label, C.byref(o), mptr, C.byref(res))
     del keep
@@ -365,10 +370,10 @@ class Batch:
     sequence is captured once and replayed."""
 
     def __init__(self, vols, label=1, robustness=1.0, tile=TILE_AUTO, chunk=0, grid_ctas=0,
-                 graph=True, crop=True):
+                 graph=True, crop=True, connectivity=6):
         self.pairs, self._keep = _pairs(vols, label)
         self.n = len(vols)
-        o = make_opts(robustness, tile, grid_ctas, chunk, crop)
+        o = make_opts(robustness, tile, grid_ctas, chunk, crop, connectivity=connectivity)
         self.h = C.c_void_p()
         check(lib().hdm_batch_create(self.pairs, self.n, C.byref(o), 1 if graph else 0,
                                      C.byref(self.h)), "hdm_batch_create")

@@ -61,7 +61,7 @@ struct PairDesc {
   int32_t crop;
   int32_t tile_kind;
   uint32_t gen;  // slot generation carried by this pair's queue entries
-  uint32_t pad;
+  int32_t conn;  // structuring element: 6 (cross), 18 (faces, xy diagonals, corners), 26 (cube)
   double robust;
   uint64_t words;  // ntiles * R, words per packed mask
   void *orig[2];

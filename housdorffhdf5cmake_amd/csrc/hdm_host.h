@@ -25,9 +25,9 @@ struct Ctx {
   int dev = -1;
   hipStream_t stream = nullptr;
   int cus = 0;
-  int occ[HDM_TILE_COUNT] = {};       // resident dilate workgroups per CU, per tile kind
-  int occ_pack[HDM_TILE_COUNT] = {};  // resident pack workgroups per CU
-  int occ_fused[HDM_TILE_COUNT] = {};  // resident fused single-pair workgroups per CU
+  int occ[kLaunchIds] = {};       // resident dilate workgroups per CU, per launch id
+  int occ_pack[kLaunchIds] = {};  // resident pack workgroups per CU
+  int occ_fused[kLaunchIds] = {};  // resident fused single-pair workgroups per CU
   double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
@@ -69,6 +69,9 @@ int get_ctx(Ctx **out);
 int resident_grid(const Ctx *c, int tile, int requested);
 uint64_t spin_ticks(const Ctx *c);
 int norm_tile(int tile);
+// launch id (hdm_tile_launch) of a tile option and a connectivity option (0 =
+// 6), or -1 when they are invalid; *conn receives the element
+int select_launch(int tile_opt, int conn_opt, int *conn);
 int barrier_mode();
 bool fused_enabled();
 int fused_grid(const Ctx *c, int tile, int requested);

@@ -82,7 +82,7 @@ int get_ctx(Ctx **out) {
       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess)
     rc = HDM_E_HIP;
   c->clock_khz = khz > 0 ? khz : 100000;
-  for (int k = 1; rc == HDM_OK && k < HDM_TILE_COUNT; ++k) {
+  for (int k = 1; rc == HDM_OK && k < kLaunchIds; ++k) {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->dilate_fn,
                                                      kWgThreads, 0) != hipSuccess ||
@@ -163,6 +163,16 @@ int barrier_mode() {
 }
 
 int norm_tile(int tile) { return (tile <= 0 || tile >= HDM_TILE_COUNT) ? HDM_TILE_64x8x8 : tile; }
+
+int select_launch(int tile_opt, int conn_opt, int *conn) {
+  if (tile_opt < 0 || tile_opt >= HDM_TILE_COUNT) return -1;
+  const int cn = conn_opt == 0 ? 6 : conn_opt;
+  if (cn != 6 && cn != 18 && cn != 26) return -1;
+  // the 18 and 26 elements run on 64x8x8 tiles (one tile per wavefront)
+  if (cn != 6 && tile_opt != HDM_TILE_AUTO && tile_opt != HDM_TILE_64x8x8) return -1;
+  *conn = cn;
+  return launch_id(norm_tile(tile_opt), cn);
+}
 
 // Launch shapes.  Pack: at most the resident waves, each walking one
 // contiguous range of tiles, two wave steps each when the grid allows.  Seed:
@@ -316,7 +326,9 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     return HDM_E_INVALID;
   if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0)) return HDM_E_INVALID;
   if (opts.ptr_kind != HDM_PTR_DEVICE && opts.ptr_kind != HDM_PTR_HOST) return HDM_E_INVALID;
-  if (opts.tile < 0 || opts.tile >= HDM_TILE_COUNT) return HDM_E_INVALID;
+  int conn = 6;
+  const int tile = select_launch(opts.tile, opts.connectivity, &conn);
+  if (tile < 0) return HDM_E_INVALID;
   if (!map_out && opts.map_out) map_out = opts.map_out;
   Ctx *c = nullptr;
   int rc = get_ctx(&c);
@@ -325,7 +337,6 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   // NULL = the legacy default stream, so the call is ordered after work the
   // caller queued on it or on any blocking stream (torch's default stream)
   hipStream_t s = (hipStream_t)opts.stream;
-  const int tile = norm_tile(opts.tile);
   const TileLaunch *tl = hdm_tile_launch(tile);
   const TileShape ts = tile_shape(tile);
   const PairGeom g = pair_geom(nx, ny, nz, ts);
@@ -348,7 +359,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
       return HDM_E_HIP;
     gd = ib, sd = ib + align256(nvox * esz);
   }
-  fill_desc(d, g, gd, sd, dtype, label, opts.robustness, opts.crop ? 1 : 0, tile, 0);
+  fill_desc(d, g, gd, sd, dtype, label, opts.robustness, opts.crop ? 1 : 0, tile, 0, conn);
   carve_pair(d, g, base + L.total);
   if (map_out) {
     if (opts.ptr_kind == HDM_PTR_DEVICE) {
@@ -555,8 +566,10 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   hdm_opts opts;
   hdm_opts_init(&opts);
   if (opts_in) opts = *opts_in;
-  if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0) || opts.tile < 0 ||
-      opts.tile >= HDM_TILE_COUNT || opts.ptr_kind != HDM_PTR_DEVICE)
+  int conn = 6;
+  const int lid = select_launch(opts.tile, opts.connectivity, &conn);
+  if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0) || lid < 0 ||
+      opts.ptr_kind != HDM_PTR_DEVICE)
     return HDM_E_INVALID;
   for (int i = 0; i < n; ++i) {
     const hdm_pair &p = pairs[i];
@@ -570,7 +583,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   hdm_batch *b = new hdm_batch();
   b->ctx = c;
   b->n = n;
-  b->tile = norm_tile(opts.tile);
+  b->tile = lid;
   b->tl = hdm_tile_launch(b->tile);
   b->ts = tile_shape(b->tile);
   b->grid = resident_grid(c, b->tile, opts.grid_ctas);
@@ -609,7 +622,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   for (int i = 0; i < n; ++i) {
     const hdm_pair &p = pairs[i];
     fill_desc(descs[i], b->geoms[i], p.gold, p.segm, p.dtype, p.label, opts.robustness,
-              opts.crop ? 1 : 0, b->tile, 0);
+              opts.crop ? 1 : 0, b->tile, 0, conn);
     carve_pair(descs[i], b->geoms[i], pb);
     pb += b->geoms[i].total;
   }

@@ -537,6 +537,13 @@ __device__ __forceinline__ Box pair_box(const PairDesc &d, const int *lo, const 
   if (d.crop) {
     b.lo[0] = lo[0], b.lo[1] = lo[1], b.lo[2] = lo[2];
     b.hi[0] = hi[0], b.hi[1] = hi[1], b.hi[2] = hi[2];
+    if (d.conn == 18 && b.hi[0] >= 0) {
+      // a shortest path of the 18 element can step one voxel outside the
+      // voxels' box in x or y (offset (2,0,2) is the corners (1,1,1) and
+      // (1,-1,1)): one more tile on each side there
+      b.lo[0] = max(b.lo[0] - 1, 0), b.hi[0] = min(b.hi[0] + 1, d.gx - 1);
+      b.lo[1] = max(b.lo[1] - 1, 0), b.hi[1] = min(b.hi[1] + 1, d.gy - 1);
+    }
   } else {
     b.lo[0] = b.lo[1] = b.lo[2] = 0;
     b.hi[0] = d.gx - 1, b.hi[1] = d.gy - 1, b.hi[2] = d.gz - 1;
@@ -696,6 +703,50 @@ __device__ __forceinline__ uint32_t flush_local_queue(LocalQueue &lq, uint64_t *
 }
 
 // ---------------------------------------------------------------- seed
+// Bit m of the result: an empty tile is reached in one step of the element
+// from a non-empty neighbour of mask m, judged from the neighbours' tile
+// flags (flags(nb) loads them).  CONN 6: a face neighbour has voxels on its
+// face toward the tile (hello.cu:1649-1663).  CONN 18 / 26: a neighbour at
+// offset o among the 26 has voxels on each of its faces that o crosses (the
+// per-face flags make this a superset of the tiles reached; a tile queued
+// early only runs steps that leave it as it is).
+template <int CONN, typename G, typename F>
+__device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_t tile, int tx,
+                                               int ty, int tz, F flags) {
+  uint32_t act = 0;
+  if constexpr (CONN == 6) {
+    // bit of the neighbour's flags that marks its face toward this tile, per
+    // direction (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z)
+    constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
+#pragma unroll
+    for (int dir = 0; dir < 6; ++dir) {
+      const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, dir);
+      if (nb == kNoTile) continue;
+      const uint32_t f = flags(nb);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) act |= ((f >> (8 * m + kFacing[dir])) & 1u) << m;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 27; ++j) {
+      if (j == 13) continue;
+      const int ox = j % 3 - 1, oy = (j / 3) % 3 - 1, oz = j / 9 - 1;
+      const int ux = tx + ox, uy = ty + oy, uz = tz + oz;
+      if (ux < b.lo[0] || ux > b.hi[0] || uy < b.lo[1] || uy > b.hi[1] || uz < b.lo[2] ||
+          uz > b.hi[2])
+        continue;
+      const uint32_t nb = (uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy);
+      const uint32_t f = flags(nb);
+      // the neighbour's faces toward this tile: -x (bit 2) when it lies at +x
+      const uint32_t need = (ox ? 1u << (ox > 0 ? 2 : 3) : 0u) | (oy ? 1u << (oy > 0 ? 4 : 5) : 0u) |
+                            (oz ? 1u << (oz > 0 ? 6 : 7) : 0u);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) act |= (uint32_t)((((f >> (8 * m)) & need) == need)) << m;
+    }
+  }
+  return act;
+}
+
 // One thread per tile decides, for both masks, the tile's state and whether
 // it enters the first queue, from its own flags and those of its six
 // neighbours (no atomics): a non-empty full tile is kFull and stays out; a
@@ -705,7 +756,7 @@ __device__ __forceinline__ uint32_t flush_local_queue(LocalQueue &lq, uint64_t *
 // (hello.cu:940-973), and a pass whose cut is zero queues nothing.  Every
 // entry is first = 1 (input from orig) and check = 1 (count first hits).
 // With one pair the launch also folds the pack partials into the totals.
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, int CONN>
 __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs, Engine *es,
@@ -746,9 +797,6 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
   const uint32_t kq = blockIdx.x % kSubQ;
   uint64_t *gq = sub_queue(q, qcap, t0 & 1, kq);
   uint32_t *gq_len = &es->qlen[t0 % 4][kq];
-  // bit of the neighbour's flags that marks its face toward this tile, per
-  // direction (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z)
-  constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
   for (uint64_t tb = (uint64_t)blockIdx.x * kWgThreads; tb < total_tiles;
        tb += (uint64_t)gridDim.x * kWgThreads) {
     const uint64_t g = tb + threadIdx.x;
@@ -765,14 +813,12 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     const Box b = npairs == 1 ? pair_box(d, s_box, s_box + 3) : pair_box(d, c);
     const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
                         ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
-    uint32_t own = 0, nbf[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t own = 0, reach = 0;
     if (in_box) {
       own = d.tflags[tile];
-#pragma unroll
-      for (int dir = 0; dir < 6; ++dir) {
-        const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, dir);
-        if (nb != kNoTile) nbf[dir] = d.tflags[nb];
-      }
+      reach = seed_reach<CONN>(d, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
+        return d.tflags[nb];
+      });
     }
     const unsigned long long nfp = npairs == 1 ? s_tot[2] : c.n_fp;
     const unsigned long long nfn = npairs == 1 ? s_tot[3] : c.n_fn;
@@ -782,9 +828,7 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     for (int m = 0; m < 2; ++m) {
       const uint32_t f = (own >> (8 * m)) & 0xFF;
       const bool any = f & 1, full = (f >> 1) & 1;
-      bool act = false;
-#pragma unroll
-      for (int dir = 0; dir < 6; ++dir) act |= ((nbf[dir] >> (8 * m + kFacing[dir])) & 1) != 0;
+      const bool act = ((reach >> m) & 1u) != 0;
       const bool needed = robust_cut(d.robust, m == 0 ? nfp : nfn) != 0;
       const uint32_t st = !in_box ? 0u : (any && full) ? kFull : (any || (act && needed)) ? 1u : 0u;
       if (in) d.state[m][tile] = st;
@@ -1248,7 +1292,7 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
 // itself, separated by grid barriers, so one call is one launch; every store
 // that a later phase reads in another workgroup is written through, and those
 // reads are agent-coherent.
-template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false>
+template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6>
 __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   static_assert(!FUSED || SINGLE, "the fused launch holds one pair");
   const PairDesc *__restrict__ descs = a.descs;
@@ -1393,7 +1437,6 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       // the rule of seed_kernel, with agent-coherent flag reads
       uint64_t *Q0 = sub_queue(q, qcap, 0, kq);
       uint32_t *Q0len = &es->qlen[0][kq];
-      constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
       for (uint64_t tb = (uint64_t)blockIdx.x * kWgThreads; tb < a.total_tiles;
            tb += (uint64_t)gridDim.x * kWgThreads) {
         const uint64_t g = tb + threadIdx.x;
@@ -1405,14 +1448,12 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         const Box &b = v0.b;
         const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
                             ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
-        uint32_t own = 0, nbf[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t own = 0, reach = 0;
         if (in_box) {
           own = ld_sc(d.tflags + tile);
-#pragma unroll
-          for (int dir = 0; dir < 6; ++dir) {
-            const uint32_t nb = neighbour(v0, b, tile, tx, ty, tz, dir);
-            if (nb != kNoTile) nbf[dir] = ld_sc(d.tflags + nb);
-          }
+          reach = seed_reach<CONN>(v0, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
+            return ld_sc(d.tflags + nb);
+          });
         }
         bool want[2];
         uint64_t ent[2];
@@ -1420,9 +1461,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         for (int m = 0; m < 2; ++m) {
           const uint32_t f = (own >> (8 * m)) & 0xFF;
           const bool any = f & 1, full = (f >> 1) & 1;
-          bool act = false;
-#pragma unroll
-          for (int dir = 0; dir < 6; ++dir) act |= ((nbf[dir] >> (8 * m + kFacing[dir])) & 1) != 0;
+          const bool act = ((reach >> m) & 1u) != 0;
           const bool needed = v0.cut[m] != 0;
           const uint32_t st = !in_box ? 0u : (any && full) ? kFull : (any || (act && needed)) ? 1u : 0u;
           if (in) st_sc((m ? v0.state[1] : v0.state[0]) + tile, st);
@@ -1546,75 +1585,207 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         uint64_t *xfw = wr ? xf1 : xf0;
         uint32_t *st_m = m ? v.state[1] : v.state[0];
         const uint64_t ro = (uint64_t)tile * R + r;
-        // neighbour tiles: lanes r < 6 also read the state of neighbour r
-        const uint32_t nb_dir = (live && r < 6) ? neighbour(v, v.b, tile, tx, ty, tz, r) : kNoTile;
-        const uint32_t nb_xl = live ? neighbour(v, v.b, tile, tx, ty, tz, 0) : kNoTile;
-        const uint32_t nb_xh = live ? neighbour(v, v.b, tile, tx, ty, tz, 1) : kNoTile;
-        const int on_face[4] = {yl == 0, yl == TY - 1, zl == 0, zl == TZ - 1};
-        const int src_row[4] = {(TY - 1) + TY * zl, TY * zl, yl + TY * (TZ - 1), yl};
-        uint32_t nb_yz[4];
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-          nb_yz[f] = (rowok && on_face[f]) ? neighbour(v, v.b, tile, tx, ty, tz, 2 + f) : kNoTile;
         // -- issue every load of this step before using any of them
         Row own = 0;
         if (rowok) own = first ? ldg(orig_m + ro) : ld_sc(bufr + ro);
         const Row other = (rowok && check) ? ldg(orig_o + ro) : Row(0);
-        const uint32_t nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
-        uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
-        if (nb_xl != kNoTile) {
-          fxl_o = ldg(xfo + 2 * (uint64_t)nb_xl + 1);
-          fxl_b = ld_sc(xfr + 2 * (uint64_t)nb_xl + 1);
-        }
-        if (nb_xh != kNoTile) {
-          fxh_o = ldg(xfo + 2 * (uint64_t)nb_xh);
-          fxh_b = ld_sc(xfr + 2 * (uint64_t)nb_xh);
-        }
-        // y / z face rows from the neighbour tiles, both candidate sources
-        Row fo[4] = {0, 0, 0, 0}, fb[4] = {0, 0, 0, 0};
+        uint32_t nb_dir, nst;  // the neighbour this lane activates and its state
+        Row dl;
+        if constexpr (CONN == 6) {
+          // neighbour tiles: lanes r < 6 also read the state of neighbour r
+          nb_dir = (live && r < 6) ? neighbour(v, v.b, tile, tx, ty, tz, r) : kNoTile;
+          const uint32_t nb_xl = live ? neighbour(v, v.b, tile, tx, ty, tz, 0) : kNoTile;
+          const uint32_t nb_xh = live ? neighbour(v, v.b, tile, tx, ty, tz, 1) : kNoTile;
+          const int on_face[4] = {yl == 0, yl == TY - 1, zl == 0, zl == TZ - 1};
+          const int src_row[4] = {(TY - 1) + TY * zl, TY * zl, yl + TY * (TZ - 1), yl};
+          uint32_t nb_yz[4];
+  #pragma unroll
+          for (int f = 0; f < 4; ++f)
+            nb_yz[f] = (rowok && on_face[f]) ? neighbour(v, v.b, tile, tx, ty, tz, 2 + f) : kNoTile;
+          nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
+          uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
+          if (nb_xl != kNoTile) {
+            fxl_o = ldg(xfo + 2 * (uint64_t)nb_xl + 1);
+            fxl_b = ld_sc(xfr + 2 * (uint64_t)nb_xl + 1);
+          }
+          if (nb_xh != kNoTile) {
+            fxh_o = ldg(xfo + 2 * (uint64_t)nb_xh);
+            fxh_b = ld_sc(xfr + 2 * (uint64_t)nb_xh);
+          }
+          // y / z face rows from the neighbour tiles, both candidate sources
+          Row fo[4] = {0, 0, 0, 0}, fb[4] = {0, 0, 0, 0};
+  #pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            if (nb_yz[f] == kNoTile) continue;
+            const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
+            fo[f] = ldg(orig_m + o);
+            fb[f] = ld_sc(bufr + o);
+          }
+          // -- neighbour states -> which candidate holds iteration tp's input
+          const uint32_t st_xl = shfl32(nst, gbase + 0), st_xh = shfl32(nst, gbase + 1);
+          const uint32_t st_f[4] = {shfl32(nst, gbase + 2), shfl32(nst, gbase + 3),
+                                    shfl32(nst, gbase + 4), shfl32(nst, gbase + 5)};
+          auto pick64 = [&](uint32_t nb, uint32_t st, uint64_t o, uint64_t bsel) -> uint64_t {
+            if (nb == kNoTile) return 0;
+            if (st & kFull) return ~0ull;
+            if (st != 0 && st - 1 < tp) return bsel;
+            return o;
+          };
+          const uint64_t wxl = pick64(nb_xl, st_xl, fxl_o, fxl_b);
+          const uint64_t wxh = pick64(nb_xh, st_xh, fxh_o, fxh_b);
+          Row face[4];
+  #pragma unroll
+          for (int f = 0; f < 4; ++f) {
+            const uint32_t nb = nb_yz[f];
+            const uint32_t st = st_f[f];
+            face[f] = nb == kNoTile ? Row(0)
+                      : (st & kFull) ? ~Row(0)
+                      : (st != 0 && st - 1 < tp) ? fb[f] : fo[f];
+          }
+          // -- 6-connectivity dilation of the row (bitDilatate + faces,
+          //    hello.cu:373-376, 1310-1542)
+          const Row up = shfl_row<Row>(own, lane - 1);
+          const Row dn = shfl_row<Row>(own, lane + 1);
+          const Row bk = shfl_row<Row>(own, lane - TY);
+          const Row fw = shfl_row<Row>(own, lane + TY);
+          dl = own | (Row)(own << 1) | (Row)(own >> 1);
+          dl |= (Row)((wxl >> r) & 1);
+          dl |= (Row)((wxh >> r) & 1) << (TXB - 1);
+          dl |= yl > 0 ? up : face[0];
+          dl |= yl < TY - 1 ? dn : face[1];
+          dl |= zl > 0 ? bk : face[2];
+          dl |= zl < TZ - 1 ? fw : face[3];
+          dl &= vb;
+          if (!rowok) dl = 0;
+        } else {
+          // 18 / 26 elements (one 64x8x8 tile per wave, lane = row): lane j
+          // < 27 reads the state of neighbour j = (ox+1) + 3(oy+1) + 9(oz+1)
+          // and, when ox != 0, both candidates of that tile's face word
+          // toward this one.  A row's (y, z) neighbours are other lanes'
+          // rows (shuffles) or, on the tile's faces, rows of the y, z and
+          // yz-diagonal neighbour tiles (ey, ez, ed: at most one of each
+          // per lane); the x neighbours across the tile's x faces come from
+          // the face words, dilated in (y, z) as 8x8 bit planes.
+          static_assert(CONN == 6 || (TXB == 64 && TY == 8 && TZ == 8), "64x8x8 tiles only");
+          const int j = lane;
+          const int jox = j % 3 - 1, joy = (j / 3) % 3 - 1, joz = j / 9 - 1;
+          nb_dir = kNoTile;
+          if (live && j < 27 && j != 13) {
+            const int ux = tx + jox, uy = ty + joy, uz = tz + joz;
+            if (ux >= v.b.lo[0] && ux <= v.b.hi[0] && uy >= v.b.lo[1] && uy <= v.b.hi[1] &&
+                uz >= v.b.lo[2] && uz <= v.b.hi[2])
+              nb_dir = (uint32_t)((int)tile + jox + joy * v.gx + joz * v.gx * v.gy);
+          }
+          nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
+          uint64_t fwo = 0, fwb = 0;  // face word of neighbour j toward this tile
+          if (nb_dir != kNoTile && jox != 0) {
+            const uint64_t o = 2 * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
+            fwo = ldg(xfo + o);
+            fwb = ld_sc(xfr + o);
+          }
+          // rows beyond the tile: ey (y face), ez (z face), ed (yz corner)
+          const int oy = yl == 0 ? -1 : yl == TY - 1 ? 1 : 0;
+          const int oz = zl == 0 ? -1 : zl == TZ - 1 ? 1 : 0;
+          const int jy = 13 + 3 * oy, jz = 13 + 9 * oz, jd = 13 + 3 * oy + 9 * oz;
+          auto nb_of = [&](int oyy, int ozz) -> uint32_t {
+            const int uy = ty + oyy, uz = tz + ozz;
+            if (uy < v.b.lo[1] || uy > v.b.hi[1] || uz < v.b.lo[2] || uz > v.b.hi[2]) return kNoTile;
+            return (uint32_t)((int)tile + oyy * v.gx + ozz * v.gx * v.gy);
+          };
+          const uint32_t nby = (rowok && oy) ? nb_of(oy, 0) : kNoTile;
+          const uint32_t nbz = (rowok && oz) ? nb_of(0, oz) : kNoTile;
+          const uint32_t nbd = (rowok && oy && oz) ? nb_of(oy, oz) : kNoTile;
+          const int ry = (oy < 0 ? TY - 1 : 0) + TY * zl;     // the row in the y neighbour
+          const int rz = yl + TY * (oz < 0 ? TZ - 1 : 0);     // in the z neighbour
+          const int rd = (oy < 0 ? TY - 1 : 0) + TY * (oz < 0 ? TZ - 1 : 0);
+          Row eyo = 0, eyb = 0, ezo = 0, ezb = 0, edo = 0, edb = 0;
+          if (nby != kNoTile) {
+            eyo = ldg(orig_m + (uint64_t)nby * R + ry);
+            eyb = ld_sc(bufr + (uint64_t)nby * R + ry);
+          }
+          if (nbz != kNoTile) {
+            ezo = ldg(orig_m + (uint64_t)nbz * R + rz);
+            ezb = ld_sc(bufr + (uint64_t)nbz * R + rz);
+          }
+          if (nbd != kNoTile) {
+            edo = ldg(orig_m + (uint64_t)nbd * R + rd);
+            edb = ld_sc(bufr + (uint64_t)nbd * R + rd);
+          }
+          // -- states -> the candidate that holds iteration tp's input; a
+          // full tile holds every valid bit of its rows
+          const Row vfull = valid_bits<Row>(tx, v.nx);
+          auto pick = [&](uint32_t nb, uint32_t st, Row o, Row bsel) -> Row {
+            if (nb == kNoTile) return 0;
+            if (st & kFull) return vfull;
+            if (st != 0 && st - 1 < tp) return bsel;
+            return o;
+          };
+          const Row ey = pick(nby, shfl32(nst, jy), eyo, eyb);
+          const Row ez = pick(nbz, shfl32(nst, jz), ezo, ezb);
+          const Row ed = pick(nbd, shfl32(nst, jd), edo, edb);
+          // face word: a full tile's face has a bit for every row it has
+          uint64_t fw = 0;
+          if (nb_dir != kNoTile && jox != 0) {
+            if (nst & kFull) {
+              const int yy = v.ny - (ty + joy) * TY, zz = v.nz - (tz + joz) * TZ;
+              const uint64_t ym = yy >= TY ? 0xFFull : ((1ull << yy) - 1);
+              uint64_t rows = 0;
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          if (nb_yz[f] == kNoTile) continue;
-          const uint64_t o = (uint64_t)nb_yz[f] * R + src_row[f];
-          fo[f] = ldg(orig_m + o);
-          fb[f] = ld_sc(bufr + o);
-        }
-        // -- neighbour states -> which candidate holds iteration tp's input
-        const uint32_t st_xl = shfl32(nst, gbase + 0), st_xh = shfl32(nst, gbase + 1);
-        const uint32_t st_f[4] = {shfl32(nst, gbase + 2), shfl32(nst, gbase + 3),
-                                  shfl32(nst, gbase + 4), shfl32(nst, gbase + 5)};
-        auto pick64 = [&](uint32_t nb, uint32_t st, uint64_t o, uint64_t bsel) -> uint64_t {
-          if (nb == kNoTile) return 0;
-          if (st & kFull) return ~0ull;
-          if (st != 0 && st - 1 < tp) return bsel;
-          return o;
-        };
-        const uint64_t wxl = pick64(nb_xl, st_xl, fxl_o, fxl_b);
-        const uint64_t wxh = pick64(nb_xh, st_xh, fxh_o, fxh_b);
-        Row face[4];
+              for (int k = 0; k < TZ; ++k) rows |= k < zz ? ym << (TY * k) : 0ull;
+              fw = rows;
+            } else {
+              fw = (nst != 0 && nst - 1 < tp) ? fwb : fwo;
+            }
+          }
+          // -- (y, z) neighbourhood of the row
+          const Row up = shfl_row<Row>(own, lane - 1), dn = shfl_row<Row>(own, lane + 1);
+          const Row ysh = (yl > 0 ? up : ey) | (yl < TY - 1 ? dn : ey);  // rows y-1, y+1
+          const Row zup = shfl_row<Row>(ez, lane - 1), zdn = shfl_row<Row>(ez, lane + 1);
+          const Row zsh = (yl > 0 ? zup : ed) | (yl < TY - 1 ? zdn : ed);  // beyond z: y-1, y+1
+          const Row yc = own | ysh;  // rows y-1 .. y+1 at z
+          Row a_row, s_row;          // a_row: rows whose voxels also spread in x; s_row: x only
+          if constexpr (CONN == 26) {
+            const Row ycz = ez | zsh;
+            const Row bk = shfl_row<Row>(yc, lane - TY), fw2 = shfl_row<Row>(yc, lane + TY);
+            a_row = yc | (zl > 0 ? bk : ycz) | (zl < TZ - 1 ? fw2 : ycz);
+            s_row = a_row;
+          } else {  // 18: faces, xy diagonals, corners
+            const Row bk = shfl_row<Row>(own, lane - TY), fw2 = shfl_row<Row>(own, lane + TY);
+            const Row sbk = shfl_row<Row>(ysh, lane - TY), sfw = shfl_row<Row>(ysh, lane + TY);
+            const Row zrow = (zl > 0 ? bk : ez) | (zl < TZ - 1 ? fw2 : ez);      // (0, +-1)
+            const Row corner = (zl > 0 ? sbk : zsh) | (zl < TZ - 1 ? sfw : zsh);  // (+-1, +-1)
+            a_row = yc | zrow;
+            s_row = yc | corner;
+          }
+          // -- the x halo: the face planes of the 9 tiles at x-1 (lanes j =
+          // 3k) and x+1 (j = 3k+2), k = (oy+1) + 3(oz+1), dilated in (y, z)
+          // like the rows (bit yl + 8 zl of a plane is row (yl, zl))
+          auto plane_of = [&](int jj) -> uint64_t {
+            return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fw, jj)) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(fw >> 32), jj)
+                    << 32);
+          };
+          auto halo = [&](int side) -> uint64_t {  // side 0: x-1, 2: x+1
+            constexpr uint64_t kY0 = 0x0101010101010101ull, kY7 = 0x8080808080808080ull;
+            uint64_t y3[3], ys[3];
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          const uint32_t nb = nb_yz[f];
-          const uint32_t st = st_f[f];
-          face[f] = nb == kNoTile ? Row(0)
-                    : (st & kFull) ? ~Row(0)
-                    : (st != 0 && st - 1 < tp) ? fb[f] : fo[f];
+            for (int kz = 0; kz < 3; ++kz) {
+              const uint64_t c = plane_of(side + 3 * (1 + 3 * kz));
+              const uint64_t m = plane_of(side + 3 * (0 + 3 * kz));
+              const uint64_t pp = plane_of(side + 3 * (2 + 3 * kz));
+              ys[kz] = ((c >> 1) & ~kY7) | ((pp & kY0) << 7) | ((c << 1) & ~kY0) | ((m & kY7) >> 7);
+              y3[kz] = c | ys[kz];
+            }
+            const uint64_t *zs = CONN == 26 ? y3 : ys;
+            return y3[1] | (zs[1] >> 8) | (zs[2] << 56) | (zs[1] << 8) | (zs[0] >> 56);
+          };
+          const uint64_t hlo = halo(0), hhi = halo(2);
+          dl = a_row | (Row)(s_row << 1) | (Row)(s_row >> 1);
+          dl |= (Row)((hlo >> r) & 1);
+          dl |= (Row)((hhi >> r) & 1) << (TXB - 1);
+          dl &= vb;
+          if (!rowok) dl = 0;
         }
-        // -- 6-connectivity dilation of the row (bitDilatate + faces,
-        //    hello.cu:373-376, 1310-1542)
-        const Row up = shfl_row<Row>(own, lane - 1);
-        const Row dn = shfl_row<Row>(own, lane + 1);
-        const Row bk = shfl_row<Row>(own, lane - TY);
-        const Row fw = shfl_row<Row>(own, lane + TY);
-        Row dl = own | (Row)(own << 1) | (Row)(own >> 1);
-        dl |= (Row)((wxl >> r) & 1);
-        dl |= (Row)((wxh >> r) & 1) << (TXB - 1);
-        dl |= yl > 0 ? up : face[0];
-        dl |= yl < TY - 1 ? dn : face[1];
-        dl |= zl > 0 ? bk : face[2];
-        dl |= zl < TZ - 1 ? fw : face[3];
-        dl &= vb;
-        if (!rowok) dl = 0;
         // -- input full -> the tile leaves the queue (hello.cu:1319, 1634-1643)
         const bool full = group_bits<R>(__ballot(live && exists && own != vb), lane) == 0;
         // -- newly covered voxels of the other mask: res & ~src & ref (hello.cu:1590-1626)
@@ -1640,6 +1811,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         // -- outputs
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
+        const uint64_t oany = group_bits<R>(__ballot(dl != 0), lane);
         const bool oyl = group_bits<R>(__ballot(yl == 0 && dl != 0), lane) != 0;
         const bool oyh = group_bits<R>(__ballot(yl == TY - 1 && dl != 0), lane) != 0;
         const bool ozl = group_bits<R>(__ballot(zl == 0 && dl != 0), lane) != 0;
@@ -1655,14 +1827,27 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
               st_sc(xfw + 2 * (uint64_t)tile, oxl);
               st_sc(xfw + 2 * (uint64_t)tile + 1, oxh);
             }
-            if (r == 6) {  // stays queued while not full
+            // lane kSelf re-queues the tile while it is not full; lanes
+            // below it activate idle neighbours the output reaches
+            constexpr int kSelf = CONN == 6 ? 6 : 27;
+            if (r == kSelf) {
               want = true;
               pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0, v.gen);
             }
-            if (r < 6 && nb_dir != kNoTile && nst == 0) {  // activate idle neighbours
-              const bool face_on = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
-                                   : r == 3 ? oyh : r == 4 ? ozl : ozh;
-              if (face_on && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
+            if (r < kSelf && nb_dir != kNoTile && nst == 0) {
+              bool reach;
+              if constexpr (CONN == 6) {
+                reach = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
+                        : r == 3 ? oyh : r == 4 ? ozl : ozh;
+              } else {  // a voxel on every face that the offset of neighbour r crosses
+                const int jox = r % 3 - 1, joy = (r / 3) % 3 - 1, joz = r / 9 - 1;
+                const uint64_t xs = jox < 0 ? oxl : jox > 0 ? oxh : oany;
+                const uint64_t ym = joy < 0 ? 0x0101010101010101ull
+                                    : joy > 0 ? 0x8080808080808080ull : ~0ull;
+                const uint64_t zm = joz < 0 ? 0xFFull : joz > 0 ? 0xFFull << 56 : ~0ull;
+                reach = (xs & ym & zm) != 0;
+              }
+              if (reach && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
                 want = true;
                 pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
               }
@@ -1822,52 +2007,59 @@ hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
   hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw);
   return hipGetLastError();
 }
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, int CONN>
 hipError_t launch_seed(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
                        PairCtr *c, int np, Engine *es, uint64_t *q, uint32_t qcap, uint64_t tt,
                        int p0) {
-  hipLaunchKernelGGL((seed_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, es,
-                     q, qcap, tt, p0);
+  hipLaunchKernelGGL((seed_kernel<TXB, TY, TZ, CONN>), grid, dim3(kWgThreads), 0, s, d, ix, c,
+                     np, es, q, qcap, tt, p0);
   return hipGetLastError();
 }
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, int CONN>
 hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
   if (a.npairs == 1)
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true>), grid, dim3(kWgThreads), 0, s, a);
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, false, CONN>), grid, dim3(kWgThreads),
+                       0, s, a);
   else
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false>), grid, dim3(kWgThreads), 0, s, a);
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false, false, CONN>), grid, dim3(kWgThreads),
+                       0, s, a);
   return hipGetLastError();
 }
 
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, int CONN>
 hipError_t launch_fused(dim3 grid, hipStream_t s, const DilateArgs &a) {
-  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, true>), grid, dim3(kWgThreads), 0, s, a);
+  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, true, CONN>), grid, dim3(kWgThreads), 0,
+                     s, a);
   return hipGetLastError();
 }
 
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, int CONN = 6>
 TileLaunch make_launch() {
   return TileLaunch{&launch_pack<TXB, TY, TZ>,
-                    &launch_seed<TXB, TY, TZ>,
-                    &launch_dilate<TXB, TY, TZ>,
-                    &launch_fused<TXB, TY, TZ>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, false>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, true>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, true, true>,
+                    &launch_seed<TXB, TY, TZ, CONN>,
+                    &launch_dilate<TXB, TY, TZ, CONN>,
+                    &launch_fused<TXB, TY, TZ, CONN>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     TXB,
                     TY,
-                    TZ};
+                    TZ,
+                    CONN};
 }
 
 }  // namespace hdm
 
-// table indexed by HDM_TILE_* (1..4)
+// table indexed by launch id: HDM_TILE_* (1..4) with the 6 element, then the
+// 64x8x8 tile with the 18 (5) and the 26 (6) element
 extern "C" const hdm::TileLaunch *hdm_tile_launch(int kind) {
-  static const hdm::TileLaunch table[5] = {
-      hdm::make_launch<64, 8, 8>(), hdm::make_launch<64, 8, 8>(), hdm::make_launch<64, 4, 4>(),
-      hdm::make_launch<64, 8, 4>(), hdm::make_launch<32, 8, 8>()};
-  if (kind < 0 || kind > 4) return nullptr;
+  static const hdm::TileLaunch table[hdm::kLaunchIds] = {
+      hdm::make_launch<64, 8, 8>(),     hdm::make_launch<64, 8, 8>(),
+      hdm::make_launch<64, 4, 4>(),     hdm::make_launch<64, 8, 4>(),
+      hdm::make_launch<32, 8, 8>(),     hdm::make_launch<64, 8, 8, 18>(),
+      hdm::make_launch<64, 8, 8, 26>()};
+  if (kind < 0 || kind >= hdm::kLaunchIds) return nullptr;
   return &table[kind];
 }
 

@@ -56,7 +56,15 @@ struct TileLaunch {
   const void *fused_fn;          // one pair, reset + pack + seed + dilation
   const void *pack_fn;
   int txb, ty, tz;
+  int conn;  // structuring element of the seed and dilation kernels
 };
+
+// launch ids (hdm_tile_launch): the HDM_TILE_* kinds with the 6 element, then
+// the 64x8x8 tile with the 18 and the 26 element
+constexpr int kLaunchIds = 7;
+inline int launch_id(int tile_kind, int conn) {
+  return conn == 18 ? 5 : conn == 26 ? 6 : tile_kind;
+}
 
 struct TileShape {
   int txb, ty, tz;
@@ -129,7 +137,7 @@ inline void carve_pair(PairDesc &d, const PairGeom &g, char *base) {
 
 inline void fill_desc(PairDesc &d, const PairGeom &g, const void *gold, const void *segm,
                       int dtype, int label, double robust, int crop, int tile_kind,
-                      uint32_t t_start) {
+                      uint32_t t_start, int conn) {
   d.gold = gold, d.segm = segm;
   d.nx = g.nx, d.ny = g.ny, d.nz = g.nz, d.wxn = g.wxn;
   d.gx = g.gx, d.gy = g.gy, d.gz = g.gz;
@@ -142,6 +150,7 @@ inline void fill_desc(PairDesc &d, const PairGeom &g, const void *gold, const vo
   d.robust = robust;
   d.words = g.words;
   d.map = nullptr;
+  d.conn = conn;
 }
 
 // Shared (per launch group) region: descriptors, indices, counters, outputs,

@@ -91,7 +91,8 @@ struct hdm_server {
   hdm_server_opts o{};
   Ctx *ctx = nullptr;
   hipStream_t s = nullptr, s_copy = nullptr;
-  int tile = HDM_TILE_64x8x8;
+  int tile = HDM_TILE_64x8x8;  // launch id
+  int conn = 6;
   const TileLaunch *tl = nullptr;
   TileShape ts{};
   LaunchCfg cfg{};
@@ -163,8 +164,10 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   hdm_server_opts o;
   hdm_server_opts_init(&o);
   if (oin) o = *oin;
-  if (o.max_slots < 2 || o.max_slots > 4096 || o.slice_iters < 1 || o.tile < 0 ||
-      o.tile >= HDM_TILE_COUNT || !(o.robustness >= 0.0 && o.robustness <= 1.0) ||
+  int conn = 6;
+  const int lid = select_launch(o.tile, o.connectivity, &conn);
+  if (o.max_slots < 2 || o.max_slots > 4096 || o.slice_iters < 1 || lid < 0 ||
+      !(o.robustness >= 0.0 && o.robustness <= 1.0) ||
       o.max_nx < 1 || o.max_ny < 1 || o.max_nz < 1 || o.hbm_budget < (1u << 20))
     return HDM_E_INVALID;
   Ctx *c = nullptr;
@@ -173,7 +176,8 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   hdm_server *s = new hdm_server();
   s->o = o;
   s->ctx = c;
-  s->tile = norm_tile(o.tile);
+  s->tile = lid;
+  s->conn = conn;
   s->tl = hdm_tile_launch(s->tile);
   s->ts = tile_shape(s->tile);
   const PairGeom gmax = pair_geom(o.max_nx, o.max_ny, o.max_nz, s->ts);
@@ -283,7 +287,8 @@ static int admit(hdm_server *s, Request &r, int p) {
     gd = base, sd = base + inb / 2;
   }
   PairDesc d{};
-  fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, s->t_engine);
+  fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, s->t_engine,
+            s->conn);
   carve_pair(d, r.geom, base + inb);
   d.gen = (++s->slot_gen[p]) & kGenMask;  // drops queue entries of the slot's last owner
   const PackIndex pi{0, 0};

@@ -24,7 +24,7 @@ RESULT_KEYS = ("hd", "sum", "count", "sum_fp", "cnt_fp", "sum_fn", "cnt_fn", "n_
 
 class Server:
     def __init__(self, hbm_budget=64 << 30, max_slots=64, slice_iters=8, tile=TILE_64x8x8,
-                 grid_ctas=0, robustness=1.0, max_dims=(512, 512, 512)):
+                 grid_ctas=0, robustness=1.0, max_dims=(512, 512, 512), connectivity=6):
         o = ServerOpts()
         lib().hdm_server_opts_init(C.byref(o))
         o.hbm_budget = int(hbm_budget)
@@ -34,6 +34,7 @@ class Server:
         o.grid_ctas = int(grid_ctas)
         o.robustness = float(robustness)
         o.max_nx, o.max_ny, o.max_nz = (int(v) for v in max_dims)
+        o.connectivity = int(connectivity)
         self.h = C.c_void_p()
         check(lib().hdm_server_create(C.byref(o), C.byref(self.h)), "hdm_server_create")
         self.max_slots = max_slots

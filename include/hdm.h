@@ -83,7 +83,12 @@ typedef struct hdm_opts {
   uint16_t *map_out;  /* optional nx*ny*nz first-hit map: d(v) on FP u FN, 0 elsewhere */
   uint32_t *hist_out; /* optional 2*hist_len first-hit histograms (FP then FN), host   */
   int32_t hist_len;   /* length of each histogram in hist_out                          */
-  int32_t reserved1;
+  int32_t connectivity; /* structuring element of the dilation: 0 or 6 the 6-neighbour
+                           cross (hello.cu:373-376, 1310-1542); 18 the element of the
+                           reference's second engine with CHEBYSHEV 1 (the six faces,
+                           the four xy-plane diagonals and the eight corners,
+                           forBench/HausdorffDistance.cu:37-60); 26 the 3x3x3 cube
+                           (Chebyshev distance).  18 and 26 need tile AUTO or 64x8x8 */
 } hdm_opts;
 
 typedef struct hdm_result {
@@ -216,7 +221,7 @@ typedef struct hdm_server_opts {
   int32_t grid_ctas;     /* 0 = occupancy-derived                           */
   double robustness;
   int32_t max_nx, max_ny, max_nz; /* largest request dims (sizes the queue)  */
-  int32_t reserved;
+  int32_t connectivity;  /* structuring element, as hdm_opts.connectivity   */
 } hdm_server_opts;
 
 typedef struct hdm_completion {

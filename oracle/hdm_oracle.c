@@ -86,20 +86,44 @@ int oracle_max_threads(void) { return omp_get_max_threads(); }
  * naive: one byte per voxel
  * ---------------------------------------------------------------------- */
 
-/* dst = src dilated by the 6-connectivity cross, clipped to the volume. */
-static void dilate6(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz) {
+/* Offsets of a structuring element: conn 6 the cross (hello.cu:373-376,
+ * 1310-1542); 18 the element of forBench/HausdorffDistance.cu with CHEBYSHEV
+ * 1 (HausdorffDistance.cuh:8): the six faces (.cu:37-44), the four xy-plane
+ * diagonals (.cu:48-51) and the eight corners (.cu:53-60); 26 the cube.
+ * Returns the number of offsets, 0 for an unknown conn. */
+static int se_offsets(int conn, int off[26][3]) {
+  int n = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int nz_ = (dx != 0) + (dy != 0) + (dz != 0);
+        int in = 0;
+        if (nz_ == 0) continue;
+        if (conn == 6) in = nz_ == 1;
+        else if (conn == 18) in = nz_ == 1 || (nz_ == 2 && dz == 0) || nz_ == 3;
+        else if (conn == 26) in = 1;
+        else return 0;
+        if (in) off[n][0] = dx, off[n][1] = dy, off[n][2] = dz, ++n;
+      }
+  return n;
+}
+
+/* dst = src dilated by the element, clipped to the volume: dst[v] = OR over
+ * offsets o of src[v + o] (the element is symmetric, so this equals the
+ * reference's scatter of every set voxel to v + o). */
+static void dilate_se(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz, int n_off,
+                      const int (*off)[3]) {
   const uint64_t sy = (uint64_t)nx, sz = (uint64_t)nx * ny;
   for (int z = 0; z < nz; ++z)
     for (int y = 0; y < ny; ++y)
       for (int x = 0; x < nx; ++x) {
         const uint64_t i = x + sy * y + sz * z;
         uint8_t v = src[i];
-        if (!v && x > 0) v = src[i - 1];
-        if (!v && x < nx - 1) v = src[i + 1];
-        if (!v && y > 0) v = src[i - sy];
-        if (!v && y < ny - 1) v = src[i + sy];
-        if (!v && z > 0) v = src[i - sz];
-        if (!v && z < nz - 1) v = src[i + sz];
+        for (int k = 0; k < n_off && !v; ++k) {
+          const int xx = x + off[k][0], yy = y + off[k][1], zz = z + off[k][2];
+          if (xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+          v = src[xx + sy * yy + sz * zz];
+        }
         dst[i] = v;
       }
 }
@@ -107,8 +131,17 @@ static void dilate6(const uint8_t *src, uint8_t *dst, int nx, int ny, int nz) {
 int oracle_naive(const void *gold, const void *segm, int dtype, int32_t label, int nx, int ny,
                  int nz, double r, uint32_t *hist_fp, uint32_t *hist_fn, int hist_len,
                  uint16_t *map, oracle_result *out) {
+  return oracle_naive_se(gold, segm, dtype, label, nx, ny, nz, r, 6, hist_fp, hist_fn, hist_len,
+                         map, out);
+}
+
+int oracle_naive_se(const void *gold, const void *segm, int dtype, int32_t label, int nx,
+                    int ny, int nz, double r, int conn, uint32_t *hist_fp, uint32_t *hist_fn,
+                    int hist_len, uint16_t *map, oracle_result *out) {
+  int off[26][3];
+  const int n_off = se_offsets(conn, off);
   if (!gold || !segm || !out || !hist_fp || !hist_fn || nx < 1 || ny < 1 || nz < 1 ||
-      !(r >= 0.0 && r <= 1.0) || hist_len < 2)
+      !(r >= 0.0 && r <= 1.0) || hist_len < 2 || n_off == 0)
     return E_INVALID;
   memset(out, 0, sizeof(*out));
   const uint64_t n = (uint64_t)nx * ny * nz;
@@ -149,7 +182,7 @@ int oracle_naive(const void *gold, const void *segm, int dtype, int32_t label, i
       uint8_t *cur = m == 0 ? cg : cs;
       const uint8_t *ref = m == 0 ? s : g; /* the other mask's original */
       uint32_t *hist = m == 0 ? hist_fp : hist_fn;
-      dilate6(cur, nx_, nx, ny, nz);
+      dilate_se(cur, nx_, nx, ny, nz, n_off, (const int(*)[3])off);
       for (uint64_t i = 0; i < n; ++i) {
         if (nx_[i] && !cur[i] && ref[i]) { /* res & ~src & ref */
           hist[t]++;
@@ -205,33 +238,90 @@ static void vpush(vec_t *q, uint32_t e) {
   q->v[q->n++] = e;
 }
 
+/* word (wx, y, z) of a packed mask, 0 outside the volume */
+static inline uint64_t word_at(const geom_t *G, const uint64_t *src, int wx, int y, int z) {
+  if (y < 0 || z < 0 || y >= G->ny || z >= G->nz) return 0;
+  return src[widx(G, wx, y, z)];
+}
+
+/* the x-neighbour bits of word (wx, y, z): w << 1 | w >> 1 plus the carries
+ * from the adjacent words of the row (bitDilatate, hello.cu:373-376, and the
+ * x faces, hello.cu:1310-1542); 0 outside the volume */
+static inline uint64_t x_shift(const geom_t *G, const uint64_t *src, int wx, int y, int z) {
+  if (y < 0 || z < 0 || y >= G->ny || z >= G->nz) return 0;
+  const uint64_t i = widx(G, wx, y, z);
+  const uint64_t w = src[i];
+#if ORACLE_MUTANT == 4
+  uint64_t d = w << 1;
+#else
+  uint64_t d = (w << 1) | (w >> 1);
+#endif
+  if (wx > 0) d |= src[i - 1] >> 63;
+  if (wx < G->wx - 1) d |= src[i + 1] << 63;
+  return d;
+}
+
+/* the element's dilation of word (tx, y, z):
+ *   6  w | x(w) | y-1 | y+1 | z-1 | z+1
+ *   18 (the forBench element) the faces, the xy diagonals (x of the y-1 and
+ *      y+1 rows) and the corners (x of the four rows at y+-1, z+-1)
+ *   26 every row of the 3x3 (y, z) neighbourhood with its x neighbours */
+static inline uint64_t dilate_word(const geom_t *G, const uint64_t *src, int conn, int tx, int y,
+                                   int z) {
+  uint64_t d = word_at(G, src, tx, y, z) | x_shift(G, src, tx, y, z);
+  if (conn == 26) {
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        if (dy || dz) d |= word_at(G, src, tx, y + dy, z + dz) | x_shift(G, src, tx, y + dy, z + dz);
+    return d;
+  }
+  d |= word_at(G, src, tx, y - 1, z) | word_at(G, src, tx, y + 1, z);
+  d |= word_at(G, src, tx, y, z - 1);
+#if ORACLE_MUTANT != 2
+  d |= word_at(G, src, tx, y, z + 1);
+#endif
+  if (conn == 18) {
+    d |= x_shift(G, src, tx, y - 1, z) | x_shift(G, src, tx, y + 1, z);
+    for (int dz = -1; dz <= 1; dz += 2)
+      for (int dy = -1; dy <= 1; dy += 2) d |= x_shift(G, src, tx, y + dy, z + dz);
+  }
+  return d;
+}
+
+/* Bit j = (ox+1) + 3(oy+1) + 9(oz+1) of the result is set when word d of row
+ * (y, z) of tile (y0, z0) has a voxel on every tile face that offset (ox, oy,
+ * oz) crosses: the neighbour tile at that offset can be reached from it in
+ * one step of the 26 element (a superset of the 18 element's reach). */
+static uint32_t reach_bits(uint64_t d, int y, int z, int y0, int z0) {
+  if (!d) return 0;
+  const int xs = 2 | (int)(d & 1) | (int)((d >> 63) << 2);
+  const int ys = 2 | (y == y0) | ((y == y0 + TYC - 1) << 2);
+  const int zs = 2 | (z == z0) | ((z == z0 + TZC - 1) << 2);
+  uint32_t m = 0;
+  for (int iz = 0; iz < 3; ++iz)
+    for (int iy = 0; iy < 3; ++iy)
+      for (int ix = 0; ix < 3; ++ix)
+        if (((xs >> ix) & 1) && ((ys >> iy) & 1) && ((zs >> iz) & 1)) m |= 1u << (ix + 3 * iy + 9 * iz);
+  return m & ~(1u << 13);
+}
+
 /* Dilate one tile of mask m from src into dst.  Returns the input-full flag;
- * faces[6] receives whether the output has bits on -x,+x,-y,+y,-z,+z. */
+ * faces[6] receives whether the output has bits on -x,+x,-y,+y,-z,+z and,
+ * for conn 18 / 26, *reach the neighbours the output reaches (reach_bits). */
 static int tile_step(const geom_t *G, const uint64_t *src, uint64_t *dst, const uint64_t *ref,
-                     int tx, int ty, int tz, int t, uint16_t *map, uint64_t *cnt, int faces[6]) {
+                     int conn, int tx, int ty, int tz, int t, uint16_t *map, uint64_t *cnt,
+                     int faces[6], uint32_t *reach) {
   int full = 1;
   const int y0 = ty * TYC, y1 = (y0 + TYC < G->ny) ? y0 + TYC : G->ny;
   const int z0 = tz * TZC, z1 = (z0 + TZC < G->nz) ? z0 + TZC : G->nz;
   const uint64_t vb = valid_bits(G, tx);
   uint64_t fxl = 0, fxh = 0, fyl = 0, fyh = 0, fzl = 0, fzh = 0, c = 0;
+  uint32_t rb = 0;
   for (int z = z0; z < z1; ++z)
     for (int y = y0; y < y1; ++y) {
       const uint64_t i = widx(G, tx, y, z);
       const uint64_t w = src[i];
-#if ORACLE_MUTANT == 4
-      uint64_t d = w | (w << 1);
-#else
-      uint64_t d = w | (w << 1) | (w >> 1);
-#endif
-      if (tx > 0) d |= src[i - 1] >> 63;
-      if (tx < G->wx - 1) d |= src[i + 1] << 63;
-      if (y > 0) d |= src[i - G->wx];
-      if (y < G->ny - 1) d |= src[i + G->wx];
-      if (z > 0) d |= src[i - (uint64_t)G->wx * G->ny];
-#if ORACLE_MUTANT != 2
-      if (z < G->nz - 1) d |= src[i + (uint64_t)G->wx * G->ny];
-#endif
-      d &= vb;
+      uint64_t d = dilate_word(G, src, conn, tx, y, z) & vb;
       if (w != vb) full = 0;
       const uint64_t nw = d & ~w & ref[i];
       if (nw) {
@@ -253,18 +343,41 @@ static int tile_step(const geom_t *G, const uint64_t *src, uint64_t *dst, const 
       if (y == y1 - 1) fyh |= d;
       if (z == z0) fzl |= d;
       if (z == z1 - 1) fzh |= d;
+      if (conn != 6) rb |= reach_bits(d, y, z, y0, z0);
     }
   faces[0] = fxl != 0, faces[1] = fxh != 0, faces[2] = fyl != 0;
   faces[3] = fyh != 0, faces[4] = fzl != 0, faces[5] = fzh != 0;
   *cnt = c;
+  *reach = rb;
   return full;
+}
+
+/* claims neighbour (ux, uy, uz) of mask m for the next queue when it is in
+ * the grid, in the box and idle */
+static inline void activate(const geom_t *G, uint8_t *state, int ux, int uy, int uz,
+                            vec_t *out) {
+  if (ux < 0 || uy < 0 || uz < 0 || ux >= G->gx || uy >= G->gy || uz >= G->gz) return;
+  if (!in_box(G, ux, uy, uz)) return;
+  const uint64_t ul = ux + (uint64_t)G->gx * (uy + (uint64_t)G->gy * uz);
+  uint8_t expect = ST_IDLE;
+  if (__atomic_compare_exchange_n(&state[ul], &expect, ST_ACTIVE, 0, __ATOMIC_RELAXED,
+                                  __ATOMIC_RELAXED))
+    vpush(out, (uint32_t)ul);
 }
 
 int oracle_tiled(const void *gold, const void *segm, int dtype, int32_t label, int nx, int ny,
                  int nz, double r, int threads, int crop, uint32_t *hist_fp, uint32_t *hist_fn,
                  int hist_len, uint16_t *map, oracle_result *out) {
+  return oracle_tiled_se(gold, segm, dtype, label, nx, ny, nz, r, 6, threads, crop, hist_fp,
+                         hist_fn, hist_len, map, out);
+}
+
+int oracle_tiled_se(const void *gold, const void *segm, int dtype, int32_t label, int nx,
+                    int ny, int nz, double r, int conn, int threads, int crop,
+                    uint32_t *hist_fp, uint32_t *hist_fn, int hist_len, uint16_t *map,
+                    oracle_result *out) {
   if (!gold || !segm || !out || !hist_fp || !hist_fn || nx < 1 || ny < 1 || nz < 1 ||
-      !(r >= 0.0 && r <= 1.0) || hist_len < 2)
+      !(r >= 0.0 && r <= 1.0) || hist_len < 2 || (conn != 6 && conn != 18 && conn != 26))
     return E_INVALID;
   if (threads > 0) omp_set_num_threads(threads);
   memset(out, 0, sizeof(*out));
@@ -342,6 +455,15 @@ int oracle_tiled(const void *gold, const void *segm, int dtype, int32_t label, i
   if (!crop) {
     G.lo[0] = G.lo[1] = G.lo[2] = 0;
     G.hi[0] = G.gx - 1, G.hi[1] = G.gy - 1, G.hi[2] = G.gz - 1;
+  } else if (conn == 18) {
+    /* a shortest path of the 18 element may step one voxel outside the
+     * voxels' box in x or y (offset (2,0,2) takes the corners (1,1,1) and
+     * (1,-1,1)): one more tile on each side in x and y */
+    for (int k = 0; k < 2; ++k) {
+      const int g = k == 0 ? G.gx : G.gy;
+      if (G.lo[k] > 0) --G.lo[k];
+      if (G.hi[k] >= 0 && G.hi[k] < g - 1) ++G.hi[k];
+    }
   }
   const uint64_t tgt[2] = {cut_of(r, nfp), cut_of(r, nfn)};
   uint64_t cov[2] = {0, 0};
@@ -363,6 +485,16 @@ int oracle_tiled(const void *gold, const void *segm, int dtype, int32_t label, i
     for (uint64_t tl = 0; tl < ntile; ++tl) {
       if (!cls[m][tl]) continue;
       const int tx = (int)(tl % G.gx), ty = (int)((tl / G.gx) % G.gy), tz = (int)(tl / ((uint64_t)G.gx * G.gy));
+      if (conn != 6) { /* the neighbours the tile's voxels reach */
+        if (!in_box(&G, tx, ty, tz)) continue;
+        uint32_t rb = 0;
+        for (int z = tz * TZC; z < nz && z < tz * TZC + TZC; ++z)
+          for (int y = ty * TYC; y < ny && y < ty * TYC + TYC; ++y)
+            rb |= reach_bits(orig[m][widx(&G, tx, y, z)], y, z, ty * TYC, tz * TZC);
+        for (int j = 0; j < 27; ++j)
+          if ((rb >> j) & 1) activate(&G, state[m], tx + j % 3 - 1, ty + (j / 3) % 3 - 1, tz + j / 9 - 1, &q[m]);
+        continue;
+      }
       for (int f = 0; f < 6; ++f) {
         const int ux = tx + dtx[f], uy = ty + dty[f], uz = tz + dtz[f];
         if (ux < 0 || uy < 0 || uz < 0 || ux >= G.gx || uy >= G.gy || uz >= G.gz) continue;
@@ -429,24 +561,22 @@ int oracle_tiled(const void *gold, const void *segm, int dtype, int32_t label, i
                     tz = (int)(tl / ((uint64_t)G.gx * G.gy));
           int faces[6];
           uint64_t c;
-          const int full = tile_step(&G, src, dst, ref, tx, ty, tz, t, map, &c, faces);
+          uint32_t rb;
+          const int full = tile_step(&G, src, dst, ref, conn, tx, ty, tz, t, map, &c, faces, &rb);
           total += c;
           if (full) {
             state[m][tl] = ST_FULL;
           } else {
             vpush(&local, tl);
           }
-          for (int f = 0; f < 6; ++f) {
-            if (!faces[f]) continue;
-            const int ux = tx + dtx[f], uy = ty + dty[f], uz = tz + dtz[f];
-            if (ux < 0 || uy < 0 || uz < 0 || ux >= G.gx || uy >= G.gy || uz >= G.gz) continue;
-            if (!in_box(&G, ux, uy, uz)) continue;
-            const uint64_t ul = ux + (uint64_t)G.gx * (uy + (uint64_t)G.gy * uz);
-            uint8_t expect = ST_IDLE;
-            if (__atomic_compare_exchange_n(&state[m][ul], &expect, ST_ACTIVE, 0,
-                                            __ATOMIC_RELAXED, __ATOMIC_RELAXED))
-              vpush(&local, (uint32_t)ul);
+          if (conn != 6) {
+            for (int j = 0; j < 27; ++j)
+              if ((rb >> j) & 1)
+                activate(&G, state[m], tx + j % 3 - 1, ty + (j / 3) % 3 - 1, tz + j / 9 - 1, &local);
+            continue;
           }
+          for (int f = 0; f < 6; ++f)
+            if (faces[f]) activate(&G, state[m], tx + dtx[f], ty + dty[f], tz + dtz[f], &local);
         }
 #pragma omp critical
         {

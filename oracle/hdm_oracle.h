@@ -46,11 +46,26 @@ int oracle_naive(const void *gold, const void *segm, int dtype, int32_t label, i
                  int nz, double robustness, uint32_t *hist_fp, uint32_t *hist_fn,
                  int hist_len, uint16_t *map, oracle_result *out);
 
-/* connectivity: 6 (the reference's element).  threads <= 0: OpenMP default.
- * crop: restrict dilation to the bounding box of non-empty tiles. */
+/* threads <= 0: OpenMP default.  crop: restrict dilation to the bounding box
+ * of non-empty tiles (widened by one tile in x and y for the 18-neighbour
+ * element, whose shortest paths can leave the box by one voxel there). */
 int oracle_tiled(const void *gold, const void *segm, int dtype, int32_t label, int nx, int ny,
                  int nz, double robustness, int threads, int crop, uint32_t *hist_fp,
                  uint32_t *hist_fn, int hist_len, uint16_t *map, oracle_result *out);
+
+/* The same two with a structuring element: conn 6 (the reference's cross,
+ * hello.cu:373-376, 1310-1542; what the functions above use), 18 (the
+ * reference's second engine with CHEBYSHEV 1: the six faces, the four
+ * diagonals in the xy plane and the eight corners,
+ * forBench/HausdorffDistance.cu:37-60) or 26 (the full 3x3x3 cube).
+ * -1 for any other conn. */
+int oracle_naive_se(const void *gold, const void *segm, int dtype, int32_t label, int nx,
+                    int ny, int nz, double robustness, int conn, uint32_t *hist_fp,
+                    uint32_t *hist_fn, int hist_len, uint16_t *map, oracle_result *out);
+int oracle_tiled_se(const void *gold, const void *segm, int dtype, int32_t label, int nx,
+                    int ny, int nz, double robustness, int conn, int threads, int crop,
+                    uint32_t *hist_fp, uint32_t *hist_fn, int hist_len, uint16_t *map,
+                    oracle_result *out);
 
 /* Fill out (hd, sums, counts, avg) from the histograms and totals. */
 void oracle_finish(const uint32_t *hist_fp, const uint32_t *hist_fn, int hist_len,

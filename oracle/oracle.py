@@ -50,6 +50,12 @@ def lib():
         L.oracle_tiled.argtypes = args + [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
                                           C.c_void_p, C.POINTER(OracleResult)]
         L.oracle_tiled.restype = C.c_int
+        L.oracle_naive_se.argtypes = args + [C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                             C.c_void_p, C.POINTER(OracleResult)]
+        L.oracle_naive_se.restype = C.c_int
+        L.oracle_tiled_se.argtypes = args + [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.c_void_p, C.POINTER(OracleResult)]
+        L.oracle_tiled_se.restype = C.c_int
         L.oracle_max_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -71,9 +77,10 @@ def _prep(gold: np.ndarray, segm: np.ndarray):
 
 
 def run(gold: np.ndarray, segm: np.ndarray, label: int = 1, robustness: float = 1.0,
-        method: str = "tiled", threads: int = 0, crop: bool = True, want_map: bool = False):
-    """Runs the oracle on [z, y, x] volumes.  Returns (rc, result dict, hist_fp,
-    hist_fn, map or None)."""
+        method: str = "tiled", threads: int = 0, crop: bool = True, want_map: bool = False,
+        conn: int = 6):
+    """Runs the oracle on [z, y, x] volumes with structuring element conn (6,
+    18 or 26).  Returns (rc, result dict, hist_fp, hist_fn, map or None)."""
     g, s, dt, nx, ny, nz = _prep(gold, segm)
     L = nx + ny + nz + 2
     hf = np.zeros(L, np.uint32)
@@ -82,12 +89,13 @@ def run(gold: np.ndarray, segm: np.ndarray, label: int = 1, robustness: float = 
     res = OracleResult()
     mptr = mp.ctypes.data if mp is not None else None
     if method == "naive":
-        rc = lib().oracle_naive(g.ctypes.data, s.ctypes.data, dt, label, nx, ny, nz, robustness,
-                                hf.ctypes.data, hn.ctypes.data, L, mptr, C.byref(res))
+        rc = lib().oracle_naive_se(g.ctypes.data, s.ctypes.data, dt, label, nx, ny, nz,
+                                   robustness, conn, hf.ctypes.data, hn.ctypes.data, L, mptr,
+                                   C.byref(res))
     else:
-        rc = lib().oracle_tiled(g.ctypes.data, s.ctypes.data, dt, label, nx, ny, nz, robustness,
-                                threads, 1 if crop else 0, hf.ctypes.data, hn.ctypes.data, L,
-                                mptr, C.byref(res))
+        rc = lib().oracle_tiled_se(g.ctypes.data, s.ctypes.data, dt, label, nx, ny, nz,
+                                   robustness, conn, threads, 1 if crop else 0, hf.ctypes.data,
+                                   hn.ctypes.data, L, mptr, C.byref(res))
     return rc, res.as_dict(), hf, hn, mp
 
 

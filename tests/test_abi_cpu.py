@@ -88,6 +88,17 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     o.robustness = 1.5
     assert L.hdm_hausdorff_u8(p, p, 4, 4, 4, 1, C.byref(o), C.byref(r)) == -1
     assert L.hdm_batch_create(None, 0, None, 0, None) == -1
+    # structuring elements: 6, 18 or 26 (0 = 6); 18 and 26 on 64x8x8 tiles only
+    L.hdm_opts_init(C.byref(o))
+    assert o.connectivity == 0
+    for conn, tile in ((7, hdm.TILE_AUTO), (-6, hdm.TILE_AUTO), (18, hdm.TILE_64x4x4),
+                       (26, hdm.TILE_32x8x8), (26, hdm.TILE_64x8x4)):
+        o.connectivity, o.tile = conn, tile
+        assert L.hdm_hausdorff_u8(p, p, 4, 4, 4, 1, C.byref(o), C.byref(r)) == -1, (conn, tile)
+    so = hdm.ServerOpts()
+    L.hdm_server_opts_init(C.byref(so))
+    so.connectivity = 10
+    assert L.hdm_server_create(C.byref(so), C.byref(C.c_void_p())) == -1
 
 
 def test_cpp_compat_header_compiles_and_reports_errors(tmp_path):
