@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(HERE, "libhdm.so")
 HDM_OK = 0
 ERRORS = {-1: "HDM_E_INVALID", -2: "HDM_E_EMPTY_MASK", -3: "HDM_E_HIP", -4: "HDM_E_NOMEM",
           -5: "HDM_E_TIMEOUT", -6: "HDM_E_BUDGET", -7: "HDM_E_NODEVICE", -8: "HDM_E_STUCK",
-          -9: "HDM_E_CAPACITY"}
+          -9: "HDM_E_CAPACITY", -10: "HDM_E_IO", -11: "HDM_E_UNSUPPORTED", -12: "HDM_E_FORMAT",
+          -13: "HDM_E_NOTFOUND"}
 PTR_DEVICE, PTR_HOST = 0, 1
 DTYPE_U8, DTYPE_I32 = 0, 1
 TILE_AUTO, TILE_64x8x8, TILE_64x4x4, TILE_64x8x4, TILE_32x8x8 = 0, 1, 2, 3, 4
@@ -72,6 +73,20 @@ class ServerOpts(C.Structure):
                 ("tile", C.c_int32), ("grid_ctas", C.c_int32), ("robustness", C.c_double),
                 ("max_nx", C.c_int32), ("max_ny", C.c_int32), ("max_nz", C.c_int32),
                 ("connectivity", C.c_int32)]
+
+
+class H5Info(C.Structure):
+    _fields_ = [("rank", C.c_int32), ("type_class", C.c_int32), ("elem_size", C.c_int32),
+                ("layout", C.c_int32), ("dims", C.c_uint64 * 8), ("n_elems", C.c_uint64),
+                ("chunk", C.c_uint64 * 8), ("n_filters", C.c_int32), ("big_endian", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        r = self.rank
+        return {"shape": tuple(int(v) for v in self.dims[:r]),
+                "chunks": tuple(int(v) for v in self.chunk[:r]) if self.layout == 2 else None,
+                "type_class": ("int", "uint", "float", "enum")[self.type_class],
+                "elem_size": self.elem_size, "layout": ("compact", "contiguous", "chunked")[self.layout],
+                "n_filters": self.n_filters, "big_endian": bool(self.big_endian)}
 
 
 class Completion(C.Structure):
@@ -138,6 +153,10 @@ def lib() -> C.CDLL:
         "hdm_synth_spec_long": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32, i32,
                                  i32, i32, i32], i32),
         "hdm_synth_host": ([C.POINTER(SynthSpec), vp, vp], i32),
+        "hdm_h5_info_get": ([C.c_char_p, C.c_char_p, C.POINTER(H5Info)], i32),
+        "hdm_h5_read": ([C.c_char_p, C.c_char_p, i32, vp, u64, i32], i32),
+        "hdm_hausdorff_h5": ([C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, i32, i32, i32,
+                              C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_double)], i32),
         "hdm_synth_device": ([C.POINTER(SynthSpec), vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
@@ -309,6 +328,45 @@ def first_hit_map(gold, segm, label: int = 1, robustness: float = 1.0, tile: int
     del keep
     check(rc, "hdm_first_hit_map_u8")
     return res.as_dict(), mp
+
+
+# ---------------------------------------------------------------- HDF5
+H5_AS_U8, H5_AS_I32, H5_AS_NONZERO = 0, 1, 2
+
+
+def h5_info(path: str, dataset: str) -> dict:
+    """Shape, element type and layout of an HDF5 dataset (the library's own
+    reader; no libhdf5)."""
+    info = H5Info()
+    check(lib().hdm_h5_info_get(os.fsencode(path), dataset.encode(), C.byref(info)),
+          f"hdm_h5_info_get {path}:{dataset}")
+    return info.as_dict()
+
+
+def h5_read(path: str, dataset: str, conversion: int = H5_AS_U8, threads: int = 0) -> np.ndarray:
+    """The whole dataset as a numpy array of its shape: uint8 (H5_AS_U8,
+    values must fit), int32 (H5_AS_I32) or uint8 0/1 (H5_AS_NONZERO, the
+    reference's bool read, forBench/hdf5Manag.cu:76)."""
+    info = h5_info(path, dataset)
+    out = np.empty(info["shape"], np.int32 if conversion == H5_AS_I32 else np.uint8)
+    check(lib().hdm_h5_read(os.fsencode(path), dataset.encode(), conversion, out.ctypes.data,
+                            out.size, threads), f"hdm_h5_read {path}:{dataset}")
+    return out
+
+
+def hausdorff_h5(gold_path: str, gold_dataset: str, segm_path: str, segm_dataset: str,
+                 nx: int, ny: int, nz: int, robustness: float = 1.0, connectivity: int = 6):
+    """The reference's loadHDF (forBench/hdf5Manag.cu:196-220): two datasets
+    of nx*ny*nz booleans (x fastest) -> result dict with 'load_ms'."""
+    o = make_opts(robustness, ptr_kind=PTR_HOST, connectivity=connectivity)
+    res = Result()
+    ms = C.c_double()
+    check(lib().hdm_hausdorff_h5(os.fsencode(gold_path), gold_dataset.encode(),
+                                 os.fsencode(segm_path), segm_dataset.encode(), nx, ny, nz,
+                                 C.byref(o), C.byref(res), C.byref(ms)), "hdm_hausdorff_h5")
+    d = res.as_dict()
+    d["load_ms"] = ms.value
+    return d
 
 
 def getHausdorffDistance_CUDA_Generic(goldStandard, algoOutput, WIDTH: int, HEIGHT: int,
