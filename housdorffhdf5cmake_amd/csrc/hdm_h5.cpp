@@ -11,6 +11,7 @@
 // (III.A.1), symbol table nodes (III.B), local heaps (III.D), fixed-array
 // chunk index (III.H... as written by HDF5 1.10 for layout version 4).
 #include <fcntl.h>
+#include <hip/hip_runtime.h>
 #include <sys/stat.h>
 #include <unistd.h>
 #include <zlib.h>
@@ -20,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1065,16 +1067,46 @@ extern "C" int hdm_hausdorff_h5(const char *gold_path, const char *gold_dataset,
                                 double *load_ms) {
   if (!out || nx < 1 || ny < 1 || nz < 1) return HDM_E_INVALID;
   const uint64_t n = (uint64_t)nx * ny * nz;
-  std::vector<uint8_t> g, s;
-  try {
-    g.resize(n);
-    s.resize(n);
-  } catch (...) {
-    return HDM_E_NOMEM;
+  // both volumes are staged in pinned host memory, kept (grow-only) across
+  // calls, so they reach the device by DMA; calls serialize on it
+  // (pageable memory when pinning fails); both datasets are checked first
+  hdm_h5_info info;
+  for (int k = 0; k < 2; ++k) {
+    const int rc = hdm_h5_info_get(k ? segm_path : gold_path, k ? segm_dataset : gold_dataset,
+                                   &info);
+    if (rc != HDM_OK) return rc;
+    if (info.n_elems != n) return HDM_E_INVALID;
   }
+  static std::mutex mu;
+  static uint8_t *pinned = nullptr;
+  static uint64_t cap = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  std::vector<uint8_t> pageable;
+  uint8_t *g = nullptr;
+  if (cap < 2 * n) {
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr, cap = 0;
+    if (hipHostMalloc((void **)&pinned, 2 * n, hipHostMallocDefault) == hipSuccess) {
+      cap = 2 * n;
+    } else {
+      (void)hipGetLastError();
+      pinned = nullptr;
+    }
+  }
+  if (pinned) {
+    g = pinned;
+  } else {
+    try {
+      pageable.resize(2 * n);
+    } catch (...) {
+      return HDM_E_NOMEM;
+    }
+    g = pageable.data();
+  }
+  uint8_t *s = g + n;
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = hdm_h5_read(gold_path, gold_dataset, HDM_H5_AS_NONZERO, g.data(), n, 0);
-  if (rc == HDM_OK) rc = hdm_h5_read(segm_path, segm_dataset, HDM_H5_AS_NONZERO, s.data(), n, 0);
+  int rc = hdm_h5_read(gold_path, gold_dataset, HDM_H5_AS_NONZERO, g, n, 0);
+  if (rc == HDM_OK) rc = hdm_h5_read(segm_path, segm_dataset, HDM_H5_AS_NONZERO, s, n, 0);
   if (load_ms)
     *load_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1084,5 +1116,5 @@ extern "C" int hdm_hausdorff_h5(const char *gold_path, const char *gold_dataset,
   if (opts_in) o = *opts_in;
   o.ptr_kind = HDM_PTR_HOST;
   o.stream = nullptr;
-  return hdm_hausdorff_u8(g.data(), s.data(), nx, ny, nz, 1, &o, out);
+  return hdm_hausdorff_u8(g, s, nx, ny, nz, 1, &o, out);
 }

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/hdm.h"
+#include "../../include/hdm_h5.h"
 #include "hdm_host.h"
 
 using namespace hdm;
@@ -284,6 +285,10 @@ extern "C" const char *hdm_strerror(int code) {
     case HDM_E_NODEVICE: return "no HIP device";
     case HDM_E_STUCK: return "work queue drained with targets uncovered";
     case HDM_E_CAPACITY: return "capacity exceeded";
+    case HDM_E_IO: return "file cannot be opened or read";
+    case HDM_E_UNSUPPORTED: return "HDF5 feature not supported by the reader";
+    case HDM_E_FORMAT: return "not an HDF5 file, or a corrupt structure";
+    case HDM_E_NOTFOUND: return "no dataset at that path";
     default: return "unknown error";
   }
 }

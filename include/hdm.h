@@ -3,8 +3,10 @@
  *
  * Two 3-D masks plus their dims go in; the exact Hausdorff distance (HD) and
  * the exact average HD (64-bit integer sum and count, divided in FP64 on the
- * host) come out.  6-connectivity morphological dilation, iterated until each
- * mask covers the other, as in the reference engine.
+ * host) come out.  Morphological dilation, iterated until each mask covers
+ * the other, as in the reference engine: the 6-neighbour cross by default,
+ * or the 18- or 26-neighbour element (hdm_opts.connectivity).  The HDF5
+ * loader is declared in hdm_h5.h.
  *
  * Conventions (all entry points):
  *   - volumes are dense, x fastest: index = x + nx*(y + ny*z)
@@ -152,9 +154,11 @@ int hdm_hausdorff_i32(const int32_t *gold, const int32_t *segm, int nx, int ny, 
 /* Replaces: int HausdorffDistance::computeDistance(Volume* img1, Volume* img2)
  * (forBench/HausdorffDistance.cuh:17, .cu:101-187) for bool volumes
  * (forBench/Volume.h:10).  A voxel is on iff its byte is 1 (bool storage
- * holds 0 or 1; the reference allocates the volumes with calloc).  Uses
- * 6-connectivity (the reference's CHEBYSHEV 1 18-neighbour element is a
- * different metric). */
+ * holds 0 or 1; the reference allocates the volumes with calloc).  With
+ * opts->connectivity = 18 the dilation uses that engine's structuring element
+ * (CHEBYSHEV 1, forBench/HausdorffDistance.cuh:8, .cu:37-60) and the result's
+ * hd is its computeDistance value; the default is the 6-neighbour cross of
+ * the reference's main engine. */
 int hdm_hausdorff_bool(const uint8_t *img1, const uint8_t *img2, int nx, int ny, int nz,
                        const hdm_opts *opts, hdm_result *out);
 

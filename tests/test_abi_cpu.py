@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = set()
-    for h in ("hdm.h", "hdm_synth.h"):
+    for h in ("hdm.h", "hdm_synth.h", "hdm_h5.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(hdm_[a-z0-9_]+)\s*\(", text):
@@ -28,7 +28,7 @@ def declared_symbols():
 def test_library_exports_every_declared_symbol():
     L = hdm.lib()
     syms = declared_symbols()
-    assert len(syms) >= 28
+    assert len(syms) >= 31 and "hdm_hausdorff_h5" in syms
     for s in syms:
         assert hasattr(L, s), s
     assert L.hdm_abi_version() == 1
