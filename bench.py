@@ -57,8 +57,9 @@ def leg_single(hdm, torch, steps, warmup):
     """Timed steps cycle over COPIES device copies of the pair, so no step
     reads its input from the Infinity Cache where the previous step left it,
     and bracket the dominant launch with HIP events (timing 3: the fused
-    launch, or the dilation kernel of a separate-kernel call); the phases are
-    timed by as many untimed calls with timing 1."""
+    launch, or the dilation kernel of a separate-kernel call; the events are
+    read after the timed region); the phases are timed by as many extra calls
+    with timing 1."""
     G, S = ct_pair(hdm)
     pairs = [(G, S)] + [(G.clone(), S.clone()) for _ in range(COPIES - 1)]
     torch.cuda.synchronize()
@@ -66,16 +67,15 @@ def leg_single(hdm, torch, steps, warmup):
     for i in range(warmup):
         res = hdm.hausdorff(*pairs[i % COPIES], timing=3)
     torch.cuda.synchronize()
-    dom = []
-    fused = False
+    hdm.timing_log()
     t0 = time.perf_counter()
     for i in range(steps):
         res = hdm.hausdorff(*pairs[i % COPIES], timing=3)
-        t = hdm.last_timings()
-        fused = t["fused"]
-        dom.append(t["total"] if fused else t["dilate"])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    dom = hdm.timing_log()  # the events of the timed steps, read after them
+    fused = hdm.last_timings()["fused"]
+    assert len(dom) == steps and min(dom) > 0, dom
     phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
     for i in range(steps):
         hdm.hausdorff(*pairs[i % COPIES], timing=True)

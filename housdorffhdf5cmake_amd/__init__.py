@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
         "hdm_opts_init": ([C.POINTER(Opts)], None), "hdm_device_count": ([], i32),
         "hdm_workspace_bytes": ([i32, i32, i32, i32], u64),
         "hdm_last_timings": ([C.POINTER(C.c_float), i32], i32),
+        "hdm_timing_log": ([C.POINTER(C.c_float), i32], i32),
         "hdm_last_trace": ([C.POINTER(C.c_uint64), i32, C.POINTER(C.c_double)], i32),
         "hdm_hausdorff_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts),
                               C.POINTER(Result)], i32),
@@ -256,6 +257,14 @@ def last_timings() -> dict:
     out = {names[i]: float(ms[i]) for i in range(min(n, 5))}
     out["fused"] = n > 5 and ms[5] == 1.0
     return out
+
+
+def timing_log(max_calls: int = 1024) -> list:
+    """Device ms of the dominant launch of every call made with timing=3
+    since the last read, oldest first (waits for their events)."""
+    ms = (C.c_float * max_calls)()
+    n = lib().hdm_timing_log(ms, max_calls)
+    return [float(ms[i]) for i in range(n)]
 
 
 def last_trace(max_iters: int = 4096):

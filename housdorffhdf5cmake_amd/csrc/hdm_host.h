@@ -40,6 +40,25 @@ struct Ctx {
   std::vector<uint64_t> last_trace;  // 8 words per iteration of a traced call
   DevMem trace;
   std::mutex mu;
+  // A call that took its result from the kernel's ready marker returns while
+  // the launch may still be leaving: its stream, waited for before the next
+  // call uses another stream or reallocates a buffer (settle)
+  uint32_t seq = 0;
+  // timing 3: one event pair per call from a ring; the durations are read
+  // when asked (hdm_last_timings, hdm_timing_log), so the call need not wait
+  static constexpr uint32_t kRing = 1024;
+  std::vector<hipEvent_t> ring;  // 2 * kRing, created on first use
+  uint32_t ring_next = 0;
+  std::vector<uint32_t> log;     // pairs recorded since the log was read, oldest first
+  int last_pair = -1;            // pair of the last timing-3 call, not yet read
+  bool last_pair_fused = false;
+  hipStream_t pending = nullptr;
+  bool has_pending = false;
+  int settle() {
+    if (!has_pending) return HDM_OK;
+    has_pending = false;
+    return hipStreamSynchronize(pending) == hipSuccess ? HDM_OK : HDM_E_HIP;
+  }
 };
 
 // launch shapes and optional phase events of one enqueue
@@ -52,6 +71,8 @@ struct LaunchCfg {
   uint64_t *trace = nullptr;       // per-iteration device trace or null
   PairOut *outs = nullptr;         // where the results go (null: the group's outs)
   EngineHead *head = nullptr;      // where the engine head goes (null: nowhere)
+  uint32_t *ready = nullptr;       // the dilation's ready marker (host) or null
+  uint32_t ready_seq = 0;
 };
 
 struct GroupPtrs {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -233,6 +234,8 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   a.trace = cfg.trace;
   a.outs = cfg.outs ? cfg.outs : P.outs;
   a.head = cfg.head;
+  a.ready = cfg.ready;
+  a.ready_seq = cfg.ready_seq;
   e = tl->dilate(dim3(cfg.grid), s, a);
   if (e != hipSuccess) return e;
   return mark(3);
@@ -319,6 +322,28 @@ extern "C" uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile) {
   return L.total + g.total;
 }
 
+// Waits until the kernel's ready marker holds seq; the stream is queried
+// every 200 us so a launch that fails or ends without the marker is reported.
+static int wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq) {
+  using clk = std::chrono::steady_clock;
+  auto next = clk::now() + std::chrono::microseconds(200);
+  for (uint32_t k = 0;; ++k) {
+    if (*ready == seq) break;
+    if ((k & 63) == 63 && clk::now() >= next) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        if (*ready == seq) break;
+        return HDM_E_HIP;
+      }
+      if (q != hipErrorNotReady) return map_hip(q);
+      next = clk::now() + std::chrono::microseconds(200);
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return HDM_OK;
+}
+
 // ---------------------------------------------------------------- single pair
 static int hausdorff_core(const void *gold, const void *segm, int dtype, int label, int nx,
                           int ny, int nz, const hdm_opts *opts_in, uint16_t *map_out,
@@ -342,6 +367,9 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   // NULL = the legacy default stream, so the call is ordered after work the
   // caller queued on it or on any blocking stream (torch's default stream)
   hipStream_t s = (hipStream_t)opts.stream;
+  // the previous call's launch may still be leaving on its stream: wait for
+  // it before work goes to another stream, or a buffer it uses is replaced
+  if (c->has_pending && c->pending != s && (rc = c->settle()) != HDM_OK) return rc;
   const TileLaunch *tl = hdm_tile_launch(tile);
   const TileShape ts = tile_shape(tile);
   const PairGeom g = pair_geom(nx, ny, nz, ts);
@@ -350,6 +378,11 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const CommonLayout L = common_layout(1, qcap, g.hist_len);
   const uint64_t nvox = (uint64_t)nx * ny * nz;
   const size_t esz = dtype == HDM_DTYPE_U8 ? 1 : 4;
+  const bool grow = L.total + g.total > c->ws.cap ||
+                    (opts.ptr_kind == HDM_PTR_HOST && 2 * align256(nvox * esz) > c->in.cap) ||
+                    (map_out && opts.ptr_kind == HDM_PTR_HOST && nvox * 2 > c->map.cap) ||
+                    (opts.timing == 2 && (size_t)g.hist_len * 64 > c->trace.cap);
+  if (grow && (rc = c->settle()) != HDM_OK) return rc;
   if ((rc = c->ws.ensure(L.total + g.total)) != HDM_OK) return rc;
   char *base = (char *)c->ws.p;
   GroupPtrs P = carve_common(base, L);
@@ -376,7 +409,26 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   }
   const PackIndex pi{0, 0};
   LaunchCfg cfg = launch_cfg(c, tile, g.row_words, g.ntiles, opts.grid_ctas, opts.chunk);
-  if (opts.timing) cfg.ev = c->ev, cfg.ev_mode = opts.timing;
+  hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int pair = -1;
+  if (opts.timing == 3) {  // a fresh pair of the ring brackets the dominant launch
+    if (c->ring.empty()) {
+      c->ring.assign(2 * Ctx::kRing, nullptr);
+      for (auto &ev : c->ring)
+        if (hipEventCreate(&ev) != hipSuccess) {
+          (void)hipGetLastError();
+          for (auto &x : c->ring)
+            if (x) (void)hipEventDestroy(x);
+          c->ring.clear();
+          return HDM_E_HIP;
+        }
+    }
+    pair = (int)(c->ring_next++ % Ctx::kRing);
+    ring_ev[2] = c->ring[2 * pair], ring_ev[3] = c->ring[2 * pair + 1];
+    cfg.ev = ring_ev, cfg.ev_mode = 3;
+  } else if (opts.timing) {
+    cfg.ev = c->ev, cfg.ev_mode = opts.timing;
+  }
   const size_t trace_bytes = (size_t)g.hist_len * 8 * sizeof(uint64_t);
   if (opts.timing == 2) {  // per-iteration device trace (diagnostic, not graph-safe)
     if ((rc = c->trace.ensure(trace_bytes)) != HDM_OK) return rc;
@@ -391,6 +443,14 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   hout->status = 1;
   cfg.outs = hout;
   cfg.head = hhead;
+  // the result can be taken from the ready marker unless the call also
+  // copies a map or histograms out, or times itself with events
+  volatile uint32_t *ready = (volatile uint32_t *)(ho + 1024);
+  const bool by_marker = !(map_out && opts.ptr_kind == HDM_PTR_HOST) &&
+                         !(opts.hist_out && opts.hist_len > 0) &&
+                         (opts.timing == 0 || opts.timing == 3);
+  if (++c->seq == 0) ++c->seq;
+  if (by_marker) cfg.ready = (uint32_t *)ready, cfg.ready_seq = c->seq;
   hipError_t e = hipSuccess;
   const bool fused = fused_enabled();
   const uint32_t ebase = c->ebase;
@@ -423,9 +483,11 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.desc = d;
     a.total_tiles = g.ntiles;
     a.ebase = c->ebase;
-    if (cfg.ev) e = hipEventRecord(c->ev[0], s);
+    a.ready = cfg.ready;
+    a.ready_seq = cfg.ready_seq;
+    if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) e = tl->fused(dim3(fused_grid(c, tile, opts.grid_ctas)), s, a);
-    if (e == hipSuccess && cfg.ev) e = hipEventRecord(c->ev[3], s);
+    if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
     // until the result is read, assume the launch used every epoch it could
     c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;
   } else {
@@ -443,7 +505,13 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
                        s) != hipSuccess)
       return HDM_E_HIP;
   }
-  if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip(e);
+  if (by_marker) {
+    if ((rc = wait_ready(s, ready, c->seq)) != HDM_OK) return rc;
+    c->pending = s, c->has_pending = true;
+  } else {
+    c->has_pending = false;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return map_hip(e);
+  }
   PairOut o;
   EngineHead es{};
   std::memcpy(&o, hout, sizeof(o));
@@ -452,14 +520,11 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   if (fused) {
     c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used
     c->last_ms[5] = 1.f;
-    if (opts.timing) {
+    if (opts.timing && opts.timing != 3) {
       float all = -1.f;
       if (hipEventElapsedTime(&all, c->ev[0], c->ev[3]) != hipSuccess) all = -1.f;
       (void)hipGetLastError();
-      if (opts.timing == 3) {  // the launch as a whole
-        c->last_ms[0] = c->last_ms[1] = c->last_ms[2] = c->last_ms[3] = -1.f;
-        c->last_ms[4] = all;
-      } else {  // phases from the launch's own clock stamps
+      {  // phases from the launch's own clock stamps
         const double tick_ms = 1.0 / c->clock_khz;
         c->last_ms[0] = (float)((es.phase[1] - es.phase[0]) * tick_ms);
         c->last_ms[1] = (float)((es.phase[2] - es.phase[1]) * tick_ms);
@@ -468,13 +533,8 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
         c->last_ms[4] = all;
       }
     }
-  } else if (opts.timing == 3) {  // the dilation only
-    float ms = -1.f;
-    if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) != hipSuccess) ms = -1.f;
-    c->last_ms[0] = c->last_ms[1] = c->last_ms[3] = c->last_ms[4] = -1.f;
-    c->last_ms[2] = ms;
+  } else if (opts.timing == 3) {
     c->last_ms[5] = 0.f;
-    (void)hipGetLastError();
   } else if (opts.timing) {
     // phases: pack (+ reset), metadata, dilation (+ finalize); total
     float ms[3] = {0, 0, 0};
@@ -487,6 +547,13 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     c->last_ms[4] = all;
     c->last_ms[5] = 0.f;
     (void)hipGetLastError();
+  }
+  if (opts.timing == 3) {  // read when asked: the dilation, or the whole fused launch
+    for (int i = 0; i < 5; ++i) c->last_ms[i] = -1.f;
+    c->last_pair = pair;
+    c->last_pair_fused = fused;
+    if (c->log.size() >= Ctx::kRing) c->log.erase(c->log.begin());
+    c->log.push_back((uint32_t)pair);
   }
   if (opts.timing == 2) {
     const size_t nrec = std::min<size_t>(es.t, (size_t)g.hist_len);
@@ -793,7 +860,35 @@ extern "C" int hdm_last_timings(float *ms, int n) {
   Ctx *c = nullptr;
   if (get_ctx(&c) != HDM_OK) return 0;
   std::lock_guard<std::mutex> lk(c->mu);
+  if (c->last_pair >= 0) {  // a timing-3 call: wait for its pair now
+    float t = -1.f;
+    hipEvent_t *p = &c->ring[2 * c->last_pair];
+    if (hipEventSynchronize(p[1]) != hipSuccess || hipEventElapsedTime(&t, p[0], p[1]) != hipSuccess)
+      t = -1.f;
+    (void)hipGetLastError();
+    c->last_ms[c->last_pair_fused ? 4 : 2] = t;
+    c->last_pair = -1;
+  }
   const int k = std::min(n, 6);
   for (int i = 0; i < k; ++i) ms[i] = c->last_ms[i];
+  return k;
+}
+
+extern "C" int hdm_timing_log(float *ms, int n) {
+  if (!ms || n < 1) return 0;
+  Ctx *c = nullptr;
+  if (get_ctx(&c) != HDM_OK) return 0;
+  std::lock_guard<std::mutex> lk(c->mu);
+  int k = 0;
+  for (uint32_t pr : c->log) {
+    if (k >= n) break;
+    float t = -1.f;
+    hipEvent_t *p = &c->ring[2 * pr];
+    if (hipEventSynchronize(p[1]) != hipSuccess || hipEventElapsedTime(&t, p[0], p[1]) != hipSuccess)
+      t = -1.f;
+    (void)hipGetLastError();
+    ms[k++] = t;
+  }
+  c->log.clear();
   return k;
 }

@@ -1996,6 +1996,10 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         h.phase[3] = FUSED ? wall_clock64() : 0ull;
         *a.head = h;
       }
+      if (a.ready) {  // every store above reaches the host before the marker
+        __threadfence_system();
+        __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
 }

@@ -41,7 +41,11 @@ struct DilateArgs {
   PairDesc desc;
   uint64_t total_tiles;
   uint32_t ebase;
-  uint32_t pad;
+  // when the launch leaves, workgroup 0 stores ready_seq to *ready (host
+  // memory) after the results and the head, behind a system-scope fence, so
+  // the host can take the results without waiting for the stream (or null)
+  uint32_t ready_seq;
+  uint32_t *ready;
 };
 
 struct TileLaunch {

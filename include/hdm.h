@@ -20,8 +20,13 @@
  *     the process and never resets the device).
  *   - device pointers are plain HIP device addresses; streams are hipStream_t
  *     passed as void*.  NULL stream = the legacy default stream (ordered with
- *     every blocking stream); the single-pair calls return only after their
- *     work on that stream is done.
+ *     every blocking stream).  A single-pair call returns once its result is
+ *     in host memory (the kernel stores it, then a ready marker): its launch
+ *     may still be leaving, so work queued after it on the same stream runs
+ *     after it, and the library itself waits for it before using another
+ *     stream or replacing a buffer.  The inputs are no longer read by then.
+ *     Calls that copy a map or histograms to the host, or time every phase,
+ *     return after their stream is done.
  *
  * Differences from the reference that are deliberate (its quirks are not
  * reproduced; results follow the mathematical definitions above):
@@ -125,10 +130,16 @@ uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
  * in one persistent kernel, phases then timed by the kernel's own wall clock
  * between its grid barriers) and 0 when it ran as separate kernels (phases
  * timed by HIP events on the stream).  With opts->timing = 3 only the
- * dominant launch is bracketed by events, the others are -1: ms[2] the
- * dilation kernel of a separate-kernel call, ms[4] the whole fused launch.
+ * dominant launch is bracketed by a pair of HIP events, the others are -1:
+ * ms[2] the dilation kernel of a separate-kernel call, ms[4] the whole fused
+ * launch; such a call does not wait for its events, this function does.
  * Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
+/* Device milliseconds of the dominant launch of each single-pair call made
+ * with opts->timing = 3 on the current device since the log was last read
+ * (at most the last 1024), oldest first; waits for their events, then clears
+ * the log.  Returns how many were written (at most n). */
+int hdm_timing_log(float *ms, int n);
 /* Per-iteration device trace of the last single-pair call on the current
  * device that ran with opts->timing = 2 (observability; no counterpart in the
  * reference, which has only printf): 8 words per iteration, wall-clock ticks

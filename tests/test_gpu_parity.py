@@ -174,3 +174,33 @@ def test_repeat_calls_are_identical():
         b = hdm.hausdorff(G, S)
         for k in ("hd", "sum", "count", "sum_fp", "sum_fn", "n_fp", "n_fn"):
             assert a[k] == b[k]
+
+
+def test_calls_across_streams_and_workspace_growth():
+    """A call returns as soon as the kernel has stored its result; the next
+    call on another stream, or one that grows the workspace, waits for the
+    previous launch first.  Inputs may be overwritten once a call returns."""
+    small = hdm.synth_spec("ellipsoids", 96, 80, 64, 6, 4, 20, 2, 0.005, 5)
+    big = hdm.synth_spec("ellipsoids", 256, 256, 160, 8, 10, 40, 2, 0.005, 6)
+    gs, ss = hdm.synth_device(small)
+    gb, sb = hdm.synth_device(big)
+    torch.cuda.synchronize()
+    want_s = oracle_expect(gs.cpu().numpy(), ss.cpu().numpy())[0]
+    want_b = oracle_expect(gb.cpu().numpy(), sb.cpu().numpy())[0]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.default_stream()]
+    for i in range(9):
+        st = streams[i % 3]
+        st.wait_stream(torch.cuda.default_stream())
+        with torch.cuda.stream(st):
+            if i % 4 == 3:
+                check_result(hdm.hausdorff(gb, sb), want_b, f"big/{i}")
+            else:
+                check_result(hdm.hausdorff(gs, ss), want_s, f"small/{i}")
+    # overwrite the inputs right after a call returns, then use them again
+    g2, s2 = gs.clone(), ss.clone()
+    check_result(hdm.hausdorff(g2, s2), want_s, "before overwrite")
+    g2.zero_()
+    s2.zero_()
+    r = hdm.hausdorff(g2, s2)
+    assert r["hd"] == 0 and r["count"] == 0
+    check_result(hdm.hausdorff(gs, ss), want_s, "after overwrite")
