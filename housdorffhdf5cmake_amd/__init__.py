@@ -188,29 +188,30 @@ def _current_stream_ptr():
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_TORCH_DTYPES = None
+
+
 def _volume_args(gold, segm):
     """(gold_ptr, segm_ptr, nx, ny, nz, dtype, ptr_kind, keepalive)."""
-    if gold.shape != segm.shape or len(gold.shape) != 3:
+    global _TORCH_DTYPES
+    shape = gold.shape
+    if shape != segm.shape or len(shape) != 3:
         raise ValueError(f"need two equal 3-D volumes, got {tuple(gold.shape)} {tuple(segm.shape)}")
-    nz, ny, nx = (int(v) for v in gold.shape)
+    nz, ny, nx = (int(v) for v in shape)
     if _is_torch(gold):
-        import torch
         if not (gold.is_cuda and segm.is_cuda):
             gold, segm = gold.numpy(), segm.numpy()
         else:
-            if gold.dtype != segm.dtype:
-                raise ValueError("dtype mismatch")
-            if gold.dtype in (torch.uint8, torch.bool):
-                dt = DTYPE_U8
-                if gold.dtype == torch.bool:
-                    gold, segm = gold.view(torch.uint8), segm.view(torch.uint8)
-            elif gold.dtype == torch.int32:
-                dt = DTYPE_I32
-            else:
-                raise TypeError(f"unsupported dtype {gold.dtype}")
-            gold, segm = gold.contiguous(), segm.contiguous()
-            return (C.c_void_p(gold.data_ptr()), C.c_void_p(segm.data_ptr()), nx, ny, nz, dt,
-                    PTR_DEVICE, (gold, segm))
+            if _TORCH_DTYPES is None:
+                import torch
+                _TORCH_DTYPES = {torch.uint8: DTYPE_U8, torch.bool: DTYPE_U8,
+                                 torch.int32: DTYPE_I32}
+            dt = _TORCH_DTYPES.get(gold.dtype)
+            if dt is None or segm.dtype != gold.dtype:
+                raise TypeError(f"unsupported or mismatched dtypes {gold.dtype} {segm.dtype}")
+            if not (gold.is_contiguous() and segm.is_contiguous()):
+                gold, segm = gold.contiguous(), segm.contiguous()
+            return gold.data_ptr(), segm.data_ptr(), nx, ny, nz, dt, PTR_DEVICE, (gold, segm)
     g = np.ascontiguousarray(gold)
     s = np.ascontiguousarray(segm)
     if g.dtype == np.bool_:
