@@ -69,9 +69,10 @@ struct PairDesc {
   uint64_t *xf_orig[2];
   uint64_t *xf_buf[2][2];
   uint32_t *state[2];
-  // per tile, low byte gold / high byte segm: bit 0 non-empty, 1 full, 2..7
-  // voxels on the -x, +x, -y, +y, -z, +z face (written by the pack pass)
-  uint16_t *tflags;
+  // per tile, low 16 bits gold / high 16 bits segm: bit 0 non-empty, 1 full,
+  // 2..7 voxels on the -x, +x, -y, +y, -z, +z face, 8..13 voxels in the two
+  // outermost layers toward -x .. +z (written by the pack pass)
+  uint32_t *tflags;
   uint32_t *hist[2];
   uint16_t *map;
 };
@@ -101,7 +102,7 @@ struct PairOut {
   unsigned long long sum_fp, cnt_fp, sum_fn, cnt_fn;
   unsigned long long n_gold, n_segm, n_fp, n_fn;
   int32_t done;  // both passes finished at the engine's current iteration
-  int32_t pad;
+  int32_t iters; // dilation iterations: the last one a pass needed, or all run
 };
 
 // The engine fields the host reads after a launch; the dilation kernel writes
@@ -141,10 +142,11 @@ struct Engine {
   // Single-pair barrier: every word carries its epoch in the top 24 bits
   // (tag()), so a reader that sees the right tag has the value that goes with
   // it and no ordering between words is needed.  arr[wg]: this iteration's
-  // hits of pass 0 and 1 and the entries the workgroup pushed; rel2[copy]:
+  // hits of pass 0 and 1, the entries the workgroup pushed and kept, and (two
+  // dilation steps per barrier) the second step's hits of pass 0 and 1; rel2[copy]:
   // word 0 flags (bit m: pass m done, bit 2: error), words 1..kSubQ+1 prefix
   // offsets of the next queue.
-  unsigned long long arr[kMaxGrid][4];
+  unsigned long long arr[kMaxGrid][8];
   unsigned long long rel2[kReleaseCopies][kLineWords];
   // tag(epoch, first global entry) of each workgroup's free wave slots: the
   // master's prefix over workgroups of (slots - entries the workgroup kept)

@@ -28,6 +28,7 @@ struct Ctx {
   int occ[kLaunchIds] = {};       // resident dilate workgroups per CU, per launch id
   int occ_pack[kLaunchIds] = {};  // resident pack workgroups per CU
   int occ_fused[kLaunchIds] = {};  // resident fused single-pair workgroups per CU
+  int occ_fused_ds[kLaunchIds] = {};  // the same, two dilation steps per barrier
   double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
@@ -95,7 +96,8 @@ int norm_tile(int tile);
 int select_launch(int tile_opt, int conn_opt, int *conn);
 int barrier_mode();
 bool fused_enabled();
-int fused_grid(const Ctx *c, int tile, int requested);
+bool ds_enabled();
+int fused_grid(const Ctx *c, int tile, int requested, bool ds = false);
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,

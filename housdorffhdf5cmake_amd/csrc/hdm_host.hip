@@ -109,6 +109,12 @@ int get_ctx(Ctx **out) {
          nb < 1))
       rc = HDM_E_HIP;
     c->occ_fused[k] = nb;
+    const void *ds_fn = hdm_tile_launch(k)->fused_ds_fn;
+    if (rc == HDM_OK && ds_fn &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ds_fn, kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ_fused_ds[k] = ds_fn ? nb : 0;
   }
   // the dilation kernel stores the result and the engine head here directly:
   // coherent (fine-grained) host memory, so the stores are seen by the host
@@ -151,8 +157,19 @@ bool fused_enabled() {
   return on;
 }
 
-int fused_grid(const Ctx *c, int tile, int requested) {
-  const int maxres = std::min(std::max(1, c->occ_fused[tile]) * c->cus, kMaxGrid);
+// The fused single-pair launch runs two dilation steps per grid barrier
+// (64x8x8 tiles, 6 element) unless HDM_DS=0 asks for one.
+bool ds_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("HDM_DS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
+
+int fused_grid(const Ctx *c, int tile, int requested, bool ds) {
+  const int occ = ds ? c->occ_fused_ds[tile] : c->occ_fused[tile];
+  const int maxres = std::min(std::max(1, occ) * c->cus, kMaxGrid);
   return requested > 0 ? std::min(requested, maxres) : maxres;
 }
 
@@ -485,8 +502,10 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.ebase = c->ebase;
     a.ready = cfg.ready;
     a.ready_seq = cfg.ready_seq;
+    const bool ds = tl->fused_ds && conn == 6 && ds_enabled();
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
-    if (e == hipSuccess) e = tl->fused(dim3(fused_grid(c, tile, opts.grid_ctas)), s, a);
+    if (e == hipSuccess)
+      e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds)), s, a);
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
     // until the result is read, assume the launch used every epoch it could
     c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;
@@ -562,7 +581,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
                           hipMemcpyDeviceToHost) != hipSuccess)
       return HDM_E_HIP;
   }
-  fill_result(out, o, &es, g.ntiles, (int)es.t);
+  fill_result(out, o, &es, g.ntiles, o.iters);
   if (o.status == HDM_OK && !o.done) return HDM_E_STUCK;  // queue drained first
   return o.status;
 }

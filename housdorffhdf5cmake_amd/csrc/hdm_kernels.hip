@@ -324,6 +324,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
   constexpr int U = 2;            // wave steps whose loads are all issued first
+                                  // (the 128-byte path below pairs steps 0 and 1)
   const int lane = threadIdx.x & 63;
   const int sub = lane / R, r = lane % R;
   const int yl = r % TY, zl = r / TY;
@@ -451,16 +452,30 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         const bool fyh = group_bits<R>(__ballot(yl == TY - 1 && w != 0), lane) != 0;
         const bool fzl = group_bits<R>(__ballot(zl == 0 && w != 0), lane) != 0;
         const bool fzh = group_bits<R>(__ballot(zl == TZ - 1 && w != 0), lane) != 0;
+        // the two outermost layers toward each face (reach within two steps;
+        // read only by the two-steps-per-barrier seed, which runs on 64x8x8)
+        constexpr bool kDeep = TXB == 64 && TY == 8 && TZ == 8;
+        bool gxl = false, gxh = false, gyl = false, gyh = false, gzl = false, gzh = false;
+        if constexpr (kDeep) {
+          gxl = group_bits<R>(__ballot(rowok && (w & 3)), lane) != 0;
+          gxh = group_bits<R>(__ballot(rowok && ((w >> (TXB - 2)) & 3)), lane) != 0;
+          gyl = group_bits<R>(__ballot(yl <= 1 && w != 0), lane) != 0;
+          gyh = group_bits<R>(__ballot(yl >= TY - 2 && w != 0), lane) != 0;
+          gzl = group_bits<R>(__ballot(zl <= 1 && w != 0), lane) != 0;
+          gzh = group_bits<R>(__ballot(zl >= TZ - 2 && w != 0), lane) != 0;
+        }
         if (in && r == 0) {
           put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile, fxl);
           put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile + 1, fxh);
         }
         flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)(fxl != 0) << 2 |
                   (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
-                  (uint32_t)fzl << 6 | (uint32_t)fzh << 7)
-                 << (8 * m);
+                  (uint32_t)fzl << 6 | (uint32_t)fzh << 7 | (uint32_t)gxl << 8 |
+                  (uint32_t)gxh << 9 | (uint32_t)gyl << 10 | (uint32_t)gyh << 11 |
+                  (uint32_t)gzl << 12 | (uint32_t)gzh << 13)
+                 << (16 * m);
       }
-      if (in && r == 0) put<COH>(d.tflags + tile, (uint16_t)flags);
+      if (in && r == 0) put<COH>(d.tflags + tile, flags);
       if (!COH && d.map && rowok) {
         uint16_t *mp = d.map + vbU[u];
         for (int k = 0; k < cntU[u]; ++k) mp[k] = 0;
@@ -710,11 +725,34 @@ __device__ __forceinline__ uint32_t flush_local_queue(LocalQueue &lq, uint64_t *
 // offset o among the 26 has voxels on each of its faces that o crosses (the
 // per-face flags make this a superset of the tiles reached; a tile queued
 // early only runs steps that leave it as it is).
-template <int CONN, typename G, typename F>
+// DS (two dilation steps per barrier, CONN 6): reached within two steps -- a
+// face neighbour with voxels in its two layers toward the tile, or an edge
+// neighbour with voxels on both faces toward it (a superset of its edge line).
+template <int CONN, bool DS = false, typename G, typename F>
 __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_t tile, int tx,
                                                int ty, int tz, F flags) {
   uint32_t act = 0;
-  if constexpr (CONN == 6) {
+  if constexpr (DS) {
+    static_assert(CONN == 6, "two steps per barrier with the 6 element");
+#pragma unroll
+    for (int j = 0; j < 27; ++j) {
+      const int ox = j % 3 - 1, oy = (j / 3) % 3 - 1, oz = j / 9 - 1;
+      const int nzc = (ox != 0) + (oy != 0) + (oz != 0);
+      if (nzc == 0 || nzc == 3) continue;
+      const int ux = tx + ox, uy = ty + oy, uz = tz + oz;
+      if (ux < b.lo[0] || ux > b.hi[0] || uy < b.lo[1] || uy > b.hi[1] || uz < b.lo[2] ||
+          uz > b.hi[2])
+        continue;
+      const uint32_t f = flags((uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy));
+      // the neighbour's faces toward this tile: one layer (bits 2..7) or two (8..13)
+      const int base = nzc == 1 ? 8 : 2;
+      const uint32_t need = (ox ? 1u << (base + (ox > 0 ? 0 : 1)) : 0u) |
+                            (oy ? 1u << (base + (oy > 0 ? 2 : 3)) : 0u) |
+                            (oz ? 1u << (base + (oz > 0 ? 4 : 5)) : 0u);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) act |= (uint32_t)((((f >> (16 * m)) & need) == need)) << m;
+    }
+  } else if constexpr (CONN == 6) {
     // bit of the neighbour's flags that marks its face toward this tile, per
     // direction (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z)
     constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
@@ -724,7 +762,7 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
       if (nb == kNoTile) continue;
       const uint32_t f = flags(nb);
 #pragma unroll
-      for (int m = 0; m < 2; ++m) act |= ((f >> (8 * m + kFacing[dir])) & 1u) << m;
+      for (int m = 0; m < 2; ++m) act |= ((f >> (16 * m + kFacing[dir])) & 1u) << m;
     }
   } else {
 #pragma unroll
@@ -741,7 +779,7 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
       const uint32_t need = (ox ? 1u << (ox > 0 ? 2 : 3) : 0u) | (oy ? 1u << (oy > 0 ? 4 : 5) : 0u) |
                             (oz ? 1u << (oz > 0 ? 6 : 7) : 0u);
 #pragma unroll
-      for (int m = 0; m < 2; ++m) act |= (uint32_t)((((f >> (8 * m)) & need) == need)) << m;
+      for (int m = 0; m < 2; ++m) act |= (uint32_t)((((f >> (16 * m)) & need) == need)) << m;
     }
   }
   return act;
@@ -826,7 +864,7 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     uint64_t ent[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const uint32_t f = (own >> (8 * m)) & 0xFF;
+      const uint32_t f = (own >> (16 * m)) & 0xFFFF;
       const bool any = f & 1, full = (f >> 1) & 1;
       const bool act = ((reach >> m) & 1u) != 0;
       const bool needed = robust_cut(d.robust, m == 0 ? nfp : nfn) != 0;
@@ -968,23 +1006,30 @@ struct PassState {
 // counters to zero.  Waits are bounded as in grid_barrier.  s_cnt[0..3] holds the four counts
 // and is zeroed here, as is *s_zero (the kept count of the queue that becomes
 // the next iteration's push target).  Returns false on a timeout.
+// DS: each engine iteration is two dilation steps; the arrival carries the
+// second step's hits too (s_cnt[4], s_cnt[5]) and the fold records them as
+// the next histogram bin, stopping a pass at the bin that reaches its cut.
+template <bool DS = false>
 __device__ __forceinline__ bool grid_barrier_single(
     Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, uint32_t *s_zero,
     PassState &ps, const PairView &d, uint32_t tdone, uint32_t *qstats, int qstats_len,
     uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red, uint32_t slots,
     uint8_t *s_kept, uint32_t *s_fs, int qz) {
+  constexpr int kArr = DS ? 6 : 4;  // arrival words
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     *s_ok = 1;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) st_sc(&es->arr[blockIdx.x][j], tag(epoch, s_cnt[j]));
-    s_cnt[0] = s_cnt[1] = s_cnt[2] = s_cnt[3] = 0;  // the next iteration's counts start here
+    for (int j = 0; j < kArr; ++j) st_sc(&es->arr[blockIdx.x][j], tag(epoch, s_cnt[j]));
+#pragma unroll
+    for (int j = 0; j < kArr; ++j) s_cnt[j] = 0;  // the next iteration's counts start here
     *s_zero = 0;
   }
   constexpr int kWords = kSubQ + 3;  // flags, kSubQ + 1 prefix offsets, free slots
+  __shared__ unsigned long long s_red2[2 * kWavesPerWg];  // second-step hits per wave
   if (blockIdx.x == 0) {
-    unsigned long long h0 = 0, h1 = 0, np = 0, nl = 0;
+    unsigned long long h0 = 0, h1 = 0, np = 0, nl = 0, g0 = 0, g1 = 0;
     int ok = 1;
     const uint64_t t0 = wall_clock64();
     // up to kPoll records per thread are polled together: all their loads are
@@ -997,20 +1042,23 @@ __device__ __forceinline__ bool grid_barrier_single(
       for (int k = 0; k < kPoll; ++k)
         if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
       while (pending) {
-        unsigned long long w[kPoll][4];
+        unsigned long long w[kPoll][kArr];
 #pragma unroll
         for (int k = 0; k < kPoll; ++k) {
           if ((pending >> k) & 1) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) w[k][j] = ld_sc(&es->arr[i0 + k * blockDim.x][j]);
+            for (int j = 0; j < kArr; ++j) w[k][j] = ld_sc(&es->arr[i0 + k * blockDim.x][j]);
           }
         }
 #pragma unroll
         for (int k = 0; k < kPoll; ++k) {
-          if (((pending >> k) & 1) && tagged(w[k][0], epoch) && tagged(w[k][1], epoch) &&
-              tagged(w[k][2], epoch) && tagged(w[k][3], epoch)) {
+          bool all = ((pending >> k) & 1) != 0;
+#pragma unroll
+          for (int j = 0; j < kArr; ++j) all = all && tagged(w[k][j], epoch);
+          if (all) {
             h0 += w[k][0] & kTagValue, h1 += w[k][1] & kTagValue;
             np += w[k][2] & kTagValue, nl += w[k][3] & kTagValue;
+            if constexpr (DS) g0 += w[k][4] & kTagValue, g1 += w[k][5] & kTagValue;
             s_kept[i0 + k * blockDim.x] = (uint8_t)(w[k][3] & kTagValue);
             pending &= ~(1u << k);
           }
@@ -1032,9 +1080,14 @@ __device__ __forceinline__ bool grid_barrier_single(
       h0 += shfl64(h0, lane ^ off);
       h1 += shfl64(h1, lane ^ off);
       nl += shfl64(nl, lane ^ off);
+      if constexpr (DS) {
+        g0 += shfl64(g0, lane ^ off);
+        g1 += shfl64(g1, lane ^ off);
+      }
     }
     __syncthreads();
     if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1, s_red[8 + wave] = nl;
+    if (DS && lane == 0) s_red2[wave] = g0, s_red2[kWavesPerWg + wave] = g1;
     if (!ok) *s_ok = 0;
     atomicAdd(&s_red[12 + threadIdx.x % kSubQ], np);
     __syncthreads();
@@ -1079,19 +1132,31 @@ __device__ __forceinline__ bool grid_barrier_single(
                                             s_red[4] + s_red[5] + s_red[6] + s_red[7]};
         const uint32_t kept = (uint32_t)(s_red[8] + s_red[9] + s_red[10] + s_red[11]);
         const int tp = (int)(tdone - d.t_start);
+        // DS: this engine iteration covered dilation steps 2tp+1 and 2tp+2
+        constexpr int kSteps = DS ? 2 : 1;
+        unsigned long long bins[2][2] = {{hits[0], 0ull}, {hits[1], 0ull}};
+        if constexpr (DS) {
+          bins[0][1] = s_red2[0] + s_red2[1] + s_red2[2] + s_red2[3];
+          bins[1][1] = s_red2[4] + s_red2[5] + s_red2[6] + s_red2[7];
+        }
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
-          if (ps.cut[m] == 0 || ps.done_from[m] <= tp || hits[m] == 0) continue;
-          uint32_t *h = m ? d.hist1 : d.hist0;
-          if (tp + 1 < d.hist_len) h[tp + 1] = (uint32_t)hits[m];
-          ps.cov[m] += hits[m];
-          ps.sum[m] += hits[m] * (unsigned long long)(tp + 1);
-          ps.hd[m] = (uint32_t)(tp + 1);
-          if (ps.cov[m] >= ps.cut[m]) ps.done_from[m] = tp + 1;
+#pragma unroll
+          for (int k = 0; k < kSteps; ++k) {
+            const int it = kSteps * tp + 1 + k;  // the dilation iteration of this bin
+            const unsigned long long hm = bins[m][k];
+            if (ps.cut[m] == 0 || ps.done_from[m] < it || hm == 0) continue;
+            uint32_t *h = m ? d.hist1 : d.hist0;
+            if (it < d.hist_len) h[it] = (uint32_t)hm;
+            ps.cov[m] += hm;
+            ps.sum[m] += hm * (unsigned long long)it;
+            ps.hd[m] = (uint32_t)it;
+            if (ps.cov[m] >= ps.cut[m]) ps.done_from[m] = it;
+          }
         }
 #pragma unroll
         for (int m = 0; m < 2; ++m)
-          if (ps.cut[m] == 0 || ps.done_from[m] <= tp + 1) flags |= 1u << m;
+          if (ps.cut[m] == 0 || ps.done_from[m] <= kSteps * tp + kSteps) flags |= 1u << m;
         es->qtotal += ps.cur_total;
         es->qmax = max(es->qmax, ps.cur_total);
         if (qstats && tdone < (uint32_t)qstats_len) qstats[tdone] = ps.cur_total;
@@ -1250,30 +1315,34 @@ __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
 // stops folding a pass at the bin where the covered count reaches the cut
 // ceil(r * total) (hello.cu:1748-1759, 1887-1895); so HD, sum and count are
 // read, not rescanned.
+// steps: dilation iterations per engine iteration (2 when each barrier
+// covers two steps)
 __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t_engine,
-                                 bool error) {
+                                 bool error, int steps = 1) {
   PairOut o;
   const unsigned long long n_fp = ld_sc(&c.n_fp), n_fn = ld_sc(&c.n_fn);
   const unsigned long long n_gold = ld_sc(&c.n_gold), n_segm = ld_sc(&c.n_segm);
-  const int tl = (int)(t_engine - d.t_start);  // pair-local iterations so far
+  const int tl = (int)(t_engine - d.t_start) * steps;  // pair-local iterations so far
   bool done = true, reached = true;
   unsigned long long sum[2], cnt[2];
-  int hd[2];
+  int hd[2], last = 0;
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
     const unsigned long long cut = robust_cut(d.robust, m == 0 ? n_fp : n_fn);
     sum[m] = cut ? ld_sc(&c.sum_d[m]) : 0ull;
     cnt[m] = cut ? ld_sc(&c.cov[m]) : 0ull;
     hd[m] = cut ? (int)ld_sc(&c.hd_last[m]) : 0;
-    done = done && (cut == 0 || ld_sc(&c.done_from[m]) <= tl);
+    const int df = ld_sc(&c.done_from[m]);
+    done = done && (cut == 0 || df <= tl);
+    if (cut) last = max(last, df);
     reached = reached && cnt[m] >= cut;
   }
+  o.iters = done ? last : tl;
   o.hd_fp = hd[0], o.hd_fn = hd[1];
   o.hd = max(hd[0], hd[1]);
   o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
   o.n_gold = n_gold, o.n_segm = n_segm, o.n_fp = n_fp, o.n_fn = n_fn;
   o.done = done ? 1 : 0;
-  o.pad = 0;
   int st = ld_sc(&c.status);
   if ((n_fp && !n_gold) || (n_fn && !n_segm))
     st = -2;  // HDM_E_EMPTY_MASK
@@ -1292,8 +1361,10 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
 // itself, separated by grid barriers, so one call is one launch; every store
 // that a later phase reads in another workgroup is written through, and those
 // reads are agent-coherent.
-template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6>
-__global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
+template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6,
+          bool DS = false>
+__device__ __forceinline__ void dilate_body(const DilateArgs &a) {
+  static_assert(!DS || (FUSED && CONN == 6), "two steps per barrier: fused single pair, 6 element");
   static_assert(!FUSED || SINGLE, "the fused launch holds one pair");
   const PairDesc *__restrict__ descs = a.descs;
   PairCtr *ctrs = a.ctrs;
@@ -1316,7 +1387,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   // single pair: this iteration's hits per pass, entries pushed to the global
   // queue and entries kept; the released flags; the master's reduction space;
   // the workgroup's own queue of the running and of the next iteration
-  __shared__ uint32_t s_cnt[4];
+  __shared__ uint32_t s_cnt[6];
   __shared__ uint32_t s_flags[1];
   __shared__ unsigned long long s_red[12 + kSubQ];
   // kept entries: at most one per wave slot (kSlots), taken by slots 0..;
@@ -1343,7 +1414,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   const uint32_t kq = blockIdx.x % kSubQ;
   if (threadIdx.x == 0) lq.n = 0;
   if (threadIdx.x < kHitTable) ht.key[threadIdx.x] = ~0u, ht.cnt[threadIdx.x] = 0;
-  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 6) s_cnt[threadIdx.x] = 0;
   if (threadIdx.x < 2) s_stn[threadIdx.x] = 0;
   if (threadIdx.x == 0) {  // no kept entries yet: every slot is free
     s_fs[0] = blockIdx.x * kSlots;
@@ -1451,7 +1522,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         uint32_t own = 0, reach = 0;
         if (in_box) {
           own = ld_sc(d.tflags + tile);
-          reach = seed_reach<CONN>(v0, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
+          reach = seed_reach<CONN, DS>(v0, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
             return ld_sc(d.tflags + nb);
           });
         }
@@ -1459,7 +1530,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         uint64_t ent[2];
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
-          const uint32_t f = (own >> (8 * m)) & 0xFF;
+          const uint32_t f = (own >> (16 * m)) & 0xFFFF;
           const bool any = f & 1, full = (f >> 1) & 1;
           const bool act = ((reach >> m) & 1u) != 0;
           const bool needed = v0.cut[m] != 0;
@@ -1475,7 +1546,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       if (threadIdx.x == 0) s_cnt[2] += nflushed, s_cnt[3] = s_stn[0];
       if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[2] = wall_clock64();
       // hands iteration 0 its flags, global prefix and free slots
-      run = grid_barrier_single(es, a.ebase + 3, spin_limit, &s_ok, s_cnt, &s_stn[1], ps, v0,
+      run = grid_barrier_single<DS>(es, a.ebase + 3, spin_limit, &s_ok, s_cnt, &s_stn[1], ps, v0,
                                 ~0u, qstats, qstats_len, nullptr, s_pre, s_flags, s_red, kSlots,
                                 s_kept, s_fs, 2);
     }
@@ -1557,7 +1628,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         // a pass whose cut was reached stops (hello.cu:1748-1759)
         const unsigned long long cut_m = m ? v.cut[1] : v.cut[0];
         if (live && (SINGLE ? pdone[m] : (cut_m == 0 || dfrom <= (int)tp))) live = false;
-        if (live && tp + 1 >= (uint32_t)v.hist_len) {
+        if (live && (DS ? 2 * tp + 2 : tp + 1) >= (uint32_t)v.hist_len) {
           st_sc(&ctrs[p].status, -8);
           live = false;
         }
@@ -1590,8 +1661,170 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         if (rowok) own = first ? ldg(orig_m + ro) : ld_sc(bufr + ro);
         const Row other = (rowok && check) ? ldg(orig_o + ro) : Row(0);
         uint32_t nb_dir, nst;  // the neighbour this lane activates and its state
-        Row dl;
-        if constexpr (CONN == 6) {
+        Row dl, d1 = 0;
+        if constexpr (DS) {
+          // Two 6-connectivity steps in one (64x8x8 tiles, lane = row): the
+          // second step's output is the union over the L1 ball of radius 2,
+          //   D2 = x2(own) | x1(rows y+-1, z+-1) | rows y+-2, z+-2 | rows (y+-1, z+-1)
+          // and the first step's D1 = x1(own) | rows y+-1, z+-1 gives the hits
+          // of the first step.  Beyond the tile: the x neighbours' rows at this
+          // lane's (y, z) (two boundary bits), rows at depth 1 and 2 of the y
+          // and z neighbours (face lanes), the yz-diagonal row (corner lanes),
+          // and the x-face words of the x-side tiles at (y, z), (y+-1), (z+-1)
+          // for the one-bit x halo of the rows y+-1, z+-1 (8x8 bit planes).
+          // Lane j < 27 reads the state of neighbour j = (ox+1) + 3(oy+1) +
+          // 9(oz+1) of the 18 that two steps can reach (no corners).
+          static_assert(TXB == 64 && TY == 8 && TZ == 8 && CONN == 6, "64x8x8, 6 element");
+          const int j = lane;
+          const int jox = j % 3 - 1, joy = (j / 3) % 3 - 1, joz = j / 9 - 1;
+          const int jn = (jox != 0) + (joy != 0) + (joz != 0);
+          nb_dir = kNoTile;
+          if (live && j < 27 && jn >= 1 && jn <= 2) {
+            const int ux = tx + jox, uy = ty + joy, uz = tz + joz;
+            if (ux >= v.b.lo[0] && ux <= v.b.hi[0] && uy >= v.b.lo[1] && uy <= v.b.hi[1] &&
+                uz >= v.b.lo[2] && uz <= v.b.hi[2])
+              nb_dir = (uint32_t)((int)tile + jox + joy * v.gx + joz * v.gx * v.gy);
+          }
+          nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
+          uint64_t fwo = 0, fwb = 0;  // face word of x-side neighbour j toward this tile
+          if (nb_dir != kNoTile && jox != 0) {
+            const uint64_t o = 2 * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
+            fwo = ldg(xfo + o);
+            fwb = ld_sc(xfr + o);
+          }
+          // the x neighbours' rows at (y, z)
+          const uint32_t nbxl = (rowok && tx > v.b.lo[0]) ? tile - 1 : kNoTile;
+          const uint32_t nbxh = (rowok && tx < v.b.hi[0]) ? tile + 1 : kNoTile;
+          Row xlo = 0, xlb = 0, xho = 0, xhb = 0;
+          if (nbxl != kNoTile) {
+            xlo = ldg(orig_m + (uint64_t)nbxl * R + r);
+            xlb = ld_sc(bufr + (uint64_t)nbxl * R + r);
+          }
+          if (nbxh != kNoTile) {
+            xho = ldg(orig_m + (uint64_t)nbxh * R + r);
+            xhb = ld_sc(bufr + (uint64_t)nbxh * R + r);
+          }
+          // rows beyond the tile in y and z, depth 1 and 2, and the yz diagonal
+          const int oy = yl == 0 ? -1 : yl == TY - 1 ? 1 : 0;
+          const int oz = zl == 0 ? -1 : zl == TZ - 1 ? 1 : 0;
+          auto nb_of = [&](int oyy, int ozz) -> uint32_t {
+            const int uy = ty + oyy, uz = tz + ozz;
+            if (uy < v.b.lo[1] || uy > v.b.hi[1] || uz < v.b.lo[2] || uz > v.b.hi[2]) return kNoTile;
+            return (uint32_t)((int)tile + oyy * v.gx + ozz * v.gx * v.gy);
+          };
+          const uint32_t nby = (rowok && oy) ? nb_of(oy, 0) : kNoTile;
+          const uint32_t nbz = (rowok && oz) ? nb_of(0, oz) : kNoTile;
+          const uint32_t nbd = (rowok && oy && oz) ? nb_of(oy, oz) : kNoTile;
+          // a neighbour above is a tail tile: its second layer may not exist
+          const bool y2 = nby != kNoTile && (oy < 0 || (ty + 1) * TY + 1 < v.ny);
+          const bool z2 = nbz != kNoTile && (oz < 0 || (tz + 1) * TZ + 1 < v.nz);
+          const uint64_t ry1 = (uint64_t)nby * R + (oy < 0 ? TY - 1 : 0) + TY * zl;
+          const uint64_t ry2 = (uint64_t)nby * R + (oy < 0 ? TY - 2 : 1) + TY * zl;
+          const uint64_t rz1 = (uint64_t)nbz * R + yl + TY * (oz < 0 ? TZ - 1 : 0);
+          const uint64_t rz2 = (uint64_t)nbz * R + yl + TY * (oz < 0 ? TZ - 2 : 1);
+          const uint64_t rdd = (uint64_t)nbd * R + (oy < 0 ? TY - 1 : 0) + TY * (oz < 0 ? TZ - 1 : 0);
+          Row y1o = 0, y1b = 0, y2o = 0, y2b = 0, z1o = 0, z1b = 0, z2o = 0, z2b = 0, edo = 0, edb = 0;
+          if (nby != kNoTile) y1o = ldg(orig_m + ry1), y1b = ld_sc(bufr + ry1);
+          if (y2) y2o = ldg(orig_m + ry2), y2b = ld_sc(bufr + ry2);
+          if (nbz != kNoTile) z1o = ldg(orig_m + rz1), z1b = ld_sc(bufr + rz1);
+          if (z2) z2o = ldg(orig_m + rz2), z2b = ld_sc(bufr + rz2);
+          if (nbd != kNoTile) edo = ldg(orig_m + rdd), edb = ld_sc(bufr + rdd);
+          // -- states -> the candidate that holds iteration tp's input
+          const Row vfull = valid_bits<Row>(tx, v.nx);
+          auto pick = [&](uint32_t nb, uint32_t st, Row o, Row bsel, Row fullv) -> Row {
+            if (nb == kNoTile) return 0;
+            if (st & kFull) return fullv;
+            if (st != 0 && st - 1 < tp) return bsel;
+            return o;
+          };
+          const uint32_t sty = shfl32(nst, 13 + 3 * oy), stz = shfl32(nst, 13 + 9 * oz);
+          const uint32_t std_ = shfl32(nst, 13 + 3 * oy + 9 * oz);
+          const uint32_t stxl = shfl32(nst, 12), stxh = shfl32(nst, 14);
+          const Row ey1 = pick(nby, sty, y1o, y1b, vfull);
+          const Row ey2 = y2 ? pick(nby, sty, y2o, y2b, vfull) : Row(0);
+          const Row ez1 = pick(nbz, stz, z1o, z1b, vfull);
+          const Row ez2 = z2 ? pick(nbz, stz, z2o, z2b, vfull) : Row(0);
+          const Row ed = pick(nbd, std_, edo, edb, vfull);
+          const Row xl = pick(nbxl, stxl, xlo, xlb, ~Row(0));  // never the x tail tile
+          const Row xh = pick(nbxh, stxh, xho, xhb, valid_bits<Row>(tx + 1, v.nx));
+          uint64_t fw = 0;  // face word, a full tile's face has a bit for every row it has
+          if (nb_dir != kNoTile && jox != 0) {
+            if (nst & kFull) {
+              const int yy = v.ny - (ty + joy) * TY, zz = v.nz - (tz + joz) * TZ;
+              const uint64_t ym = yy >= TY ? 0xFFull : ((1ull << yy) - 1);
+              uint64_t rows = 0;
+#pragma unroll
+              for (int k = 0; k < TZ; ++k) rows |= k < zz ? ym << (TY * k) : 0ull;
+              fw = rows;
+            } else {
+              fw = (nst != 0 && nst - 1 < tp) ? fwb : fwo;
+            }
+          }
+          // -- the rows around (y, z): shuffles inside the tile, loaded rows
+          // beyond; each is folded in as it comes, so few stay live.  f1: the
+          // rows y+-1, z+-1; f2: rows y+-2, z+-2 and the four (y+-1, z+-1)
+          Row f1, f2;
+          {
+            Row t = shfl_row<Row>(own, lane - 1);
+            f1 = yl > 0 ? t : ey1;
+            t = shfl_row<Row>(own, lane + 1);
+            f1 |= yl < TY - 1 ? t : ey1;
+            t = shfl_row<Row>(own, lane - TY);
+            f1 |= zl > 0 ? t : ez1;
+            t = shfl_row<Row>(own, lane + TY);
+            f1 |= zl < TZ - 1 ? t : ez1;
+            Row e = shfl_row<Row>(ey1, lane - 1);
+            t = shfl_row<Row>(own, lane - 2);
+            f2 = yl > 1 ? t : yl == 1 ? e : ey2;
+            e = shfl_row<Row>(ey1, lane + 1);
+            t = shfl_row<Row>(own, lane + 2);
+            f2 |= yl < TY - 2 ? t : yl == TY - 2 ? e : ey2;
+            e = shfl_row<Row>(ez1, lane - TY);
+            t = shfl_row<Row>(own, lane - 2 * TY);
+            f2 |= zl > 1 ? t : zl == 1 ? e : ez2;
+            e = shfl_row<Row>(ez1, lane + TY);
+            t = shfl_row<Row>(own, lane + 2 * TY);
+            f2 |= zl < TZ - 2 ? t : zl == TZ - 2 ? e : ez2;
+            // (y-1, z-1), (y+1, z-1), (y-1, z+1), (y+1, z+1): inside the tile,
+            // or an edge lane's neighbour in ey1 / ez1, or the corner's ed
+            const Row eym8 = shfl_row<Row>(ey1, lane - TY), eyp8 = shfl_row<Row>(ey1, lane + TY);
+            const Row ezm1 = shfl_row<Row>(ez1, lane - 1), ezp1 = shfl_row<Row>(ez1, lane + 1);
+            t = shfl_row<Row>(own, lane - TY - 1);
+            f2 |= yl > 0 ? (zl > 0 ? t : ezm1) : (zl > 0 ? eym8 : ed);
+            t = shfl_row<Row>(own, lane - TY + 1);
+            f2 |= yl < TY - 1 ? (zl > 0 ? t : ezp1) : (zl > 0 ? eym8 : ed);
+            t = shfl_row<Row>(own, lane + TY - 1);
+            f2 |= yl > 0 ? (zl < TZ - 1 ? t : ezm1) : (zl < TZ - 1 ? eyp8 : ed);
+            t = shfl_row<Row>(own, lane + TY + 1);
+            f2 |= yl < TY - 1 ? (zl < TZ - 1 ? t : ezp1) : (zl < TZ - 1 ? eyp8 : ed);
+          }
+          // -- x halo of the rows y+-1, z+-1: the x-side face planes (lanes
+          // 12, 9, 15, 3, 21 at x-1; 14, 11, 17, 5, 23 at x+1), shifted by one
+          // in y or z
+          auto plane_of = [&](int jj) -> uint64_t {
+            return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fw, jj)) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(fw >> 32), jj)
+                    << 32);
+          };
+          auto cross = [&](int side) -> uint64_t {  // side 0: x-1, 2: x+1
+            constexpr uint64_t kY0 = 0x0101010101010101ull, kY7 = 0x8080808080808080ull;
+            const uint64_t c = plane_of(side + 12), ym = plane_of(side + 9), yp = plane_of(side + 15);
+            const uint64_t zm = plane_of(side + 3), zp = plane_of(side + 21);
+            return ((c >> 1) & ~kY7) | ((yp & kY0) << 7) | ((c << 1) & ~kY0) | ((ym & kY7) >> 7) |
+                   (c >> 8) | (zp << 56) | (c << 8) | (zm >> 56);
+          };
+          const uint64_t clo = cross(0), chi = cross(2);
+          const Row x1own = own | (Row)(own << 1) | (Row)(own >> 1) | (Row)(xl >> 63) |
+                            (Row)(xh << 63);
+          const Row x2own = x1own | (Row)(own << 2) | (Row)(own >> 2) | (Row)((xl >> 62) & 1) |
+                            (Row)((xl >> 63) << 1) | (Row)(((xh >> 1) & 1) << 63) |
+                            (Row)((xh & 1) << 62);
+          const Row x1f = f1 | (Row)(f1 << 1) | (Row)(f1 >> 1) | (Row)((clo >> r) & 1) |
+                          (Row)(((chi >> r) & 1) << 63);
+          d1 = (x1own | f1) & vb;
+          dl = (x2own | x1f | f2) & vb;
+          if (!rowok) d1 = 0, dl = 0;
+        } else if constexpr (CONN == 6) {
           // neighbour tiles: lanes r < 6 also read the state of neighbour r
           nb_dir = (live && r < 6) ? neighbour(v, v.b, tile, tx, ty, tz, r) : kNoTile;
           const uint32_t nb_xl = live ? neighbour(v, v.b, tile, tx, ty, tz, 0) : kNoTile;
@@ -1789,17 +2022,24 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
         // -- input full -> the tile leaves the queue (hello.cu:1319, 1634-1643)
         const bool full = group_bits<R>(__ballot(live && exists && own != vb), lane) == 0;
         // -- newly covered voxels of the other mask: res & ~src & ref (hello.cu:1590-1626)
-        const Row nw = dl & (Row)~own & other;
+        //    (DS: d1 holds the first step, hits at 2tp+1; dl the second, 2tp+2)
+        const Row nw = DS ? (Row)(d1 & ~own & other) : (Row)(dl & ~own & other);
+        const Row nw2 = DS ? (Row)(dl & ~d1 & other) : Row(0);
         const uint32_t cnt = group_sum<R>((uint32_t)popc_row(nw), lane);
+        const uint32_t cnt2 = DS ? group_sum<R>((uint32_t)popc_row(nw2), lane) : 0u;
         const uint32_t rem = group_sum<R>((uint32_t)popc_row((Row)(other & ~dl)), lane);
-        if (nw && v.map) {
-          Row bits = nw;
+        if ((nw | nw2) && v.map) {
           const uint64_t vrow =
               (uint64_t)tx * TXB + (uint64_t)v.nx * ((uint64_t)y + (uint64_t)v.ny * z);
-          while (bits) {
-            const int kb = (int)__builtin_ctzll((uint64_t)bits);
-            bits &= bits - 1;
-            v.map[vrow + kb] = (uint16_t)(tp + 1);
+#pragma unroll
+          for (int k = 0; k < (DS ? 2 : 1); ++k) {
+            Row bits = k ? nw2 : nw;
+            const uint16_t it = (uint16_t)(DS ? 2 * tp + 1 + k : tp + 1);
+            while (bits) {
+              const int kb = (int)__builtin_ctzll((uint64_t)bits);
+              bits &= bits - 1;
+              v.map[vrow + kb] = it;
+            }
           }
         }
         if (live && r == 0 && cnt) {
@@ -1808,10 +2048,14 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
           else
             hit_add(ht, ctrs, (uint32_t)p * 2 + m, cnt);
         }
+        if (DS && live && r == 0 && cnt2) atomicAdd(&s_cnt[4 + m], cnt2);
         // -- outputs
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
         const uint64_t oany = group_bits<R>(__ballot(dl != 0), lane);
+        // DS: the two outermost columns toward -x / +x
+        const uint64_t ox2l = DS ? group_bits<R>(__ballot((dl & 3) != 0), lane) : 0ull;
+        const uint64_t ox2h = DS ? group_bits<R>(__ballot(((dl >> (TXB - 2)) & 3) != 0), lane) : 0ull;
         const bool oyl = group_bits<R>(__ballot(yl == 0 && dl != 0), lane) != 0;
         const bool oyh = group_bits<R>(__ballot(yl == TY - 1 && dl != 0), lane) != 0;
         const bool ozl = group_bits<R>(__ballot(zl == 0 && dl != 0), lane) != 0;
@@ -1829,14 +2073,24 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
             }
             // lane kSelf re-queues the tile while it is not full; lanes
             // below it activate idle neighbours the output reaches
-            constexpr int kSelf = CONN == 6 ? 6 : 27;
+            constexpr int kSelf = (CONN == 6 && !DS) ? 6 : 27;
             if (r == kSelf) {
               want = true;
               pent = make_entry(tile, (uint32_t)p, (uint32_t)m, 0, rem != 0, v.gen);
             }
             if (r < kSelf && nb_dir != kNoTile && nst == 0) {
               bool reach;
-              if constexpr (CONN == 6) {
+              if constexpr (DS) {  // within two steps: two layers of a face, or an edge line
+                const int jox = r % 3 - 1, joy = (r / 3) % 3 - 1, joz = r / 9 - 1;
+                const bool face = (jox != 0) + (joy != 0) + (joz != 0) == 1;
+                const uint64_t xs = jox < 0 ? (face ? ox2l : oxl) : jox > 0 ? (face ? ox2h : oxh) : oany;
+                const uint64_t ym = joy < 0 ? (face ? 0x0303030303030303ull : 0x0101010101010101ull)
+                                    : joy > 0 ? (face ? 0xC0C0C0C0C0C0C0C0ull : 0x8080808080808080ull)
+                                              : ~0ull;
+                const uint64_t zm = joz < 0 ? (face ? 0xFFFFull : 0xFFull)
+                                    : joz > 0 ? (face ? 0xFFFFull << 48 : 0xFFull << 56) : ~0ull;
+                reach = (xs & ym & zm) != 0;
+              } else if constexpr (CONN == 6) {
                 reach = r == 0 ? oxl != 0 : r == 1 ? oxh != 0 : r == 2 ? oyl
                         : r == 3 ? oyh : r == 4 ? ozl : ozh;
               } else {  // a voxel on every face that the offset of neighbour r crosses
@@ -1948,7 +2202,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       }
     };
     if (SINGLE) {
-      if (!grid_barrier_single(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
+      if (!grid_barrier_single<DS>(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
                                v0, tdone, qstats, qstats_len, tr, s_pre, s_flags, s_red, kSlots,
                                s_kept, s_fs, (t + 2) % 4))
         break;
@@ -1970,7 +2224,7 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
   if (blockIdx.x == 0) {
     const bool err = ld_sc(&es->error) != 0;
     if constexpr (FUSED) {
-      if (threadIdx.x == 0) a.outs[0] = finalize_pair(a.desc, ctrs[0], t, err);
+      if (threadIdx.x == 0) a.outs[0] = finalize_pair(a.desc, ctrs[0], t, err, DS ? 2 : 1);
     } else {
       for (int p = threadIdx.x; p < npairs; p += blockDim.x)
         a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
@@ -2002,6 +2256,18 @@ __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
       }
     }
   }
+}
+
+template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6>
+__global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
+  dilate_body<TXB, TY, TZ, SINGLE, FUSED, CONN>(a);
+}
+
+// The fused launch with two dilation steps per barrier.  Its step needs
+// about 195 VGPRs (two resident workgroups per CU); held to three, it spills
+// in the step and measured slower, so it runs with two.
+__global__ __launch_bounds__(kWgThreads) void dilate_ds_kernel(DilateArgs a) {
+  dilate_body<64, 8, 8, true, true, 6, true>(a);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -2036,16 +2302,24 @@ hipError_t launch_fused(dim3 grid, hipStream_t s, const DilateArgs &a) {
                      s, a);
   return hipGetLastError();
 }
+// two dilation steps per barrier (64x8x8, 6 element)
+hipError_t launch_fused_ds(dim3 grid, hipStream_t s, const DilateArgs &a) {
+  hipLaunchKernelGGL(dilate_ds_kernel, grid, dim3(kWgThreads), 0, s, a);
+  return hipGetLastError();
+}
 
 template <int TXB, int TY, int TZ, int CONN = 6>
 TileLaunch make_launch() {
+  constexpr bool kDS = TXB == 64 && TY == 8 && TZ == 8 && CONN == 6;
   return TileLaunch{&launch_pack<TXB, TY, TZ>,
                     &launch_seed<TXB, TY, TZ, CONN>,
                     &launch_dilate<TXB, TY, TZ, CONN>,
                     &launch_fused<TXB, TY, TZ, CONN>,
+                    kDS ? &launch_fused_ds : nullptr,
                     (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
+                    kDS ? (const void *)&dilate_ds_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     TXB,
                     TY,

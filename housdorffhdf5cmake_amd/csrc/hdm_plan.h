@@ -55,9 +55,12 @@ struct TileLaunch {
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
   hipError_t (*dilate)(dim3, hipStream_t, const DilateArgs &);
   hipError_t (*fused)(dim3, hipStream_t, const DilateArgs &);  // one pair, one launch
+  // the fused launch with two dilation steps per barrier (null: not built)
+  hipError_t (*fused_ds)(dim3, hipStream_t, const DilateArgs &);
   const void *dilate_fn;         // several pairs
   const void *dilate_single_fn;  // one pair
   const void *fused_fn;          // one pair, reset + pack + seed + dilation
+  const void *fused_ds_fn;
   const void *pack_fn;
   int txb, ty, tz;
   int conn;  // structuring element of the seed and dilation kernels
@@ -110,7 +113,7 @@ inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
   g.buf_bytes = align256(g.words * (ts.txb / 8));
   g.xf_bytes = align256((uint64_t)g.ntiles * 16);
   g.st_bytes = align256((uint64_t)g.ntiles * 4);
-  g.tf_bytes = align256((uint64_t)g.ntiles * 2);
+  g.tf_bytes = align256((uint64_t)g.ntiles * 4);
   g.hist_bytes = align256((uint64_t)g.hist_len * 4);
   g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + g.tf_bytes + 2 * g.hist_bytes;
   return g;
@@ -134,7 +137,7 @@ inline void carve_pair(PairDesc &d, const PairGeom &g, char *base) {
     for (int s = 0; s < 2; ++s) d.xf_buf[m][s] = (uint64_t *)take(g.xf_bytes);
   d.state[0] = (uint32_t *)take(g.st_bytes);
   d.state[1] = (uint32_t *)take(g.st_bytes);
-  d.tflags = (uint16_t *)take(g.tf_bytes);
+  d.tflags = (uint32_t *)take(g.tf_bytes);
   d.hist[0] = (uint32_t *)take(g.hist_bytes);
   d.hist[1] = (uint32_t *)take(g.hist_bytes);
 }
