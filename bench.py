@@ -74,7 +74,8 @@ def leg_single(hdm, torch, steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     dom = hdm.timing_log()  # the events of the timed steps, read after them
-    fused = hdm.last_timings()["fused"]
+    lt = hdm.last_timings()
+    fused, steps_per_barrier = lt["fused"], lt["steps"]
     assert len(dom) == steps and min(dom) > 0, dom
     phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
     for i in range(steps):
@@ -85,16 +86,19 @@ def leg_single(hdm, torch, steps, warmup):
     avg = {k: sum(v) / len(v) for k, v in phases.items()}
     avg["dominant_timed_steps"] = sum(dom) / len(dom)
     avg["fused"] = fused
+    avg["steps_per_barrier"] = steps_per_barrier
     del pairs
     return G, S, res, dt, nvox, avg
 
 
 def roofline(res, avg, nvox, tile=1):
     """Algorithmic bytes per launch of each phase over its mean event time."""
+    # a queued (tile, mask) entry does the work of steps_per_barrier dilation
+    # steps on its tile (two when the launch runs two steps per barrier)
     algo = {
         "pack": 2.25 * nvox,  # read 2 uint8 masks, write 2 bits per voxel
         "meta": 0.25 * nvox,  # read both packed masks
-        "dilate": float(res["queue_total"]) * TILE_BYTES[tile],
+        "dilate": float(res["queue_total"]) * TILE_BYTES[tile] * avg.get("steps_per_barrier", 1),
     }
     rows = {}
     for k, b in algo.items():
@@ -208,18 +212,21 @@ def leg_long(hdm, torch, reps):
     avg = {k: sum(v) / len(v) for k, v in phases.items()}
     hdm.hausdorff(G, S, timing=2)
     tr = hdm.last_trace()
+    steps = hdm.last_timings()["steps"]  # dilation steps per grid barrier
     occ = [r["queued"] for r in tr]
     it_us = [r["next_start_us"] for r in tr if r["next_start_us"]]
-    dil_bytes = float(res["queue_total"]) * TILE_BYTES[1]
+    dil_bytes = float(res["queue_total"]) * TILE_BYTES[1] * steps
     return G, S, res, {
         "workload": "long_512x512x512_ellipsoid_80_60_50_cube5_gap300", "hd": res["hd"],
         "avg_hd": res["avg_hd"], "iterations": res["iterations"],
+        "dilation_steps_per_barrier": steps,
         "ms_per_pair": round(dt * 1e3, 3), "mask_voxels_per_s": round(2 * G.numel() / dt, 1),
         "phase_ms": {k: round(v, 4) for k, v in avg.items()},
+        "dilate_us_per_step": round(avg["dilate"] * 1e3 / max(1, res["iterations"]), 2),
         "queue_total": res["queue_total"], "queue_max": res["queue_max"],
-        "queue_per_iteration": {"min": min(occ), "median": statistics.median(occ),
-                                "max": max(occ), "first8": occ[:8], "last8": occ[-8:]},
-        "us_per_iteration_median": round(statistics.median(it_us), 2) if it_us else None,
+        "queue_per_barrier_period": {"min": min(occ), "median": statistics.median(occ),
+                                     "max": max(occ), "first8": occ[:8], "last8": occ[-8:]},
+        "us_per_barrier_period_median_traced": round(statistics.median(it_us), 2) if it_us else None,
         "dilate_algo_bytes": int(dil_bytes),
         "dilate_gbs": round(dil_bytes / (avg["dilate"] * 1e-3) / 1e9, 1)}
 
@@ -367,8 +374,9 @@ def main():
         "result": {k: res[k] for k in ("hd", "avg_hd", "count", "sum", "iterations",
                                        "queue_total", "queue_max")},
         "pairs_per_s": round(args.steps / dt, 2),
-        "phase_ms": {k: round(v, 4) for k, v in avg.items() if k != "fused"},
+        "phase_ms": {k: round(v, 4) for k, v in avg.items() if k not in ("fused", "steps_per_barrier")},
         "fused_launch": bool(avg["fused"]),
+        "dilation_steps_per_barrier": int(avg["steps_per_barrier"]),
     }
     line["roofline"] = roofline(res, avg, nvox)
     if args.copy_probe > 0:

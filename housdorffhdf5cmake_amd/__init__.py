@@ -252,11 +252,12 @@ def last_timings() -> dict:
     """Device ms of the phases of the last call made with timing=True; "fused"
     is True when the call ran as one launch (phases then come from the
     kernel's own clock, and timing=3 fills only "total")."""
-    ms = (C.c_float * 6)()
-    n = lib().hdm_last_timings(ms, 6)
+    ms = (C.c_float * 7)()
+    n = lib().hdm_last_timings(ms, 7)
     names = ("pack", "meta", "dilate", "finalize", "total")
     out = {names[i]: float(ms[i]) for i in range(min(n, 5))}
     out["fused"] = n > 5 and ms[5] == 1.0
+    out["steps"] = int(ms[6]) if n > 6 else 1  # dilation steps per grid barrier
     return out
 
 

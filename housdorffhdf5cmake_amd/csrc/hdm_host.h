@@ -33,7 +33,9 @@ struct Ctx {
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
   hipEvent_t ev[5] = {};     // phase boundaries of a timed call
-  float last_ms[6] = {};     // pack, seed, dilation, 0, all; 1 when the call was one fused launch
+  // pack, seed, dilation, 0, all; 1 when the call was one fused launch;
+  // dilation steps per grid barrier (1 or 2)
+  float last_ms[7] = {};
   // epoch base of the next fused launch's barrier tags; the tagged words of
   // the workspace's engine are cleared whenever the base restarts at 0
   uint32_t ebase = 0;

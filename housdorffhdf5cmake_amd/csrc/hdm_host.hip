@@ -471,6 +471,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   hipError_t e = hipSuccess;
   const bool fused = fused_enabled();
   const uint32_t ebase = c->ebase;
+  bool ds = false;  // two dilation steps per barrier
   if (fused) {
     // one launch: reset, pack, seed and dilation.  Its barrier tags use
     // epochs above every earlier launch's on this workspace; a new workspace
@@ -502,7 +503,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.ebase = c->ebase;
     a.ready = cfg.ready;
     a.ready_seq = cfg.ready_seq;
-    const bool ds = tl->fused_ds && conn == 6 && ds_enabled();
+    ds = tl->fused_ds && conn == 6 && ds_enabled();
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess)
       e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds)), s, a);
@@ -536,6 +537,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   std::memcpy(&o, hout, sizeof(o));
   std::memcpy(&es, hhead, sizeof(es));
   if (o.status == 1) return HDM_E_HIP;
+  c->last_ms[6] = ds ? 2.f : 1.f;
   if (fused) {
     c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used
     c->last_ms[5] = 1.f;
@@ -888,7 +890,7 @@ extern "C" int hdm_last_timings(float *ms, int n) {
     c->last_ms[c->last_pair_fused ? 4 : 2] = t;
     c->last_pair = -1;
   }
-  const int k = std::min(n, 6);
+  const int k = std::min(n, 7);
   for (int i = 0; i < k; ++i) ms[i] = c->last_ms[i];
   return k;
 }

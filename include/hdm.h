@@ -109,7 +109,9 @@ typedef struct hdm_result {
   uint64_t n_gold, n_segm; /* voxels on in each mask                                */
   uint64_t n_fp, n_fn;     /* |FP|, |FN| before any robust cut                       */
   double avg_hd;      /* (double)sum / (double)count, 0 when count == 0             */
-  uint64_t queue_total; /* sum over iterations of queued (tile, mask) entries        */
+  uint64_t queue_total; /* sum over barrier periods of queued (tile, mask) entries;
+                           an entry covers two dilation steps when the launch runs
+                           two per barrier (hdm_last_timings ms[6])                 */
   uint32_t queue_max;   /* largest per-iteration queue                              */
   uint32_t tiles;       /* tiles of the (cropped) tile grid                         */
 } hdm_result;
@@ -133,6 +135,8 @@ uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
  * dominant launch is bracketed by a pair of HIP events, the others are -1:
  * ms[2] the dilation kernel of a separate-kernel call, ms[4] the whole fused
  * launch; such a call does not wait for its events, this function does.
+ * ms[6] is the number of dilation steps the call ran per grid barrier (2 for
+ * the fused 64x8x8 launch with the 6 element, else 1).
  * Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
 /* Device milliseconds of the dominant launch of each single-pair call made
