@@ -80,6 +80,9 @@ struct PairDesc {
 struct PairCtr {
   unsigned long long n_gold, n_segm, n_fp, n_fn;  // totals (written before the dilation)
   unsigned long long cov[2];                      // covered so far; barrier master only
+  // the part of cov covered by the second of two dilation steps per engine
+  // iteration (barrier master only; zero with one step per iteration)
+  unsigned long long cov2[2];
   // Sum of first-hit iterations of the covered voxels and the last iteration
   // with a hit, per pass; the master keeps them with cov while it folds, so
   // they cover exactly the histogram bins up to the pass's stop.
@@ -92,7 +95,9 @@ struct PairCtr {
   int32_t done_from[2];
   uint32_t hd_last[2];
   int32_t pad;
-  unsigned long long hits[2][kHitSlots];        // cumulative hits, spread over slots
+  // cumulative hits, spread over slots: [m] pass m, or with two dilation
+  // steps per engine iteration [2k + m] pass m in step k of each iteration
+  unsigned long long hits[4][kHitSlots];
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
   int32_t box[6][kSumSlots];  // pack partials of lo[0..2] (min) and hi[0..2] (max)
 };

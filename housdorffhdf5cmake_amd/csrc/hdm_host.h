@@ -95,7 +95,7 @@ uint64_t spin_ticks(const Ctx *c);
 int norm_tile(int tile);
 // launch id (hdm_tile_launch) of a tile option and a connectivity option (0 =
 // 6), or -1 when they are invalid; *conn receives the element
-int select_launch(int tile_opt, int conn_opt, int *conn);
+int select_launch(int tile_opt, int conn_opt, int *conn, bool ds = false);
 int barrier_mode();
 bool fused_enabled();
 bool ds_enabled();

@@ -157,8 +157,9 @@ bool fused_enabled() {
   return on;
 }
 
-// The fused single-pair launch runs two dilation steps per grid barrier
-// (64x8x8 tiles, 6 element) unless HDM_DS=0 asks for one.
+// The dilation runs two steps per grid barrier (64x8x8 tiles, 6 element) --
+// the fused single-pair launch, batches and the server -- unless HDM_DS=0
+// asks for one.
 bool ds_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("HDM_DS");
@@ -183,14 +184,16 @@ int barrier_mode() {
 
 int norm_tile(int tile) { return (tile <= 0 || tile >= HDM_TILE_COUNT) ? HDM_TILE_64x8x8 : tile; }
 
-int select_launch(int tile_opt, int conn_opt, int *conn) {
+// ds: the 64x8x8 tile with the 6 element takes the launch id whose seed and
+// dilation kernels run two steps per engine iteration
+int select_launch(int tile_opt, int conn_opt, int *conn, bool ds) {
   if (tile_opt < 0 || tile_opt >= HDM_TILE_COUNT) return -1;
   const int cn = conn_opt == 0 ? 6 : conn_opt;
   if (cn != 6 && cn != 18 && cn != 26) return -1;
   // the 18 and 26 elements run on 64x8x8 tiles (one tile per wavefront)
   if (cn != 6 && tile_opt != HDM_TILE_AUTO && tile_opt != HDM_TILE_64x8x8) return -1;
   *conn = cn;
-  return launch_id(norm_tile(tile_opt), cn);
+  return launch_id(norm_tile(tile_opt), cn, ds);
 }
 
 // Launch shapes.  Pack: at most the resident waves, each walking one
@@ -660,7 +663,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   hdm_opts_init(&opts);
   if (opts_in) opts = *opts_in;
   int conn = 6;
-  const int lid = select_launch(opts.tile, opts.connectivity, &conn);
+  const int lid = select_launch(opts.tile, opts.connectivity, &conn, ds_enabled());
   if (!(opts.robustness >= 0.0 && opts.robustness <= 1.0) || lid < 0 ||
       opts.ptr_kind != HDM_PTR_DEVICE)
     return HDM_E_INVALID;

@@ -179,6 +179,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     if (tid == 0) {
       c.n_gold = c.n_segm = c.n_fp = c.n_fn = 0;
       c.cov[0] = c.cov[1] = 0;
+      c.cov2[0] = c.cov2[1] = 0;
       c.sum_d[0] = c.sum_d[1] = 0;
       c.lo[0] = c.lo[1] = c.lo[2] = 0x7FFFFFFF;
       c.hi[0] = c.hi[1] = c.hi[2] = -1;
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
       c.done_from[0] = c.done_from[1] = 0x7FFFFFFF;
       c.hd_last[0] = c.hd_last[1] = 0;
     }
-    for (int i = tid; i < 2 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
+    for (int i = tid; i < 4 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
     for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) (&c.sums[0][0])[i] = 0;
     for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
       (&c.box[0][0])[i] = i < 3 * kSumSlots ? 0x7FFFFFFF : -1;
@@ -794,7 +795,9 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
 // (hello.cu:940-973), and a pass whose cut is zero queues nothing.  Every
 // entry is first = 1 (input from orig) and check = 1 (count first hits).
 // With one pair the launch also folds the pack partials into the totals.
-template <int TXB, int TY, int TZ, int CONN>
+// DS: the dilation runs two steps per engine iteration, so an empty tile is
+// queued when a neighbour's voxels reach it within two steps.
+template <int TXB, int TY, int TZ, int CONN, bool DS = false>
 __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs, Engine *es,
@@ -854,7 +857,7 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     uint32_t own = 0, reach = 0;
     if (in_box) {
       own = d.tflags[tile];
-      reach = seed_reach<CONN>(d, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
+      reach = seed_reach<CONN, DS>(d, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
         return d.tflags[nb];
       });
     }
@@ -1277,10 +1280,10 @@ __device__ __forceinline__ bool grid_sync_tagged(Engine *es, uint32_t epoch, uin
   return *s_ok != 0;
 }
 
-// LDS table of per-(pair, mask) hit counts of the running iteration
+// LDS table of per-(pair, step, mask) hit counts of the running iteration
 constexpr int kHitTable = 64;
 struct HitTable {
-  uint32_t key[kHitTable];  // pair * 2 + mask, or ~0u
+  uint32_t key[kHitTable];  // pair * 4 + step * 2 + mask, or ~0u
   uint32_t cnt[kHitTable];
 };
 __device__ __forceinline__ void hit_add(HitTable &h, PairCtr *ctrs, uint32_t key, uint32_t v) {
@@ -1292,7 +1295,7 @@ __device__ __forceinline__ void hit_add(HitTable &h, PairCtr *ctrs, uint32_t key
       return;
     }
   }
-  atomicAdd(&ctrs[key >> 1].hits[key & 1][blockIdx.x % kHitSlots],
+  atomicAdd(&ctrs[key >> 2].hits[key & 3][blockIdx.x % kHitSlots],
             (unsigned long long)v);  // table full
 }
 __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
@@ -1300,7 +1303,7 @@ __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
   if (threadIdx.x < kHitTable) {
     const uint32_t k = h.key[threadIdx.x];
     if (k != ~0u) {
-      atomicAdd(&ctrs[k >> 1].hits[k & 1][blockIdx.x % kHitSlots],
+      atomicAdd(&ctrs[k >> 2].hits[k & 3][blockIdx.x % kHitSlots],
                 (unsigned long long)h.cnt[threadIdx.x]);
       h.key[threadIdx.x] = ~0u;
       h.cnt[threadIdx.x] = 0;
@@ -1364,7 +1367,8 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
 template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6,
           bool DS = false>
 __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
-  static_assert(!DS || (FUSED && CONN == 6), "two steps per barrier: fused single pair, 6 element");
+  static_assert(!DS || (TXB == 64 && TY == 8 && TZ == 8 && CONN == 6),
+                "two steps per barrier: 64x8x8 tiles, 6 element");
   static_assert(!FUSED || SINGLE, "the fused launch holds one pair");
   const PairDesc *__restrict__ descs = a.descs;
   PairCtr *ctrs = a.ctrs;
@@ -1587,7 +1591,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     bool pdone[2] = {false, false};
     if (single) {  // stop as soon as both passes are done (uniform decision)
       if (first_read) {
-        const int tp0 = (int)(t - descs[0].t_start);
+        const int tp0 = (int)(t - descs[0].t_start) * (DS ? 2 : 1);  // dilation steps
         pdone[0] = pass_done(descs[0], ctrs[0], 0, tp0);
         pdone[1] = pass_done(descs[0], ctrs[0], 1, tp0);
       } else {  // delivered with the release
@@ -1627,7 +1631,8 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         if (!SINGLE && live && entry_gen(ent) != (v.gen & kGenMask)) live = false;  // stale
         // a pass whose cut was reached stops (hello.cu:1748-1759)
         const unsigned long long cut_m = m ? v.cut[1] : v.cut[0];
-        if (live && (SINGLE ? pdone[m] : (cut_m == 0 || dfrom <= (int)tp))) live = false;
+        if (live && (SINGLE ? pdone[m] : (cut_m == 0 || dfrom <= (DS ? 2 : 1) * (int)tp)))
+          live = false;
         if (live && (DS ? 2 * tp + 2 : tp + 1) >= (uint32_t)v.hist_len) {
           st_sc(&ctrs[p].status, -8);
           live = false;
@@ -2046,9 +2051,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           if (SINGLE)
             atomicAdd(&s_cnt[m], cnt);
           else
-            hit_add(ht, ctrs, (uint32_t)p * 2 + m, cnt);
+            hit_add(ht, ctrs, (uint32_t)p * 4 + m, cnt);
         }
-        if (DS && live && r == 0 && cnt2) atomicAdd(&s_cnt[4 + m], cnt2);
+        if (DS && live && r == 0 && cnt2) {
+          if (SINGLE)
+            atomicAdd(&s_cnt[4 + m], cnt2);
+          else
+            hit_add(ht, ctrs, (uint32_t)p * 4 + 2 + m, cnt2);
+        }
         // -- outputs
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
@@ -2166,7 +2176,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     auto fold = [&]() {
       // kHitSlots lanes per (pair, mask): each loads one cumulative slot, the
       // group sums them; the increase over the last fold is this iteration's
-      // histogram bin
+      // histogram bin.  Two steps per iteration: the counters of each step
+      // give the bins 2tp+1 and 2tp+2 (cov holds both steps, cov2 the
+      // second's part), folded in order up to the one that reaches the cut.
+      constexpr int kSteps = DS ? 2 : 1;
       const int s = threadIdx.x % kHitSlots;
       for (int base = 0; base < 2 * npairs; base += kWgThreads / kHitSlots) {
         const int i = base + threadIdx.x / kHitSlots;
@@ -2177,22 +2190,52 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           const uint32_t ts = descs[pp].t_start;
           act = tdone >= ts;
           tpp = (int)(tdone - ts);
-          if (act && ld_sc(&ctrs[pp].done_from[mm]) <= tpp) act = false;
+          if (act && ld_sc(&ctrs[pp].done_from[mm]) <= kSteps * tpp) act = false;
         }
         unsigned long long v = act ? ld_sc(&ctrs[pp].hits[mm][s]) : 0ull;
+        unsigned long long v2 = (DS && act) ? ld_sc(&ctrs[pp].hits[2 + mm][s]) : 0ull;
 #pragma unroll
-        for (int off = kHitSlots / 2; off >= 1; off >>= 1) v += shfl64(v, (threadIdx.x & 63) ^ off);
+        for (int off = kHitSlots / 2; off >= 1; off >>= 1) {
+          v += shfl64(v, (threadIdx.x & 63) ^ off);
+          if constexpr (DS) v2 += shfl64(v2, (threadIdx.x & 63) ^ off);
+        }
         if (!act || s != 0) continue;
         PairCtr &cc = ctrs[pp];
-        const unsigned long long before = ld_sc(&cc.cov[mm]);
-        if (v == before) continue;
         const PairDesc &dd = descs[pp];
-        if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = (uint32_t)(v - before);
-        st_sc(&cc.cov[mm], v);
-        st_sc(&cc.sum_d[mm], ld_sc(&cc.sum_d[mm]) + (v - before) * (unsigned long long)(tpp + 1));
-        st_sc(&cc.hd_last[mm], (uint32_t)(tpp + 1));
         const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
-        if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
+        if constexpr (DS) {
+          unsigned long long cov = ld_sc(&cc.cov[mm]);
+          const unsigned long long c2 = ld_sc(&cc.cov2[mm]);
+          const unsigned long long bin[2] = {v - (cov - c2), v2 - c2};
+          if (bin[0] == 0 && bin[1] == 0) continue;
+          unsigned long long sum = ld_sc(&cc.sum_d[mm]);
+          uint32_t hd = ld_sc(&cc.hd_last[mm]);
+          int df = 0;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const int it = 2 * tpp + 1 + k;
+            if (bin[k] == 0 || df) continue;
+            if (it < dd.hist_len) dd.hist[mm][it] = (uint32_t)bin[k];
+            const unsigned long long before = cov;
+            cov += bin[k];
+            sum += bin[k] * (unsigned long long)it;
+            hd = (uint32_t)it;
+            if (before < cut && cov >= cut) df = it;
+          }
+          st_sc(&cc.cov[mm], cov);
+          st_sc(&cc.cov2[mm], v2);
+          st_sc(&cc.sum_d[mm], sum);
+          st_sc(&cc.hd_last[mm], hd);
+          if (df) st_sc(&cc.done_from[mm], df);
+        } else {
+          const unsigned long long before = ld_sc(&cc.cov[mm]);
+          if (v == before) continue;
+          if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = (uint32_t)(v - before);
+          st_sc(&cc.cov[mm], v);
+          st_sc(&cc.sum_d[mm], ld_sc(&cc.sum_d[mm]) + (v - before) * (unsigned long long)(tpp + 1));
+          st_sc(&cc.hd_last[mm], (uint32_t)(tpp + 1));
+          if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
+        }
       }
       if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 2) % 4][threadIdx.x], 0u);
       if (threadIdx.x == 0) {
@@ -2227,7 +2270,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       if (threadIdx.x == 0) a.outs[0] = finalize_pair(a.desc, ctrs[0], t, err, DS ? 2 : 1);
     } else {
       for (int p = threadIdx.x; p < npairs; p += blockDim.x)
-        a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err);
+        a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err, DS ? 2 : 1);
     }
     if (threadIdx.x == 0) {
       // entries left for a next iteration (one pair: the master's count,
@@ -2258,9 +2301,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
   }
 }
 
-template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6>
+template <int TXB, int TY, int TZ, bool SINGLE, bool FUSED = false, int CONN = 6,
+          bool DS = false>
 __global__ __launch_bounds__(kWgThreads) void dilate_kernel(DilateArgs a) {
-  dilate_body<TXB, TY, TZ, SINGLE, FUSED, CONN>(a);
+  dilate_body<TXB, TY, TZ, SINGLE, FUSED, CONN, DS>(a);
 }
 
 // The fused launch with two dilation steps per barrier.  Its step needs
@@ -2277,22 +2321,22 @@ hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
   hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw);
   return hipGetLastError();
 }
-template <int TXB, int TY, int TZ, int CONN>
+template <int TXB, int TY, int TZ, int CONN, bool DS>
 hipError_t launch_seed(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
                        PairCtr *c, int np, Engine *es, uint64_t *q, uint32_t qcap, uint64_t tt,
                        int p0) {
-  hipLaunchKernelGGL((seed_kernel<TXB, TY, TZ, CONN>), grid, dim3(kWgThreads), 0, s, d, ix, c,
-                     np, es, q, qcap, tt, p0);
+  hipLaunchKernelGGL((seed_kernel<TXB, TY, TZ, CONN, DS>), grid, dim3(kWgThreads), 0, s, d, ix,
+                     c, np, es, q, qcap, tt, p0);
   return hipGetLastError();
 }
-template <int TXB, int TY, int TZ, int CONN>
+template <int TXB, int TY, int TZ, int CONN, bool DS>
 hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
   if (a.npairs == 1)
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, false, CONN>), grid, dim3(kWgThreads),
-                       0, s, a);
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, false, CONN, DS>), grid,
+                       dim3(kWgThreads), 0, s, a);
   else
-    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false, false, CONN>), grid, dim3(kWgThreads),
-                       0, s, a);
+    hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, false, false, CONN, DS>), grid,
+                       dim3(kWgThreads), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2308,35 +2352,41 @@ hipError_t launch_fused_ds(dim3 grid, hipStream_t s, const DilateArgs &a) {
   return hipGetLastError();
 }
 
-template <int TXB, int TY, int TZ, int CONN = 6>
+// DS: every launch of the entry runs two dilation steps per engine iteration
+// (its fused launch is dilate_ds_kernel)
+template <int TXB, int TY, int TZ, int CONN = 6, bool DS = false>
 TileLaunch make_launch() {
   constexpr bool kDS = TXB == 64 && TY == 8 && TZ == 8 && CONN == 6;
+  static_assert(!DS || kDS, "two steps per barrier: 64x8x8 tiles, 6 element");
   return TileLaunch{&launch_pack<TXB, TY, TZ>,
-                    &launch_seed<TXB, TY, TZ, CONN>,
-                    &launch_dilate<TXB, TY, TZ, CONN>,
-                    &launch_fused<TXB, TY, TZ, CONN>,
+                    &launch_seed<TXB, TY, TZ, CONN, DS>,
+                    &launch_dilate<TXB, TY, TZ, CONN, DS>,
+                    DS ? &launch_fused_ds : &launch_fused<TXB, TY, TZ, CONN>,
                     kDS ? &launch_fused_ds : nullptr,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN>,
-                    (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN, DS>,
+                    (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN, DS>,
+                    DS ? (const void *)&dilate_ds_kernel
+                       : (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
                     kDS ? (const void *)&dilate_ds_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     TXB,
                     TY,
                     TZ,
-                    CONN};
+                    CONN,
+                    DS ? 2 : 1};
 }
 
 }  // namespace hdm
 
 // table indexed by launch id: HDM_TILE_* (1..4) with the 6 element, then the
-// 64x8x8 tile with the 18 (5) and the 26 (6) element
+// 64x8x8 tile with the 18 (5) and the 26 (6) element, and the 64x8x8 tile with
+// the 6 element at two dilation steps per engine iteration (7)
 extern "C" const hdm::TileLaunch *hdm_tile_launch(int kind) {
   static const hdm::TileLaunch table[hdm::kLaunchIds] = {
       hdm::make_launch<64, 8, 8>(),     hdm::make_launch<64, 8, 8>(),
       hdm::make_launch<64, 4, 4>(),     hdm::make_launch<64, 8, 4>(),
       hdm::make_launch<32, 8, 8>(),     hdm::make_launch<64, 8, 8, 18>(),
-      hdm::make_launch<64, 8, 8, 26>()};
+      hdm::make_launch<64, 8, 8, 26>(), hdm::make_launch<64, 8, 8, 6, true>()};
   if (kind < 0 || kind >= hdm::kLaunchIds) return nullptr;
   return &table[kind];
 }

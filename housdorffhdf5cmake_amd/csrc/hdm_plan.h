@@ -63,13 +63,17 @@ struct TileLaunch {
   const void *fused_ds_fn;
   const void *pack_fn;
   int txb, ty, tz;
-  int conn;  // structuring element of the seed and dilation kernels
+  int conn;   // structuring element of the seed and dilation kernels
+  int steps;  // dilation steps per engine iteration of the seed and dilation kernels
 };
 
 // launch ids (hdm_tile_launch): the HDM_TILE_* kinds with the 6 element, then
-// the 64x8x8 tile with the 18 and the 26 element
-constexpr int kLaunchIds = 7;
-inline int launch_id(int tile_kind, int conn) {
+// the 64x8x8 tile with the 18 and the 26 element, then the 64x8x8 tile with the
+// 6 element at two dilation steps per engine iteration (ds)
+constexpr int kLaunchIds = 8;
+constexpr int kLaunchDs = 7;
+inline int launch_id(int tile_kind, int conn, bool ds = false) {
+  if (ds && conn == 6 && tile_kind == HDM_TILE_64x8x8) return kLaunchDs;
   return conn == 18 ? 5 : conn == 26 ? 6 : tile_kind;
 }
 

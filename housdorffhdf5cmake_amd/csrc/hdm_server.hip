@@ -165,7 +165,7 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   hdm_server_opts_init(&o);
   if (oin) o = *oin;
   int conn = 6;
-  const int lid = select_launch(o.tile, o.connectivity, &conn);
+  const int lid = select_launch(o.tile, o.connectivity, &conn, ds_enabled());
   if (o.max_slots < 2 || o.max_slots > 4096 || o.slice_iters < 1 || lid < 0 ||
       !(o.robustness >= 0.0 && o.robustness <= 1.0) ||
       o.max_nx < 1 || o.max_ny < 1 || o.max_nz < 1 || o.hbm_budget < (1u << 20))
@@ -352,7 +352,8 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   a.descs = s->P.descs;
   a.ctrs = s->P.ctrs;
   a.npairs = nslots;
-  a.max_iters = s->o.slice_iters;
+  // engine iterations of slice_iters dilation steps
+  a.max_iters = (s->o.slice_iters + s->tl->steps - 1) / s->tl->steps;
   a.es = s->P.engine;
   a.q = s->P.q;
   a.qcap = s->qcap;

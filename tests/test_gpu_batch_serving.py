@@ -47,6 +47,31 @@ def test_batch_equals_single_and_oracle():
         assert rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
 
 
+@pytest.mark.parametrize("robust", [0.5, 0.95])
+def test_batch_robust_cut_equals_single_and_oracle(robust):
+    """A batch runs two dilation steps per grid barrier (64x8x8 tiles, 6
+    element): each step's first hits are their own histogram bin, and a pass
+    stops at the bin whose covered count reaches ceil(r * n)
+    (hello.cu:1748-1759) -- also when that bin is the first of the two, whose
+    second step's hits then count for nothing.  One-pair batches run the
+    single-pair engine loop."""
+    vols = mixed_pairs(6, seed=29)
+    alone = [hdm.hausdorff(g, s, robustness=robust) for g, s in vols]
+    b = hdm.Batch(vols, robustness=robust, graph=False)
+    b.run()
+    out = b.sync()
+    b.close()
+    for o, a in zip(out, alone):
+        assert same(o, a)
+    one = hdm.Batch(vols[2:3], robustness=robust, graph=False)
+    one.run()
+    assert same(one.sync()[0], alone[2])
+    one.close()
+    for (g, s), a in zip(vols[:3], alone[:3]):
+        rc, want, _, _, _ = oracle.run(g.cpu().numpy(), s.cpu().numpy(), robustness=robust)
+        assert rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
+
+
 def test_graph_replays_are_identical_with_eager_work_between():
     vols = mixed_pairs(8, seed=5)
     alone = [hdm.hausdorff(g, s) for g, s in vols]
