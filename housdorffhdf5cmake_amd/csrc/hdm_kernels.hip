@@ -329,7 +329,8 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   const int lane = threadIdx.x & 63;
   const int sub = lane / R, r = lane % R;
   const int yl = r % TY, zl = r / TY;
-  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg +
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
   constexpr uint64_t kStep = (uint64_t)TPW * U;
   const uint64_t per = ((total_tiles + nw - 1) / nw + kStep - 1) / kStep * kStep;
@@ -338,6 +339,15 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   PackAcc acc;
   acc_clear(acc);
   int ap = -1;  // pair the accumulator belongs to (wave-uniform)
+  // the pair of the wave's running position (wave-uniform): one search at the
+  // start of its range, then a step past each pair boundary as the range is
+  // walked in order; only a lane beyond the running pair's end searches
+  int cp = 0;
+  uint64_t ce = total_tiles;  // end of pair cp
+  if (npairs > 1 && w0 < w1) {
+    cp = find_pair(npairs, w0, [&](int i) { return idx[i].tile_base; });
+    ce = cp + 1 < npairs ? idx[cp + 1].tile_base : total_tiles;
+  }
   for (uint64_t base = w0; base < w1; base += kStep) {
     bool inU[U], okU[U];
     int pU[U], txU[U], tyU[U], tzU[U], cntU[U];
@@ -349,8 +359,15 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const uint64_t g = base + (uint64_t)u * TPW + sub;
       inU[u] = g < w1;
       pU[u] = 0;
-      if (npairs > 1)
-        pU[u] = find_pair(npairs, inU[u] ? g : w1 - 1, [&](int i) { return idx[i].tile_base; });
+      if (npairs > 1) {
+        const uint64_t g0 = base + (uint64_t)u * TPW;  // the step's first tile
+        while (g0 < w1 && g0 >= ce) {
+          ++cp;
+          ce = cp + 1 < npairs ? idx[cp + 1].tile_base : total_tiles;
+        }
+        pU[u] = cp;
+        if (inU[u] && g >= ce) pU[u] = find_pair(npairs, g, [&](int i) { return idx[i].tile_base; });
+      }
       aU[u] = bU[u] = 0;
       okU[u] = false;
       tileU[u] = 0;
@@ -1612,6 +1629,15 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     // one step of queue entry ent (live: a real entry) by the R lanes of its
     // group; pushes the entries it produces
     auto step_tile = [&](uint64_t ent, bool live) {
+        if constexpr (TPW == 1 && !SINGLE) {
+          // one tile per wave: every lane holds the same entry, so the tile's
+          // coordinates and its pair's fields live in scalar registers and
+          // the descriptor is read with scalar loads (one pair keeps its
+          // fields in registers anyway, and measured no faster with it)
+          ent = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(ent >> 32))
+                 << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)ent);
+        }
         const uint32_t tile = (uint32_t)ent;
         const int p = SINGLE ? 0 : (int)((ent >> 32) & 0xFFFF);
         const int m = (int)((ent >> 48) & 1);
@@ -2145,12 +2171,29 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       gfirst = min(L, s_fs[1]);
     }
     // global entries beyond the free slots (one pair) or all of them
-    // (several pairs), round robin over the waves
-    for (uint32_t base = gfirst + gwave * step; base < L; base += nwave * step) {
-      for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
-        const uint32_t ei = base + k + sub;
-        const bool live = ei < L;
-        step_tile(live ? global_entry(ei) : 0ull, live);
+    // (several pairs), round robin over the waves: chunks of step entries
+    // at base, base + nwave * step, ...
+    if constexpr (SINGLE) {
+      for (uint32_t base = gfirst + gwave * step; base < L; base += nwave * step) {
+        for (uint32_t k = 0; k < step && base + k < L; k += TPW) {
+          const uint32_t ei = base + k + sub;
+          const bool live = ei < L;
+          step_tile(live ? global_entry(ei) : 0ull, live);
+        }
+      }
+    } else {  // the same order, each entry loaded while the one before steps
+      uint32_t base = gfirst + gwave * step, k = 0;
+      uint64_t nxt = (base < L && base + sub < L) ? global_entry(base + sub) : 0ull;
+      while (base < L) {
+        const bool live = base + k + sub < L;
+        const uint64_t cur = nxt;
+        k += TPW;
+        if (k >= step || base + k >= L) base += nwave * step, k = 0;
+        if (base < L) {
+          const uint32_t e2 = base + k + sub;
+          nxt = e2 < L ? global_entry(e2) : 0ull;
+        }
+        step_tile(cur, live);
       }
     }
     if (tr) {
