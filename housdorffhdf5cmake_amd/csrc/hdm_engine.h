@@ -33,6 +33,13 @@ constexpr int kHitSlots = 16;      // per (pair, mask) per-iteration hit counter
 constexpr int kSumSlots = 64;      // per (pair, field) partial voxel counts
 constexpr int kMaxGrid = 4096;     // persistent workgroups (arrival flags)
 constexpr int kLineWords = 32;     // one 128-byte line
+// x-face words per tile: bit y + TY z of word [side] is the voxel of row (y, z)
+// in column 0 (side 0) or TXB-1 (side 1); 64x8x8 tiles also keep the second
+// column from each face in words [2 + side] (columns 1 and TXB-2), the part
+// of a neighbour that two dilation steps reach
+constexpr int xf_words(int txb, int ty, int tz) {
+  return (txb == 64 && ty == 8 && tz == 8) ? 4 : 2;
+}
 constexpr int kReleaseCopies = 32; // replicas of the barrier release line
 
 constexpr uint32_t kFull = 0x80000000u;

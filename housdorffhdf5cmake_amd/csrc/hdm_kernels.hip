@@ -324,6 +324,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
+  constexpr int kXf = xf_words(TXB, TY, TZ);
   constexpr int U = 2;            // wave steps whose loads are all issued first
                                   // (the 128-byte path below pairs steps 0 and 1)
   const int lane = threadIdx.x & 63;
@@ -471,20 +472,28 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         const bool fzl = group_bits<R>(__ballot(zl == 0 && w != 0), lane) != 0;
         const bool fzh = group_bits<R>(__ballot(zl == TZ - 1 && w != 0), lane) != 0;
         // the two outermost layers toward each face (reach within two steps;
-        // read only by the two-steps-per-barrier seed, which runs on 64x8x8)
+        // read only by the two-steps-per-barrier seed and steps, which run on
+        // 64x8x8): the second column from each x face as face words, and flags
         constexpr bool kDeep = TXB == 64 && TY == 8 && TZ == 8;
         bool gxl = false, gxh = false, gyl = false, gyh = false, gzl = false, gzh = false;
+        uint64_t fx2l = 0, fx2h = 0;
         if constexpr (kDeep) {
-          gxl = group_bits<R>(__ballot(rowok && (w & 3)), lane) != 0;
-          gxh = group_bits<R>(__ballot(rowok && ((w >> (TXB - 2)) & 3)), lane) != 0;
+          fx2l = group_bits<R>(__ballot(rowok && ((w >> 1) & 1)), lane);
+          fx2h = group_bits<R>(__ballot(rowok && ((w >> (TXB - 2)) & 1)), lane);
+          gxl = (fxl | fx2l) != 0;
+          gxh = (fxh | fx2h) != 0;
           gyl = group_bits<R>(__ballot(yl <= 1 && w != 0), lane) != 0;
           gyh = group_bits<R>(__ballot(yl >= TY - 2 && w != 0), lane) != 0;
           gzl = group_bits<R>(__ballot(zl <= 1 && w != 0), lane) != 0;
           gzh = group_bits<R>(__ballot(zl >= TZ - 2 && w != 0), lane) != 0;
         }
         if (in && r == 0) {
-          put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile, fxl);
-          put<COH>(d.xf_orig[m] + 2 * (uint64_t)tile + 1, fxh);
+          put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile, fxl);
+          put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile + 1, fxh);
+          if constexpr (kDeep) {
+            put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile + 2, fx2l);
+            put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile + 3, fx2h);
+          }
         }
         flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)(fxl != 0) << 2 |
                   (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
@@ -1401,6 +1410,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;
+  constexpr int kXf = xf_words(TXB, TY, TZ);
   __shared__ LocalQueue lq;
   __shared__ HitTable ht;
   __shared__ uint32_t s_pre[kSubQ + 1];
@@ -1698,11 +1708,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           // second step's output is the union over the L1 ball of radius 2,
           //   D2 = x2(own) | x1(rows y+-1, z+-1) | rows y+-2, z+-2 | rows (y+-1, z+-1)
           // and the first step's D1 = x1(own) | rows y+-1, z+-1 gives the hits
-          // of the first step.  Beyond the tile: the x neighbours' rows at this
-          // lane's (y, z) (two boundary bits), rows at depth 1 and 2 of the y
+          // of the first step.  Beyond the tile: rows at depth 1 and 2 of the y
           // and z neighbours (face lanes), the yz-diagonal row (corner lanes),
-          // and the x-face words of the x-side tiles at (y, z), (y+-1), (z+-1)
-          // for the one-bit x halo of the rows y+-1, z+-1 (8x8 bit planes).
+          // the x-face words of the x-side tiles at (y, z), (y+-1), (z+-1) for
+          // the one-bit x halo of the rows y+-1, z+-1 (8x8 bit planes), and the
+          // second-column face words of the x neighbours at (y, z) for the
+          // two-bit x halo of the row itself.
           // Lane j < 27 reads the state of neighbour j = (ox+1) + 3(oy+1) +
           // 9(oz+1) of the 18 that two steps can reach (no corners).
           static_assert(TXB == 64 && TY == 8 && TZ == 8 && CONN == 6, "64x8x8, 6 element");
@@ -1717,23 +1728,18 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
               nb_dir = (uint32_t)((int)tile + jox + joy * v.gx + joz * v.gx * v.gy);
           }
           nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
-          uint64_t fwo = 0, fwb = 0;  // face word of x-side neighbour j toward this tile
+          // face words of x-side neighbour j toward this tile: the outermost
+          // column, and for the neighbours at (y, z) (lanes 12, 14) the second
+          uint64_t fwo = 0, fwb = 0, fw2o = 0, fw2b = 0;
+          const bool x_only = joy == 0 && joz == 0;
           if (nb_dir != kNoTile && jox != 0) {
-            const uint64_t o = 2 * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
+            const uint64_t o = kXf * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
             fwo = ldg(xfo + o);
             fwb = ld_sc(xfr + o);
-          }
-          // the x neighbours' rows at (y, z)
-          const uint32_t nbxl = (rowok && tx > v.b.lo[0]) ? tile - 1 : kNoTile;
-          const uint32_t nbxh = (rowok && tx < v.b.hi[0]) ? tile + 1 : kNoTile;
-          Row xlo = 0, xlb = 0, xho = 0, xhb = 0;
-          if (nbxl != kNoTile) {
-            xlo = ldg(orig_m + (uint64_t)nbxl * R + r);
-            xlb = ld_sc(bufr + (uint64_t)nbxl * R + r);
-          }
-          if (nbxh != kNoTile) {
-            xho = ldg(orig_m + (uint64_t)nbxh * R + r);
-            xhb = ld_sc(bufr + (uint64_t)nbxh * R + r);
+            if (x_only) {
+              fw2o = ldg(xfo + o + 2);
+              fw2b = ld_sc(xfr + o + 2);
+            }
           }
           // rows beyond the tile in y and z, depth 1 and 2, and the yz diagonal
           const int oy = yl == 0 ? -1 : yl == TY - 1 ? 1 : 0;
@@ -1770,15 +1776,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           };
           const uint32_t sty = shfl32(nst, 13 + 3 * oy), stz = shfl32(nst, 13 + 9 * oz);
           const uint32_t std_ = shfl32(nst, 13 + 3 * oy + 9 * oz);
-          const uint32_t stxl = shfl32(nst, 12), stxh = shfl32(nst, 14);
           const Row ey1 = pick(nby, sty, y1o, y1b, vfull);
           const Row ey2 = y2 ? pick(nby, sty, y2o, y2b, vfull) : Row(0);
           const Row ez1 = pick(nbz, stz, z1o, z1b, vfull);
           const Row ez2 = z2 ? pick(nbz, stz, z2o, z2b, vfull) : Row(0);
           const Row ed = pick(nbd, std_, edo, edb, vfull);
-          const Row xl = pick(nbxl, stxl, xlo, xlb, ~Row(0));  // never the x tail tile
-          const Row xh = pick(nbxh, stxh, xho, xhb, valid_bits<Row>(tx + 1, v.nx));
-          uint64_t fw = 0;  // face word, a full tile's face has a bit for every row it has
+          // face words; a full tile's face has a bit for every row it has, and
+          // so has its second column unless it is a +x tail tile one column wide
+          uint64_t fw = 0, fw2 = 0;
           if (nb_dir != kNoTile && jox != 0) {
             if (nst & kFull) {
               const int yy = v.ny - (ty + joy) * TY, zz = v.nz - (tz + joz) * TZ;
@@ -1787,8 +1792,11 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
 #pragma unroll
               for (int k = 0; k < TZ; ++k) rows |= k < zz ? ym << (TY * k) : 0ull;
               fw = rows;
+              fw2 = (x_only && (jox < 0 || v.nx - (tx + 1) * TXB >= 2)) ? rows : 0ull;
             } else {
-              fw = (nst != 0 && nst - 1 < tp) ? fwb : fwo;
+              const bool from_buf = nst != 0 && nst - 1 < tp;
+              fw = from_buf ? fwb : fwo;
+              fw2 = from_buf ? fw2b : fw2o;
             }
           }
           // -- the rows around (y, z): shuffles inside the tile, loaded rows
@@ -1832,11 +1840,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           // -- x halo of the rows y+-1, z+-1: the x-side face planes (lanes
           // 12, 9, 15, 3, 21 at x-1; 14, 11, 17, 5, 23 at x+1), shifted by one
           // in y or z
-          auto plane_of = [&](int jj) -> uint64_t {
-            return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)fw, jj)) |
-                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(fw >> 32), jj)
+          auto lane_word = [&](uint64_t w, int jj) -> uint64_t {
+            return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w, jj)) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), jj)
                     << 32);
           };
+          auto plane_of = [&](int jj) -> uint64_t { return lane_word(fw, jj); };
           auto cross = [&](int side) -> uint64_t {  // side 0: x-1, 2: x+1
             constexpr uint64_t kY0 = 0x0101010101010101ull, kY7 = 0x8080808080808080ull;
             const uint64_t c = plane_of(side + 12), ym = plane_of(side + 9), yp = plane_of(side + 15);
@@ -1845,11 +1854,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
                    (c >> 8) | (zp << 56) | (c << 8) | (zm >> 56);
           };
           const uint64_t clo = cross(0), chi = cross(2);
-          const Row x1own = own | (Row)(own << 1) | (Row)(own >> 1) | (Row)(xl >> 63) |
-                            (Row)(xh << 63);
-          const Row x2own = x1own | (Row)(own << 2) | (Row)(own >> 2) | (Row)((xl >> 62) & 1) |
-                            (Row)((xl >> 63) << 1) | (Row)(((xh >> 1) & 1) << 63) |
-                            (Row)((xh & 1) << 62);
+          // the row's x halo: columns 63 (bl1) and 62 (bl2) of the x-1
+          // neighbour, 0 (bh1) and 1 (bh2) of the x+1 neighbour
+          const Row bl1 = (Row)((plane_of(12) >> r) & 1), bh1 = (Row)((plane_of(14) >> r) & 1);
+          const Row bl2 = (Row)((lane_word(fw2, 12) >> r) & 1);
+          const Row bh2 = (Row)((lane_word(fw2, 14) >> r) & 1);
+          const Row x1own = own | (Row)(own << 1) | (Row)(own >> 1) | bl1 | (Row)(bh1 << 63);
+          const Row x2own = x1own | (Row)(own << 2) | (Row)(own >> 2) | bl2 | (Row)(bl1 << 1) |
+                            (Row)(bh2 << 63) | (Row)(bh1 << 62);
           const Row x1f = f1 | (Row)(f1 << 1) | (Row)(f1 >> 1) | (Row)((clo >> r) & 1) |
                           (Row)(((chi >> r) & 1) << 63);
           d1 = (x1own | f1) & vb;
@@ -1869,12 +1881,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
           uint64_t fxl_o = 0, fxl_b = 0, fxh_o = 0, fxh_b = 0;
           if (nb_xl != kNoTile) {
-            fxl_o = ldg(xfo + 2 * (uint64_t)nb_xl + 1);
-            fxl_b = ld_sc(xfr + 2 * (uint64_t)nb_xl + 1);
+            fxl_o = ldg(xfo + kXf * (uint64_t)nb_xl + 1);
+            fxl_b = ld_sc(xfr + kXf * (uint64_t)nb_xl + 1);
           }
           if (nb_xh != kNoTile) {
-            fxh_o = ldg(xfo + 2 * (uint64_t)nb_xh);
-            fxh_b = ld_sc(xfr + 2 * (uint64_t)nb_xh);
+            fxh_o = ldg(xfo + kXf * (uint64_t)nb_xh);
+            fxh_b = ld_sc(xfr + kXf * (uint64_t)nb_xh);
           }
           // y / z face rows from the neighbour tiles, both candidate sources
           Row fo[4] = {0, 0, 0, 0}, fb[4] = {0, 0, 0, 0};
@@ -1943,7 +1955,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           nst = nb_dir != kNoTile ? ld_sc(st_m + nb_dir) : 0u;
           uint64_t fwo = 0, fwb = 0;  // face word of neighbour j toward this tile
           if (nb_dir != kNoTile && jox != 0) {
-            const uint64_t o = 2 * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
+            const uint64_t o = kXf * (uint64_t)nb_dir + (jox < 0 ? 1 : 0);
             fwo = ldg(xfo + o);
             fwb = ld_sc(xfr + o);
           }
@@ -2089,9 +2101,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         const uint64_t oxl = group_bits<R>(__ballot(dl & 1), lane);
         const uint64_t oxh = group_bits<R>(__ballot((dl >> (TXB - 1)) & 1), lane);
         const uint64_t oany = group_bits<R>(__ballot(dl != 0), lane);
-        // DS: the two outermost columns toward -x / +x
-        const uint64_t ox2l = DS ? group_bits<R>(__ballot((dl & 3) != 0), lane) : 0ull;
-        const uint64_t ox2h = DS ? group_bits<R>(__ballot(((dl >> (TXB - 2)) & 3) != 0), lane) : 0ull;
+        // DS: the second column from each x face, and either of the two outermost
+        const uint64_t o2l = DS ? group_bits<R>(__ballot((dl >> 1) & 1), lane) : 0ull;
+        const uint64_t o2h = DS ? group_bits<R>(__ballot((dl >> (TXB - 2)) & 1), lane) : 0ull;
+        const uint64_t ox2l = oxl | o2l, ox2h = oxh | o2h;
         const bool oyl = group_bits<R>(__ballot(yl == 0 && dl != 0), lane) != 0;
         const bool oyh = group_bits<R>(__ballot(yl == TY - 1 && dl != 0), lane) != 0;
         const bool ozl = group_bits<R>(__ballot(zl == 0 && dl != 0), lane) != 0;
@@ -2104,8 +2117,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           } else {
             st_sc(bufw + ro, dl);
             if (r == 0) {
-              st_sc(xfw + 2 * (uint64_t)tile, oxl);
-              st_sc(xfw + 2 * (uint64_t)tile + 1, oxh);
+              st_sc(xfw + kXf * (uint64_t)tile, oxl);
+              st_sc(xfw + kXf * (uint64_t)tile + 1, oxh);
+              if constexpr (DS) {
+                st_sc(xfw + kXf * (uint64_t)tile + 2, o2l);
+                st_sc(xfw + kXf * (uint64_t)tile + 3, o2h);
+              }
             }
             // lane kSelf re-queues the tile while it is not full; lanes
             // below it activate idle neighbours the output reaches

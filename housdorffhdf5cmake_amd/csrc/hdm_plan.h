@@ -115,7 +115,7 @@ inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
   g.row_words = (uint64_t)g.wxn * ny * nz;
   g.hist_len = nx + ny + nz + 2;
   g.buf_bytes = align256(g.words * (ts.txb / 8));
-  g.xf_bytes = align256((uint64_t)g.ntiles * 16);
+  g.xf_bytes = align256((uint64_t)g.ntiles * 8 * xf_words(ts.txb, ts.ty, ts.tz));
   g.st_bytes = align256((uint64_t)g.ntiles * 4);
   g.tf_bytes = align256((uint64_t)g.ntiles * 4);
   g.hist_bytes = align256((uint64_t)g.hist_len * 4);
