@@ -317,6 +317,7 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
             "speedup": round(t_plain / t_batch, 2),
             "batched_mask_voxels_per_s": round(2 * nvox / t_batch, 1),
             "batched_device_ms_median": round(statistics.median(ms), 3),
+            "queue_total": int(out[0]["queue_total"]), "queue_max": int(out[0]["queue_max"]),
             "tile": best["tile"], "chunk": best["chunk"], "graph": True,
             "identical_to_plain": bool(same),
             "autotune": [{k: e[k] for k in ("tile", "chunk", "ms", "status")}

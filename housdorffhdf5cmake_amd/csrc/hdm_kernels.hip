@@ -1636,6 +1636,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     }
     uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
     uint32_t *qout_len = &es->qlen[(t + 1) % 4][kq];
+    // several pairs: a step's claims of idle neighbours (compare-and-swap on
+    // their state) are pushed with the next step's entries, so the swap's
+    // round trip overlaps that step's loads; claim_old: the state the swap
+    // found (0: this lane won the tile), claim_ent: the entry (0: no claim)
+    uint32_t claim_old = 0;
+    uint64_t claim_ent = 0;
     // one step of queue entry ent (live: a real entry) by the R lanes of its
     // group; pushes the entries it produces
     auto step_tile = [&](uint64_t ent, bool live) {
@@ -2111,6 +2117,8 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         const bool ozh = group_bits<R>(__ballot(zl == TZ - 1 && dl != 0), lane) != 0;
         bool want = false;
         uint64_t pent = 0;
+        uint32_t new_old = 0;  // this step's claim (several pairs)
+        uint64_t new_ent = 0;
         if (live) {
           if (full) {
             if (r == 0) or_g(st_m + tile, kFull);
@@ -2154,9 +2162,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
                 const uint64_t zm = joz < 0 ? 0xFFull : joz > 0 ? 0xFFull << 56 : ~0ull;
                 reach = (xs & ym & zm) != 0;
               }
-              if (reach && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
-                want = true;
-                pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
+              if constexpr (SINGLE) {
+                if (reach && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
+                  want = true;
+                  pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
+                }
+              } else if (reach) {
+                new_old = cas_g(st_m + nb_dir, 0u, tp + 2);
+                new_ent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
               }
             }
           }
@@ -2164,13 +2177,22 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         if constexpr (SINGLE)
           push_sticky(want, pent, s_st[(t + 1) & 1], &s_stn[(t + 1) & 1], kStick, lq, Qout,
                       qout_len, lane, &s_cnt[2]);
-        else
-          push_entries(want, pent, lq, Qout, qout_len, lane);
+        else {  // lane kSelf's own entry, lanes below it the claims won by the step before
+          const bool won = claim_ent != 0 && claim_old == 0u;
+          push_entries(want || won, want ? pent : claim_ent, lq, Qout, qout_len, lane);
+          claim_old = new_old;
+          claim_ent = new_ent;
+        }
     };
     // global entry ei of the queue of iteration t
     auto global_entry = [&](uint32_t ei) {
       int sq = 0;
-      while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
+      if constexpr (TPW == 1) {  // ei is wave-uniform: lane k < kSubQ - 1 compares start k + 1
+        const uint32_t pre = s_pre[1 + (lane & (kSubQ - 1))];
+        sq = __popcll(__ballot(lane < kSubQ - 1 && pre <= ei));
+      } else {
+        while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
+      }
       return ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
     };
     uint32_t gfirst = 0;  // global entries below this were taken in round 1
@@ -2212,6 +2234,9 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         }
         step_tile(cur, live);
       }
+      // the claims of the wave's last step
+      push_entries(claim_ent != 0 && claim_old == 0u, claim_ent, lq, Qout, qout_len, lane);
+      claim_ent = 0;
     }
     if (tr) {
       __syncthreads();
