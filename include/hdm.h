@@ -235,7 +235,9 @@ typedef struct hdm_server hdm_server;
 typedef struct hdm_server_opts {
   uint64_t hbm_budget;   /* bytes of device memory for inputs + workspace   */
   int32_t max_slots;     /* requests in flight at once (<= 4096)            */
-  int32_t slice_iters;   /* dilation iterations per persistent launch       */
+  int32_t slice_iters;   /* dilation steps per persistent launch, rounded up
+                            to whole engine iterations (two steps each on
+                            64x8x8 tiles with the 6 element)               */
   int32_t tile;          /* HDM_TILE_*                                      */
   int32_t grid_ctas;     /* 0 = occupancy-derived                           */
   double robustness;
