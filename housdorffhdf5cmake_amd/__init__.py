@@ -13,6 +13,7 @@ Reference-shaped entry point (hello.cu:2025-2027):
 from __future__ import annotations
 
 import ctypes as C
+import struct
 import os
 
 import numpy as np
@@ -54,7 +55,13 @@ class Result(C.Structure):
                 ("queue_total", C.c_uint64), ("queue_max", C.c_uint32), ("tiles", C.c_uint32)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        return dict(zip(_RESULT_NAMES, _RESULT_LAYOUT.unpack_from(self)))
+
+
+# hdm_result as one unpack: 4 int32, 10 uint64, a double, a uint64, 2 uint32
+_RESULT_NAMES = tuple(k for k, _ in Result._fields_)
+_RESULT_LAYOUT = struct.Struct("<4i10QdQ2I")
+assert _RESULT_LAYOUT.size == C.sizeof(Result)
 
 
 class Pair(C.Structure):
