@@ -25,7 +25,8 @@ struct Ctx {
   int dev = -1;
   hipStream_t stream = nullptr;
   int cus = 0;
-  int occ[kLaunchIds] = {};       // resident dilate workgroups per CU, per launch id
+  int occ[kLaunchIds] = {};        // resident dilate workgroups per CU, several pairs
+  int occ_single[kLaunchIds] = {}; // the same for the one-pair dilation kernel
   int occ_pack[kLaunchIds] = {};  // resident pack workgroups per CU
   int occ_fused[kLaunchIds] = {};  // resident fused single-pair workgroups per CU
   int occ_fused_ds[kLaunchIds] = {};  // the same, two dilation steps per barrier
@@ -90,7 +91,7 @@ struct GroupPtrs {
 
 int map_hip(hipError_t e);
 int get_ctx(Ctx **out);
-int resident_grid(const Ctx *c, int tile, int requested);
+int resident_grid(const Ctx *c, int tile, int requested, int npairs);
 uint64_t spin_ticks(const Ctx *c);
 int norm_tile(int tile);
 // launch id (hdm_tile_launch) of a tile option and a connectivity option (0 =
@@ -106,7 +107,7 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
                          uint64_t total_rw, uint64_t total_tiles, uint32_t qcap,
                          const LaunchCfg &cfg, int qstats_len);
 LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
-                     int grid_req, int chunk);
+                     int grid_req, int chunk, int npairs);
 void fill_result(hdm_result *out, const PairOut &o, const EngineHead *es, uint32_t tiles,
                  int iterations);
 

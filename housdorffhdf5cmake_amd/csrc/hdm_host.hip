@@ -91,12 +91,12 @@ int get_ctx(Ctx **out) {
         nb < 1)
       rc = HDM_E_HIP;
     c->occ[k] = nb;
-    if (rc == HDM_OK &&  // both dilation variants must fit the same resident grid
+    if (rc == HDM_OK &&  // the one-pair kernel has its own resident grid
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->dilate_single_fn,
                                                       kWgThreads, 0) != hipSuccess ||
          nb < 1))
       rc = HDM_E_HIP;
-    c->occ[k] = std::min(c->occ[k], nb);
+    c->occ_single[k] = nb;
     if (rc == HDM_OK &&
         (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hdm_tile_launch(k)->pack_fn,
                                                       kWgThreads, 0) != hipSuccess ||
@@ -133,8 +133,11 @@ int get_ctx(Ctx **out) {
   return HDM_OK;
 }
 
-int resident_grid(const Ctx *c, int tile, int requested) {
-  const int maxres = std::min(c->occ[tile] * c->cus, kMaxGrid);
+// npairs selects the dilation kernel the grid is for (launch_dilate runs the
+// one-pair kernel for npairs == 1)
+int resident_grid(const Ctx *c, int tile, int requested, int npairs) {
+  const int occ = npairs == 1 ? c->occ_single[tile] : c->occ[tile];
+  const int maxres = std::min(occ * c->cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
   // every resident workgroup: dense iterations want many waves, and the
   // barrier polls arrivals in batches, so a sparse queue pays little for them
@@ -200,7 +203,7 @@ int select_launch(int tile_opt, int conn_opt, int *conn, bool ds) {
 // contiguous range of tiles, two wave steps each when the grid allows.  Seed:
 // one thread per tile.  Dilation: occupancy-bounded.
 LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_tiles,
-                     int grid_req, int chunk) {
+                     int grid_req, int chunk, int npairs) {
   LaunchCfg L;
   (void)total_rw;
   const uint64_t rows = (uint64_t)tile_shape(tile).rows();
@@ -211,7 +214,7 @@ LaunchCfg launch_cfg(const Ctx *c, int tile, uint64_t total_rw, uint64_t total_t
   L.pack_blocks = (unsigned)std::max<uint64_t>(1, std::min(pack_want, pack_res));
   const uint64_t seed_want = (total_tiles + kWgThreads - 1) / kWgThreads;
   L.seed_blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(seed_want, 8192));
-  L.grid = resident_grid(c, tile, grid_req);
+  L.grid = resident_grid(c, tile, grid_req, npairs);
   L.chunk = std::max(1, chunk);
   L.spin = spin_ticks(c);
   return L;
@@ -428,7 +431,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     }
   }
   const PackIndex pi{0, 0};
-  LaunchCfg cfg = launch_cfg(c, tile, g.row_words, g.ntiles, opts.grid_ctas, opts.chunk);
+  LaunchCfg cfg = launch_cfg(c, tile, g.row_words, g.ntiles, opts.grid_ctas, opts.chunk, 1);
   hipEvent_t ring_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int pair = -1;
   if (opts.timing == 3) {  // a fresh pair of the ring brackets the dominant launch
@@ -682,7 +685,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   b->tile = lid;
   b->tl = hdm_tile_launch(b->tile);
   b->ts = tile_shape(b->tile);
-  b->grid = resident_grid(c, b->tile, opts.grid_ctas);
+  b->grid = resident_grid(c, b->tile, opts.grid_ctas, n);
   std::vector<PairDesc> descs(n);
   std::vector<PackIndex> idx(n);
   uint64_t pair_bytes = 0, qcap = 0;
@@ -704,7 +707,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   }
   b->qcap = (uint32_t)qcap;
   b->qstats_len = maxh;
-  b->cfg = launch_cfg(c, b->tile, b->total_rw, b->total_tiles, opts.grid_ctas, opts.chunk);
+  b->cfg = launch_cfg(c, b->tile, b->total_rw, b->total_tiles, opts.grid_ctas, opts.chunk, n);
   b->grid = b->cfg.grid;
   const CommonLayout L = common_layout(n, qcap, maxh);
   if (hipMalloc(&b->ws, L.total + pair_bytes) != hipSuccess) {

@@ -208,7 +208,7 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   s->slot_gen.assign(o.max_slots, 0);
   s->freed_after.assign(o.max_slots, 0);
   s->slices = 1;
-  s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1);
+  s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1, o.max_slots);  // several pairs
   // every slot starts unused: t_start beyond any engine iteration keeps it out
   // of the per-iteration bookkeeping
   std::vector<PairDesc> d(o.max_slots);
@@ -295,7 +295,7 @@ static int admit(hdm_server *s, Request &r, int p) {
   hipError_t e = hdm_launch_reset(s->s, s->P.descs, &d, &pi, s->P.idx, s->P.ctrs, p, 1,
                                   s->P.engine, 0, s->qcap);
   if (e != hipSuccess) return map_hip(e);
-  const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1);
+  const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1, 1);
   e = s->tl->pack(dim3(lc.pack_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
                   r.geom.ntiles);
   if (e == hipSuccess)  // one pair: the seed pass folds the totals itself
