@@ -106,7 +106,7 @@ struct PairCtr {
   // steps per engine iteration [2k + m] pass m in step k of each iteration
   unsigned long long hits[4][kHitSlots];
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
-  int32_t box[6][kSumSlots];  // pack partials of lo[0..2] (min) and hi[0..2] (max)
+  int32_t box[6][kSumSlots];  // pack partials of lo[0..2] and hi[0..2] (box_enc_*)
 };
 
 struct PairOut {
@@ -119,6 +119,19 @@ struct PairOut {
 
 // The engine fields the host reads after a launch; the dilation kernel writes
 // a copy next to the pair results when it leaves.
+// Pack partials with zero as the empty value, so zeroed memory is a valid
+// start: voxel counts (gold, segm, FP, FN) and the box of non-empty tiles,
+// lo[k] as box_enc_lo (0x7FFFFFFF - lo) and hi[k] as box_enc_hi (hi + 1), both
+// combined by max.
+__host__ __device__ inline int32_t box_enc_lo(int32_t v) { return 0x7FFFFFFF - v; }
+__host__ __device__ inline int32_t box_enc_hi(int32_t v) { return v + 1; }
+__host__ __device__ inline int32_t box_dec_lo(int32_t e) { return 0x7FFFFFFF - e; }
+__host__ __device__ inline int32_t box_dec_hi(int32_t e) { return e - 1; }
+struct PackSums {
+  unsigned long long sums[4][kSumSlots];
+  int32_t box[6][kSumSlots];
+};
+
 struct EngineHead {
   uint32_t t;      // completed engine iterations
   uint32_t error;  // 1: a barrier wait exceeded its limit
@@ -163,6 +176,10 @@ struct Engine {
   // tag(epoch, first global entry) of each workgroup's free wave slots: the
   // master's prefix over workgroups of (slots - entries the workgroup kept)
   unsigned long long fslot[kMaxGrid];
+  // pack partials of the fused single-pair launch, two sets used by turns:
+  // a launch packs into one and zeroes the other for the next launch, so no
+  // barrier has to separate the zeroing from the pack
+  PackSums acc[2];
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

@@ -41,6 +41,9 @@ struct Ctx {
   // the workspace's engine are cleared whenever the base restarts at 0
   uint32_t ebase = 0;
   void *ebase_ws = nullptr;  // the workspace the base belongs to
+  // the set of Engine::acc the next fused launch packs into; each launch
+  // zeroes the other set, and clearing the engine zeroes both
+  int acc_set = 0;
   std::vector<uint64_t> last_trace;  // 8 words per iteration of a traced call
   DevMem trace;
   std::mutex mu;

@@ -509,10 +509,13 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.ebase = c->ebase;
     a.ready = cfg.ready;
     a.ready_seq = cfg.ready_seq;
+    a.acc_set = c->acc_set;
     ds = tl->fused_ds && conn == 6 && ds_enabled();
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
-    if (e == hipSuccess)
+    if (e == hipSuccess) {
       e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds)), s, a);
+      if (e == hipSuccess) c->acc_set ^= 1;  // the launch zeroes the other set first
+    }
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
     // until the result is read, assume the launch used every epoch it could
     c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;

@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     for (int i = tid; i < 4 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
     for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) (&c.sums[0][0])[i] = 0;
     for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
-      (&c.box[0][0])[i] = i < 3 * kSumSlots ? 0x7FFFFFFF : -1;
+      (&c.box[0][0])[i] = 0;
     uint32_t *h0 = write_single ? single.hist[0] : descs[p].hist[0];
     uint32_t *h1 = write_single ? single.hist[1] : descs[p].hist[1];
     const int len = write_single ? single.hist_len : descs[p].hist_len;
@@ -271,20 +271,23 @@ __device__ __forceinline__ void acc_clear(PackAcc &a) {
   a.mn[0] = a.mn[1] = a.mn[2] = 0x7FFFFFFF;
   a.mx[0] = a.mx[1] = a.mx[2] = -1;
 }
-__device__ __forceinline__ void acc_publish(PairCtr &c, const PackAcc &a, int slot) {
+// into the pack partials (sums, box) of a PairCtr or a PackSums
+template <typename S>
+__device__ __forceinline__ void acc_publish(S &c, const PackAcc &a, int slot) {
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     if (a.c[f]) atomicAdd(&c.sums[f][slot], (unsigned long long)a.c[f]);
   if (a.mx[0] >= 0) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      atomicMin(&c.box[k][slot], a.mn[k]);
-      atomicMax(&c.box[3 + k][slot], a.mx[k]);
+      atomicMax(&c.box[k][slot], box_enc_lo(a.mn[k]));
+      atomicMax(&c.box[3 + k][slot], box_enc_hi(a.mx[k]));
     }
   }
 }
 // wave-reduce the accumulator, lane 0 publishes
-__device__ __forceinline__ void acc_flush(PairCtr &c, PackAcc &a, int lane, int slot) {
+template <typename S>
+__device__ __forceinline__ void acc_flush(S &c, PackAcc &a, int lane, int slot) {
   PackAcc r;
 #pragma unroll
   for (int f = 0; f < 4; ++f) r.c[f] = wave_sum(a.c[f], lane);
@@ -320,7 +323,7 @@ template <int TXB, int TY, int TZ, bool COH>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
-                                           uint16_t (*s_pieces)[1024]) {
+                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr) {
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -446,7 +449,12 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const int p = pU[u];
       const int lead = __builtin_amdgcn_readfirstlane(p);
       if (ap != lead) {
-        if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+        if (ap >= 0) {
+          if (alt)
+            acc_flush(*alt, acc, lane, slot);
+          else
+            acc_flush(ctrs[ap], acc, lane, slot);
+        }
         ap = lead;
       }
       const bool in = inU[u], rowok = okU[u];
@@ -525,14 +533,22 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         if (c2) atomicAdd(&cp.sums[2][slot], (unsigned long long)c2);
         if (c3) atomicAdd(&cp.sums[3][slot], (unsigned long long)c3);
         if (ne) {
-          atomicMin(&cp.box[0][slot], txU[u]), atomicMax(&cp.box[3][slot], txU[u]);
-          atomicMin(&cp.box[1][slot], tyU[u]), atomicMax(&cp.box[4][slot], tyU[u]);
-          atomicMin(&cp.box[2][slot], tzU[u]), atomicMax(&cp.box[5][slot], tzU[u]);
+          atomicMax(&cp.box[0][slot], box_enc_lo(txU[u]));
+          atomicMax(&cp.box[3][slot], box_enc_hi(txU[u]));
+          atomicMax(&cp.box[1][slot], box_enc_lo(tyU[u]));
+          atomicMax(&cp.box[4][slot], box_enc_hi(tyU[u]));
+          atomicMax(&cp.box[2][slot], box_enc_lo(tzU[u]));
+          atomicMax(&cp.box[5][slot], box_enc_hi(tzU[u]));
         }
       }
     }
   }
-  if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+  if (ap >= 0) {
+    if (alt)
+      acc_flush(*alt, acc, lane, slot);
+    else
+      acc_flush(ctrs[ap], acc, lane, slot);
+  }
 }
 
 template <int TXB, int TY, int TZ>
@@ -563,10 +579,10 @@ __global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
   int bx[6];
 #pragma unroll
   for (int k = 0; k < 6; ++k)
-    bx[k] = k < 3 ? wave_min(c.box[k][lane], lane) : wave_max(c.box[k][lane], lane);
+    bx[k] = wave_max(c.box[k][lane], lane);
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) c.lo[k] = bx[k], c.hi[k] = bx[3 + k];
+    for (int k = 0; k < 3; ++k) c.lo[k] = box_dec_lo(bx[k]), c.hi[k] = box_dec_hi(bx[3 + k]);
   }
 }
 
@@ -846,7 +862,8 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     int bx[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k)
-      bx[k] = k < 3 ? wave_min(c.box[k][lane], lane) : wave_max(c.box[k][lane], lane);
+      bx[k] = k < 3 ? box_dec_lo(wave_max(c.box[k][lane], lane))
+                    : box_dec_hi(wave_max(c.box[k][lane], lane));
     if (lane == 0) {
 #pragma unroll
       for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
@@ -1456,15 +1473,20 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
   uint32_t t = 0;
   bool run = true;
   // epoch of a dilation barrier = eoff + the iteration that follows it; the
-  // fused launch's prologue uses ebase + 1 .. ebase + 3
+  // fused launch's prologue uses ebase + 2 (end of the pack) and ebase + 3
   const uint32_t eoff = FUSED ? a.ebase + 3 : 0u;
   __shared__ unsigned long long s_phase[3];
   if constexpr (FUSED) {
     const PairDesc &d = a.desc;
     PairCtr &c0 = ctrs[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[0] = wall_clock64();
-    // R: workgroup 0 zeroes the pair's counters and histograms and the
-    // engine's state (barrier words are tagged with ever larger epochs)
+    // R: workgroup 0 zeroes the pair's counters and histograms, the engine's
+    // state (barrier words are tagged with ever larger epochs) and the pack
+    // partials the next launch packs into.  Nothing here is read before the
+    // barrier that ends the pack, except by workgroup 0 itself, and this
+    // launch packs into the set the previous launch zeroed, so the pack
+    // starts at once.
+    PackSums *acc = &es->acc[a.acc_set & 1];
     if (blockIdx.x == 0) {
       const int tid = threadIdx.x;
       if (tid == 0) {
@@ -1478,19 +1500,17 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         st_sc(&es->t, 0u), st_sc(&es->error, 0u), st_sc(&es->qmax, 0u);
         st_sc(&es->qtotal, 0ull), st_sc(&es->qcap, qcap), st_sc(&es->qnext, 0u);
       }
-      for (int i = tid; i < 2 * kHitSlots; i += blockDim.x) st_sc(&c0.hits[0][0] + i, 0ull);
-      for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&c0.sums[0][0] + i, 0ull);
-      for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
-        st_sc(&c0.box[0][0] + i, i < 3 * kSumSlots ? 0x7FFFFFFF : -1);
+      PackSums *next = &es->acc[(a.acc_set & 1) ^ 1];
+      for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&next->sums[0][0] + i, 0ull);
+      for (int i = tid; i < 6 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
       for (int i = tid; i < d.hist_len; i += blockDim.x)
         st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
       for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
     }
-    run = grid_sync_tagged(es, a.ebase + 1, spin_limit, &s_ok);
     // P: pack, every store written through
-    if (run) {
+    {
       __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-      pack_tiles<TXB, TY, TZ, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces);
+      pack_tiles<TXB, TY, TZ, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc);
       if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[1] = wall_clock64();
       run = grid_sync_tagged(es, a.ebase + 2, spin_limit, &s_ok);
     }
@@ -1503,15 +1523,15 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         unsigned long long v[4];
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
-          v[f] = ld_sc(&c0.sums[f][lane]);
+          v[f] = ld_sc(&acc->sums[f][lane]);
 #pragma unroll
           for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
         }
         int bx[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const int x = ld_sc(&c0.box[k][lane]);
-          bx[k] = k < 3 ? wave_min(x, lane) : wave_max(x, lane);
+          const int x = wave_max(ld_sc(&acc->box[k][lane]), lane);
+          bx[k] = k < 3 ? box_dec_lo(x) : box_dec_hi(x);
         }
         if (lane == 0) {
 #pragma unroll

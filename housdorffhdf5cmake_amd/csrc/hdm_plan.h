@@ -46,6 +46,7 @@ struct DilateArgs {
   // the host can take the results without waiting for the stream (or null)
   uint32_t ready_seq;
   uint32_t *ready;
+  int32_t acc_set;  // fused launch: the set of Engine::acc it packs into (0 or 1)
 };
 
 struct TileLaunch {
