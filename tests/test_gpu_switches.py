@@ -1,0 +1,70 @@
+"""The comparison switches: HDM_DS=0 (one dilation step per grid barrier) and
+HDM_FUSED=0 (a single pair as four kernels) give the same results as the
+default two-step fused launch, bit for bit, for single pairs, batches and
+robust cuts.  The switches are read once per process, so the switched runs
+happen in a child process on the same GPU."""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+import housdorffhdf5cmake_amd as hdm
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KEYS = ("hd", "sum", "count", "sum_fp", "cnt_fp", "sum_fn", "cnt_fn", "n_fp", "n_fn", "hd_fp",
+        "hd_fn", "iterations")
+SPECS = [("ellipsoids", 160, 128, 96, 8, 6, 24, 2, 0.005, 41),
+         ("ellipsoids", 67, 45, 31, 6, 4, 20, 2, 0.01, 7),
+         ("long", 128, 96, 128, 30, 64, 48, 20, 15, 12, 60, 4)]
+
+CHILD = r"""
+import json, sys
+sys.path.insert(0, sys.argv[1])
+import torch
+import housdorffhdf5cmake_amd as hdm
+specs = json.loads(sys.argv[2])
+keys = json.loads(sys.argv[3])
+vols = [hdm.synth_device(hdm.synth_spec(s[0], *s[1:])) for s in specs]
+torch.cuda.synchronize()
+out = {"single": [], "robust": [], "batch": []}
+for g, s in vols:
+    out["single"].append({k: hdm.hausdorff(g, s)[k] for k in keys})
+    out["robust"].append({k: hdm.hausdorff(g, s, robustness=0.9)[k] for k in keys})
+b = hdm.Batch(vols, graph=False)
+b.run()
+out["batch"] = [{k: r[k] for k in keys if k != "iterations"} for r in b.sync()]
+b.close()
+print(json.dumps(out))
+"""
+
+
+def run_here():
+    vols = [hdm.synth_device(hdm.synth_spec(s[0], *s[1:])) for s in SPECS]
+    torch.cuda.synchronize()
+    out = {"single": [], "robust": [], "batch": []}
+    for g, s in vols:
+        out["single"].append({k: hdm.hausdorff(g, s)[k] for k in KEYS})
+        out["robust"].append({k: hdm.hausdorff(g, s, robustness=0.9)[k] for k in KEYS})
+    b = hdm.Batch(vols, graph=False)
+    b.run()
+    out["batch"] = [{k: r[k] for k in KEYS if k != "iterations"} for r in b.sync()]
+    b.close()
+    return out
+
+
+@pytest.mark.parametrize("switch", ["HDM_DS", "HDM_FUSED"])
+def test_switched_off_paths_give_the_same_results(switch):
+    here = run_here()
+    env = dict(os.environ, **{switch: "0"})
+    r = subprocess.run([sys.executable, "-c", CHILD, ROOT, json.dumps(SPECS), json.dumps(KEYS)],
+                       capture_output=True, text=True, timeout=240, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    there = json.loads(r.stdout.strip().splitlines()[-1])
+    assert there == here
