@@ -1656,10 +1656,11 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     }
     uint64_t *Qout = sub_queue(q, qcap, (t + 1) & 1, kq);
     uint32_t *qout_len = &es->qlen[(t + 1) % 4][kq];
-    // several pairs: a step's claims of idle neighbours (compare-and-swap on
-    // their state) are pushed with the next step's entries, so the swap's
-    // round trip overlaps that step's loads; claim_old: the state the swap
-    // found (0: this lane won the tile), claim_ent: the entry (0: no claim)
+    // a step's claims of idle neighbours (compare-and-swap on their state)
+    // are pushed with the next step's entries, or after the wave's last step,
+    // so the swap's round trip overlaps that step's loads or the stores the
+    // barrier waits for anyway; claim_old: the state the swap found (0: this
+    // lane won the tile), claim_ent: the entry (0: no claim)
     uint32_t claim_old = 0;
     uint64_t claim_ent = 0;
     // one step of queue entry ent (live: a real entry) by the R lanes of its
@@ -2137,7 +2138,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         const bool ozh = group_bits<R>(__ballot(zl == TZ - 1 && dl != 0), lane) != 0;
         bool want = false;
         uint64_t pent = 0;
-        uint32_t new_old = 0;  // this step's claim (several pairs)
+        uint32_t new_old = 0;  // this step's claim
         uint64_t new_ent = 0;
         if (live) {
           if (full) {
@@ -2182,27 +2183,22 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
                 const uint64_t zm = joz < 0 ? 0xFFull : joz > 0 ? 0xFFull << 56 : ~0ull;
                 reach = (xs & ym & zm) != 0;
               }
-              if constexpr (SINGLE) {
-                if (reach && cas_g(st_m + nb_dir, 0u, tp + 2) == 0u) {
-                  want = true;
-                  pent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
-                }
-              } else if (reach) {
+              if (reach) {
                 new_old = cas_g(st_m + nb_dir, 0u, tp + 2);
                 new_ent = make_entry(nb_dir, (uint32_t)p, (uint32_t)m, 1, 1, v.gen);
               }
             }
           }
         }
+        // lane kSelf's own entry, lanes below it the claims won by the step before
+        const bool won = claim_ent != 0 && claim_old == 0u;
         if constexpr (SINGLE)
-          push_sticky(want, pent, s_st[(t + 1) & 1], &s_stn[(t + 1) & 1], kStick, lq, Qout,
-                      qout_len, lane, &s_cnt[2]);
-        else {  // lane kSelf's own entry, lanes below it the claims won by the step before
-          const bool won = claim_ent != 0 && claim_old == 0u;
+          push_sticky(want || won, want ? pent : claim_ent, s_st[(t + 1) & 1],
+                      &s_stn[(t + 1) & 1], kStick, lq, Qout, qout_len, lane, &s_cnt[2]);
+        else
           push_entries(want || won, want ? pent : claim_ent, lq, Qout, qout_len, lane);
-          claim_old = new_old;
-          claim_ent = new_ent;
-        }
+        claim_old = new_old;
+        claim_ent = new_ent;
     };
     // global entry ei of the queue of iteration t
     auto global_entry = [&](uint32_t ei) {
@@ -2240,6 +2236,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           step_tile(live ? global_entry(ei) : 0ull, live);
         }
       }
+      // the claims of the wave's last step
+      push_sticky(claim_ent != 0 && claim_old == 0u, claim_ent, s_st[(t + 1) & 1],
+                  &s_stn[(t + 1) & 1], kStick, lq, Qout, qout_len, lane, &s_cnt[2]);
+      claim_ent = 0;
     } else {  // the same order, each entry loaded while the one before steps
       uint32_t base = gfirst + gwave * step, k = 0;
       uint64_t nxt = (base < L && base + sub < L) ? global_entry(base + sub) : 0ull;
