@@ -1025,6 +1025,10 @@ struct PassState {
   int done_from[2];
   uint32_t cur_total;   // entries of the iteration running (global + kept)
   uint32_t next_total;  // entries queued for the next one
+  // the engine's queue totals: kept here between barriers and stored when
+  // the launch leaves, so no barrier waits on them
+  unsigned long long qtotal;
+  uint32_t qmax;
 };
 
 // Grid barrier of a single-pair launch.  It has the same structure as
@@ -1203,8 +1207,8 @@ __device__ __forceinline__ bool grid_barrier_single(
 #pragma unroll
         for (int m = 0; m < 2; ++m)
           if (ps.cut[m] == 0 || ps.done_from[m] <= kSteps * tp + kSteps) flags |= 1u << m;
-        es->qtotal += ps.cur_total;
-        es->qmax = max(es->qmax, ps.cur_total);
+        ps.qtotal += ps.cur_total;
+        ps.qmax = max(ps.qmax, ps.cur_total);
         if (qstats && tdone < (uint32_t)qstats_len) qstats[tdone] = ps.cur_total;
         ps.next_total = acc + kept;
         ps.cur_total = ps.next_total;
@@ -1612,6 +1616,8 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         ps.hd[m] = ld_sc(&c0.hd_last[m]);
         ps.done_from[m] = ld_sc(&c0.done_from[m]);
       }
+      ps.qtotal = ld_sc(&es->qtotal);
+      ps.qmax = ld_sc(&es->qmax);
     }
     __syncthreads();
     t = ld_sc(&es->t);
@@ -2344,7 +2350,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       }
       if (threadIdx.x < kSubQ) st_sc(&es->qlen[(t + 2) % 4][threadIdx.x], 0u);
       if (threadIdx.x == 0) {
-        es->qtotal += L;
+        es->qtotal += L;  // several pairs: the fold reads pair state from memory anyway
         es->qmax = max(es->qmax, L);
         if (qstats && (int)tdone < qstats_len) qstats[tdone] = L;
       }
@@ -2383,13 +2389,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       const uint32_t qn = SINGLE ? ps.next_total : s_pre[kSubQ];
       st_sc(&es->t, t);
       st_sc(&es->qnext, qn);
+      if (SINGLE) st_sc(&es->qtotal, ps.qtotal), st_sc(&es->qmax, ps.qmax);
       if (a.head) {
         EngineHead h;
         h.t = t;
         h.error = err ? 1u : 0u;
-        h.qmax = es->qmax;
+        h.qmax = SINGLE ? ps.qmax : es->qmax;
         h.nwg = gridDim.x;
-        h.qtotal = es->qtotal;
+        h.qtotal = SINGLE ? ps.qtotal : es->qtotal;
         h.qcap = es->qcap;
         h.qnext = qn;
         h.phase[0] = FUSED ? s_phase[0] : 0ull;
