@@ -144,7 +144,15 @@ int resident_grid(const Ctx *c, int tile, int requested, int npairs) {
   return maxres;
 }
 
-uint64_t spin_ticks(const Ctx *c) { return (uint64_t)(c->clock_khz * 1000.0 * 2.0); }
+// Bound of every grid-barrier wait: 2 s of wall-clock ticks.  HDM_SPIN_TICKS
+// (read at each call or batch / server creation) replaces it, so tests can
+// make the waits expire and check the timeout path; every workgroup then
+// leaves its loop and the call returns HDM_E_TIMEOUT.
+uint64_t spin_ticks(const Ctx *c) {
+  const char *e = std::getenv("HDM_SPIN_TICKS");
+  if (e && *e) return std::strtoull(e, nullptr, 10);
+  return (uint64_t)(c->clock_khz * 1000.0 * 2.0);
+}
 
 // Grid barrier: 0 (default) flag array with agent-coherent data accesses and
 // no cache maintenance; 1 the same plus L2 write-back / invalidate at every

@@ -68,3 +68,51 @@ def test_switched_off_paths_give_the_same_results(switch):
     assert r.returncode == 0, r.stderr[-2000:]
     there = json.loads(r.stdout.strip().splitlines()[-1])
     assert there == here
+
+
+CHILD_TIMEOUT = r"""
+import json, os, sys
+sys.path.insert(0, sys.argv[1])
+import torch
+import housdorffhdf5cmake_amd as hdm
+keys = json.loads(sys.argv[2])
+g, s = hdm.synth_device(hdm.synth_spec("ellipsoids", 160, 128, 96, 8, 6, 24, 2, 0.005, 41))
+torch.cuda.synchronize()
+def pick(r):
+    return {k: r[k] for k in keys}
+def batch_run():
+    b = hdm.Batch([(g, s), (s, g)], graph=False)
+    b.run()
+    try:
+        return [pick(r) for r in b.sync()]
+    except hdm.HDMError as e:
+        return e.code
+    finally:
+        b.close()
+out = {"before": pick(hdm.hausdorff(g, s)), "batch_before": batch_run()}
+os.environ["HDM_SPIN_TICKS"] = "0"  # every barrier wait expires at once
+out["timeout"] = hdm.hausdorff(g, s, raise_on_error=False)["status"]
+out["batch_timeout"] = batch_run()
+del os.environ["HDM_SPIN_TICKS"]
+out["after"] = pick(hdm.hausdorff(g, s))
+out["batch_after"] = batch_run()
+print(json.dumps(out))
+"""
+
+
+def test_barrier_timeout_is_reported_and_the_next_calls_are_exact():
+    """With the barrier wait bound forced to zero (HDM_SPIN_TICKS=0), a single
+    pair and a batch end with HDM_E_TIMEOUT (-5) instead of waiting, every
+    workgroup leaving its loop; the calls after it, with the normal bound,
+    return exactly the results from before (engine state, epoch tags and the
+    alternating pack partials recover)."""
+    keys = [k for k in KEYS if k != "iterations"]
+    r = subprocess.run([sys.executable, "-c", CHILD_TIMEOUT, ROOT, json.dumps(keys)],
+                       capture_output=True, text=True, timeout=240,
+                       env={k: v for k, v in os.environ.items() if k != "HDM_SPIN_TICKS"})
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["timeout"] == -5
+    assert out["batch_timeout"] == -5
+    assert out["after"] == out["before"]
+    assert out["batch_after"] == out["batch_before"]
