@@ -91,6 +91,24 @@ def leg_single(hdm, torch, steps, warmup):
     return G, S, res, dt, nvox, avg
 
 
+def copy_bandwidth(torch, mib=1024, reps=5):
+    """Attainable HBM bandwidth on this box: a device-to-device copy of mib
+    MiB (read + write bytes over the event time, torch's current stream, where
+    the copy runs), mean of reps copies after one warm-up."""
+    a = torch.empty(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return round(2 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
+
+
 def roofline(res, avg, nvox, tile=1):
     """Algorithmic bytes per launch of each phase over its mean event time."""
     # a queued (tile, mask) entry does the work of steps_per_barrier dilation
@@ -187,7 +205,7 @@ def leg_small(hdm, torch, steps):
         res = hdm.hausdorff(G, S)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
-    return G, S, res, {"workload": "sphere_64x64x64_r20_shift_2_1_0", "hd": res["hd"],
+    return G, S, res, {"workload": "sphere_64x64x64_r20_shift_2_1_0_flip1pct", "hd": res["hd"],
                        "avg_hd": res["avg_hd"], "ms_per_pair": round(dt * 1e3, 4),
                        "pairs_per_s": round(1 / dt, 1), "mask_voxels_per_s": round(2 * G.numel() / dt, 1)}
 
@@ -380,6 +398,9 @@ def main():
         "dilation_steps_per_barrier": int(avg["steps_per_barrier"]),
     }
     line["roofline"] = roofline(res, avg, nvox)
+    copy_gbs = copy_bandwidth(torch)
+    line["roofline"]["attainable_copy_gbs"] = copy_gbs
+    line["roofline"]["frac_of_attainable"] = round(line["roofline"]["achieved"] / copy_gbs, 4)
     if args.copy_probe > 0:
         a = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
