@@ -369,8 +369,9 @@ def main():
     ap.add_argument("--long-reps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="also run a plain device-to-device copy of MIB MiB three times "
-                         "(the known-size kernel that scales HBM byte counters)")
+                    help="size of the device-to-device copy timed for the attainable "
+                         "bandwidth (default 1024); a run's copies all have this size, the "
+                         "known-size kernel that scales HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine is single-device (a pair or a batch per "
@@ -398,16 +399,11 @@ def main():
         "dilation_steps_per_barrier": int(avg["steps_per_barrier"]),
     }
     line["roofline"] = roofline(res, avg, nvox)
-    copy_gbs = copy_bandwidth(torch)
+    # every copy of the run has this size, so it is also the known-size
+    # kernel that scales the HBM byte counters
+    copy_gbs = copy_bandwidth(torch, mib=args.copy_probe if args.copy_probe > 0 else 1024)
     line["roofline"]["attainable_copy_gbs"] = copy_gbs
     line["roofline"]["frac_of_attainable"] = round(line["roofline"]["achieved"] / copy_gbs, 4)
-    if args.copy_probe > 0:
-        a = torch.ones(args.copy_probe << 20, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
     extra = []
     if not args.no_extra and not args.no_small:
         log("small 64^3 ...")
