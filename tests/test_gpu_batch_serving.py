@@ -72,6 +72,36 @@ def test_batch_robust_cut_equals_single_and_oracle(robust):
         assert rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
 
 
+def test_batch_of_small_and_degenerate_volumes_equals_single():
+    """Lines, single voxels, volumes smaller than one tile, odd dims and
+    random sparse volumes in one batch (two dilation steps per barrier over
+    pairs of every size): each result equals the pair computed alone."""
+    rng = np.random.default_rng(31)
+    host = []
+    for shape in ((1, 1, 200), (200, 1, 1), (1, 150, 1), (3, 1, 129), (5, 7, 9), (1, 1, 1)):
+        g = np.zeros(shape, np.uint8)
+        s = np.zeros(shape, np.uint8)
+        g.reshape(-1)[0] = 1
+        s.reshape(-1)[-1] = 1
+        host.append((g, s))
+    for _ in range(6):
+        nz, ny, nx = (int(v) for v in rng.integers(3, 70, size=3))
+        g = (rng.random((nz, ny, nx)) < 0.01).astype(np.uint8)
+        s = (rng.random((nz, ny, nx)) < 0.01).astype(np.uint8)
+        g[0, 0, 0] = s[-1, -1, -1] = 1
+        host.append((g, s))
+    vols = [(torch.from_numpy(g).cuda(), torch.from_numpy(s).cuda()) for g, s in host]
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    for robust in (1.0, 0.75):
+        want = alone if robust == 1.0 else [hdm.hausdorff(g, s, robustness=robust) for g, s in vols]
+        b = hdm.Batch(vols, robustness=robust, graph=False)
+        b.run()
+        out = b.sync()
+        b.close()
+        for i, (o, a) in enumerate(zip(out, want)):
+            assert same(o, a), (robust, i, host[i][0].shape)
+
+
 def test_graph_replays_are_identical_with_eager_work_between():
     vols = mixed_pairs(8, seed=5)
     alone = [hdm.hausdorff(g, s) for g, s in vols]
