@@ -14,7 +14,11 @@
 //   dilate_kernel    persistent kernel: one grid barrier per iteration, a
 //                    queue of the tiles still changing (mainPassKernel,
 //                    hello.cu:1066-1896); when it leaves, its workgroup 0
-//                    writes every pair's HD, sums and counts (finalize_pair)
+//                    writes every pair's HD, sums and counts (finalize_pair).
+//                    The one-pair variant can run the reset, pack and seed
+//                    itself first (FUSED: one launch per call); on 64x8x8
+//                    tiles with the 6 element an iteration is two dilation
+//                    steps (DS; dilate_ds_kernel is the fused one-pair form)
 //
 // Same-address atomics serialize (tens of ns each on MI355X), so every hot
 // counter is spread: partial voxel counts over kSumSlots, per-iteration hits
