@@ -2387,6 +2387,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       for (int p = threadIdx.x; p < npairs; p += blockDim.x)
         a.outs[p] = finalize_pair(descs[p], ctrs[p], t, err, DS ? 2 : 1);
     }
+    // with a ready marker, every thread's results reach the host before
+    // thread 0 stores it
+    if (a.ready) __threadfence_system();
+    __syncthreads();
     if (threadIdx.x == 0) {
       // entries left for a next iteration (one pair: the master's count,
       // which includes the workgroups' own queues)
@@ -2409,7 +2413,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         h.phase[3] = FUSED ? wall_clock64() : 0ull;
         *a.head = h;
       }
-      if (a.ready) {  // every store above reaches the host before the marker
+      if (a.ready) {  // the head too reaches the host before the marker
         __threadfence_system();
         __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       }

@@ -154,8 +154,12 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
         max_in_flight = max(max_in_flight, f)
         if i >= n and q == 0 and f == 0:
             break
-        if q == 0 and f == 0 and i < n:  # idle until the next arrival
-            time.sleep(max(0.0, min(0.001, arrivals[i] - (now() - t0))))
+        if q == 0 and f == 0 and i < n:  # idle until the next arrival:
+            # sleep to 0.2 ms before it, then spin (a sleep overshoots by the
+            # kernel's timer slack, which would count in the latency)
+            gap = arrivals[i] - (now() - t0)
+            if gap > 3e-4:
+                time.sleep(min(0.001, gap - 2e-4))
     srv.close()
     span = max(t_last - t0, 1e-9)
     lat_ms = np.array(lat) * 1e3
