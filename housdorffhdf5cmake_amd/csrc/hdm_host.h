@@ -101,6 +101,10 @@ int norm_tile(int tile);
 // 6), or -1 when they are invalid; *conn receives the element
 int select_launch(int tile_opt, int conn_opt, int *conn, bool ds = false);
 int barrier_mode();
+// Spins until the launch on s has stored seq into *ready (pinned coherent
+// host memory), checking the stream now and then so a launch that ended
+// without it is an error, not a hang.
+int wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq);
 bool fused_enabled();
 bool ds_enabled();
 int fused_grid(const Ctx *c, int tile, int requested, bool ds = false);

@@ -121,6 +121,9 @@ int get_ctx(Ctx **out) {
   // once the kernel has completed
   if (rc == HDM_OK && hipHostMalloc(&c->host_out, 4096, hipHostMallocCoherent) != hipSuccess)
     rc = HDM_E_HIP;
+  // pinned memory can come back from an earlier allocation with its contents:
+  // no ready marker left there may match this context's first call
+  if (rc == HDM_OK) std::memset(c->host_out, 0, 4096);
   for (int i = 0; rc == HDM_OK && i < 5; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = HDM_E_HIP;
   if (rc != HDM_OK) {
@@ -355,7 +358,7 @@ extern "C" uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile) {
 
 // Waits until the kernel's ready marker holds seq; the stream is queried
 // every 200 us so a launch that fails or ends without the marker is reported.
-static int wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq) {
+int hdm::wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq) {
   using clk = std::chrono::steady_clock;
   auto next = clk::now() + std::chrono::microseconds(200);
   for (uint32_t k = 0;; ++k) {

@@ -111,7 +111,8 @@ struct hdm_server {
   std::vector<uint64_t> freed_after;
   uint64_t slices = 0;
   std::vector<hipEvent_t> event_pool;
-  void *host = nullptr;  // pinned: outs + engine head
+  void *host = nullptr;  // pinned: outs + engine head + ready marker (zeroed at creation)
+  uint32_t seq = 0;      // ready marker value of the last slice
   uint64_t next_id = 1;
   uint32_t t_engine = 0;
 };
@@ -202,6 +203,9 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
     hdm_server_destroy(s);
     return HDM_E_NOMEM;
   }
+  // pinned memory comes back from earlier allocations with their contents: a
+  // marker left by an earlier server must not match this one's first slice
+  std::memset(s->host, 0, align256(sizeof(PairOut) * o.max_slots) + 256);
   s->arena.init(arena);
   s->P = carve_common((char *)s->common, L);
   s->slots.resize(o.max_slots);
@@ -362,9 +366,20 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   a.flags = barrier_mode();
   a.outs = (PairOut *)h;
   a.head = hhead;
+  // the slice stores its sequence number after the results and the head;
+  // the step takes them from there instead of waiting for the stream (the
+  // launch's workgroups touch no pair data after its last barrier, so the
+  // slots and arena blocks freed below are free for the next admissions,
+  // whose work the stream orders after the launch)
+  volatile uint32_t *ready =
+      (volatile uint32_t *)(h + align256(sizeof(PairOut) * nslots) + 192);
+  if (++s->seq == 0) ++s->seq;
+  a.ready = (uint32_t *)ready;
+  a.ready_seq = s->seq;
   hipError_t e = s->tl->dilate(dim3(s->cfg.grid), s->s, a);
-  if (e == hipSuccess) e = hipStreamSynchronize(s->s);
   if (e != hipSuccess) return map_hip(e);
+  const int wr = wait_ready(s->s, ready, s->seq);
+  if (wr != HDM_OK) return wr;
   EngineHead head{};
   std::memcpy(&head, hhead, sizeof(head));
   if (head.nwg == 0) return HDM_E_HIP;  // the launch did not complete

@@ -166,6 +166,23 @@ def test_server_mix_equals_alone():
     srv.close()
 
 
+def test_servers_in_a_row_each_one_slice():
+    """Each server here runs exactly one slice, so the one created after it
+    may get the same pinned block back with the first slice's ready marker
+    still in it; every server must still wait for its own slice."""
+    vols = mixed_pairs(2, seed=37)
+    alone = [hdm.hausdorff(g, s) for g, s in vols]
+    for k in range(6):
+        srv = serving.Server(hbm_budget=1 << 30, max_slots=4, slice_iters=64,
+                             max_dims=(256, 128, 100))
+        g, s = vols[k % 2]
+        rid = srv.submit(g, s)
+        done = srv.drain(60)
+        srv.close()
+        assert [c["id"] for c in done] == [rid]
+        assert done[0]["status"] == 0 and same(done[0], alone[k % 2]), k
+
+
 def test_server_errors_budget_and_capacity():
     vols = mixed_pairs(3, seed=17)
     g0 = torch.zeros((16, 16, 64), dtype=torch.uint8, device="cuda")
