@@ -2289,42 +2289,55 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     // the histogram and the covered counts, decide each pass's stop
     // (hello.cu:1748-1759), reset the sub-queue counters that iteration t+1 fills
     auto fold = [&]() {
-      // kHitSlots lanes per (pair, mask): each loads one cumulative slot, the
-      // group sums them; the increase over the last fold is this iteration's
-      // histogram bin.  Two steps per iteration: the counters of each step
-      // give the bins 2tp+1 and 2tp+2 (cov holds both steps, cov2 the
+      // per lanes per (pair, mask): each sums kHitSlots / per cumulative hit
+      // slots, the group sums the lanes; the increase over the last fold is
+      // this iteration's histogram bin.  per shrinks with the pair count, so
+      // up to 64 pairs fold in one pass, and every load of an item (its pair's
+      // start, stop state and counters, and the hit slots) is issued at once:
+      // one round trip per pass.  Two steps per iteration: the counters of each
+      // step give the bins 2tp+1 and 2tp+2 (cov holds both steps, cov2 the
       // second's part), folded in order up to the one that reaches the cut.
       constexpr int kSteps = DS ? 2 : 1;
-      const int s = threadIdx.x % kHitSlots;
-      for (int base = 0; base < 2 * npairs; base += kWgThreads / kHitSlots) {
-        const int i = base + threadIdx.x / kHitSlots;
+      int per = kHitSlots;
+      while (per > 2 && per * 2 * npairs > kWgThreads) per >>= 1;
+      const int s = threadIdx.x % per;
+      for (int base = 0; base < 2 * npairs; base += kWgThreads / per) {
+        const int i = base + threadIdx.x / per;
         const int pp = i >> 1, mm = i & 1;
-        bool act = i < 2 * npairs;
-        int tpp = 0;
-        if (act) {
-          const uint32_t ts = descs[pp].t_start;
-          act = tdone >= ts;
-          tpp = (int)(tdone - ts);
-          if (act && ld_sc(&ctrs[pp].done_from[mm]) <= kSteps * tpp) act = false;
+        const bool in = i < 2 * npairs;
+        unsigned long long v = 0, v2 = 0, cov = 0, c2 = 0, sum = 0, nm = 0;
+        uint32_t ts = 0, hd = 0;
+        int df0 = 0;
+        if (in) {
+          for (int k = s; k < kHitSlots; k += per) {
+            v += ld_sc(&ctrs[pp].hits[mm][k]);
+            if constexpr (DS) v2 += ld_sc(&ctrs[pp].hits[2 + mm][k]);
+          }
+          if (s == 0) {
+            ts = descs[pp].t_start;
+            df0 = ld_sc(&ctrs[pp].done_from[mm]);
+            cov = ld_sc(&ctrs[pp].cov[mm]);
+            sum = ld_sc(&ctrs[pp].sum_d[mm]);
+            if constexpr (DS) {
+              c2 = ld_sc(&ctrs[pp].cov2[mm]);
+              hd = ld_sc(&ctrs[pp].hd_last[mm]);
+            }
+            nm = mm == 0 ? ctrs[pp].n_fp : ctrs[pp].n_fn;
+          }
         }
-        unsigned long long v = act ? ld_sc(&ctrs[pp].hits[mm][s]) : 0ull;
-        unsigned long long v2 = (DS && act) ? ld_sc(&ctrs[pp].hits[2 + mm][s]) : 0ull;
-#pragma unroll
-        for (int off = kHitSlots / 2; off >= 1; off >>= 1) {
+        for (int off = per / 2; off >= 1; off >>= 1) {
           v += shfl64(v, (threadIdx.x & 63) ^ off);
           if constexpr (DS) v2 += shfl64(v2, (threadIdx.x & 63) ^ off);
         }
-        if (!act || s != 0) continue;
+        if (!in || s != 0 || tdone < ts) continue;
+        const int tpp = (int)(tdone - ts);
+        if (df0 <= kSteps * tpp) continue;  // the pass stopped at an earlier bin
         PairCtr &cc = ctrs[pp];
         const PairDesc &dd = descs[pp];
-        const unsigned long long cut = robust_cut(dd.robust, mm == 0 ? cc.n_fp : cc.n_fn);
+        const unsigned long long cut = robust_cut(dd.robust, nm);
         if constexpr (DS) {
-          unsigned long long cov = ld_sc(&cc.cov[mm]);
-          const unsigned long long c2 = ld_sc(&cc.cov2[mm]);
           const unsigned long long bin[2] = {v - (cov - c2), v2 - c2};
           if (bin[0] == 0 && bin[1] == 0) continue;
-          unsigned long long sum = ld_sc(&cc.sum_d[mm]);
-          uint32_t hd = ld_sc(&cc.hd_last[mm]);
           int df = 0;
 #pragma unroll
           for (int k = 0; k < 2; ++k) {
@@ -2343,11 +2356,11 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           st_sc(&cc.hd_last[mm], hd);
           if (df) st_sc(&cc.done_from[mm], df);
         } else {
-          const unsigned long long before = ld_sc(&cc.cov[mm]);
+          const unsigned long long before = cov;
           if (v == before) continue;
           if (tpp + 1 < dd.hist_len) dd.hist[mm][tpp + 1] = (uint32_t)(v - before);
           st_sc(&cc.cov[mm], v);
-          st_sc(&cc.sum_d[mm], ld_sc(&cc.sum_d[mm]) + (v - before) * (unsigned long long)(tpp + 1));
+          st_sc(&cc.sum_d[mm], sum + (v - before) * (unsigned long long)(tpp + 1));
           st_sc(&cc.hd_last[mm], (uint32_t)(tpp + 1));
           if (before < cut && v >= cut) st_sc(&cc.done_from[mm], tpp + 1);
         }
