@@ -7,6 +7,9 @@
 // the engine iteration at which they start, so every pair keeps its own
 // iteration count), and retire as soon as both of their passes are done.
 // One step = admit, one slice of slice_iters engine iterations, retire.
+// A lone request (nothing in flight, nothing else waiting) skips the slots:
+// it runs as the library's one-pair call on the server's stream, one fused
+// launch instead of the admission kernels and a slice.
 // Host-pointer inputs are copied on a separate copy stream; the engine stream
 // waits on an event per request, so copies overlap the dilation.
 #include <hip/hip_runtime.h>
@@ -311,9 +314,54 @@ static int admit(hdm_server *s, Request &r, int p) {
   return HDM_OK;
 }
 
+// The lone request at the head of the FIFO as a one-pair call (hdm_hausdorff_*)
+// on the server's stream: the same results as the pair computed alone, which
+// every request's result equals.  Its workspace is the library context's (the
+// one the one-pair calls use, sized by the request), not the arena.  A
+// per-request outcome (done, empty mask, timeout, stuck) completes the
+// request; any other error is the step's and the request stays queued.
+static int run_lone(hdm_server *s, hdm_completion *done, int *n_done) {
+  Request r = s->pending.front();
+  hdm_opts o;
+  hdm_opts_init(&o);
+  o.robustness = s->o.robustness;
+  o.ptr_kind = r.ptr_kind;
+  o.tile = s->o.tile;
+  o.connectivity = s->o.connectivity;
+  o.stream = s->s;
+  hdm_result res;
+  std::memset(&res, 0, sizeof(res));
+  r.t_admit = now_s();
+  const int rc =
+      r.dtype == HDM_DTYPE_U8
+          ? hdm_hausdorff_u8((const uint8_t *)r.gold, (const uint8_t *)r.segm, r.nx, r.ny, r.nz,
+                             (uint8_t)r.label, &o, &res)
+          : hdm_hausdorff_i32((const int32_t *)r.gold, (const int32_t *)r.segm, r.nx, r.ny,
+                              r.nz, (int32_t)r.label, &o, &res);
+  if (rc != HDM_OK && rc != HDM_E_EMPTY_MASK && rc != HDM_E_TIMEOUT && rc != HDM_E_STUCK)
+    return rc;
+  s->pending.pop_front();
+  hdm_completion &c = done[(*n_done)++];
+  std::memset(&c, 0, sizeof(c));
+  c.id = r.id;
+  c.status = rc;
+  c.result = res;
+  c.t_submit = r.t_submit;
+  c.t_admit = r.t_admit;
+  c.t_done = now_s();
+  return HDM_OK;
+}
+
 extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done) {
   if (!s || (!done && cap > 0) || !n_done) return HDM_E_INVALID;
   *n_done = 0;
+  if (s->pending.size() == 1 && cap > 0 &&
+      (s->pending.front().dtype != HDM_DTYPE_U8 ||
+       (s->pending.front().label >= 0 && s->pending.front().label <= 255))) {
+    bool idle = true;
+    for (auto &r : s->slots) idle = idle && r.id == 0;
+    if (idle) return run_lone(s, done, n_done);
+  }
   // admit in arrival order while a slot and arena room exist
   int admitted = 0;
   bool quarantined = false;

@@ -106,7 +106,8 @@ def make_pool(per_class=4, seed=500):
 
 
 def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
-                per_class=4, max_slots=64, slice_iters=8, seed=11, return_pool=False):
+                per_class=4, max_slots=64, slice_iters=8, seed=11, return_pool=False,
+                grid_ctas=0):
     """verify (optional): callable(gold, segm, result) -> bool, an independent
     check of each pool pair's result computed alone (the caller's checker).
     With return_pool, returns (summary, [(gold, segm, result alone)])."""
@@ -125,7 +126,7 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
     members = rng.integers(0, per_class, size=n)
     picks = [int(c) * per_class + int(m) for c, m in zip(classes, members)]
     srv = Server(hbm_budget=budget_bytes, max_slots=max_slots, slice_iters=slice_iters,
-                 max_dims=max(VOLUMES))
+                 max_dims=max(VOLUMES), grid_ctas=grid_ctas)
     lat, waits, ok = [], [], True
     id2req = {}
     done_n, nvox_done, max_in_flight, i = 0, 0, 0, 0

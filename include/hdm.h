@@ -264,7 +264,10 @@ int hdm_server_create(const hdm_server_opts *o, hdm_server **out);
 int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx, int ny,
                       int nz, int dtype, int label, int ptr_kind, uint64_t *id_out);
 /* Drive the engine: admit, launch one slice, retire.  Fills up to cap
- * completions.  Returns HDM_OK. */
+ * completions.  Returns HDM_OK.  A lone request (nothing in flight, nothing
+ * else waiting) completes in the step as one one-pair launch on the server's
+ * stream (the path of hdm_hausdorff_*, whose workspace is the library's and
+ * not the arena). */
 int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done);
 /* Requests queued, in flight, and device bytes held. */
 int hdm_server_load(hdm_server *s, int *queued, int *in_flight, uint64_t *bytes_used);

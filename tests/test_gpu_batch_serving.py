@@ -167,20 +167,55 @@ def test_server_mix_equals_alone():
 
 
 def test_servers_in_a_row_each_one_slice():
-    """Each server here runs exactly one slice, so the one created after it
-    may get the same pinned block back with the first slice's ready marker
-    still in it; every server must still wait for its own slice."""
+    """Each server here runs exactly one slice (two requests, admitted
+    together), so the one created after it may get the same pinned block back
+    with the first slice's ready marker still in it; every server must still
+    wait for its own slice."""
     vols = mixed_pairs(2, seed=37)
     alone = [hdm.hausdorff(g, s) for g, s in vols]
     for k in range(6):
         srv = serving.Server(hbm_budget=1 << 30, max_slots=4, slice_iters=64,
                              max_dims=(256, 128, 100))
-        g, s = vols[k % 2]
-        rid = srv.submit(g, s)
+        order = [k % 2, 1 - k % 2]
+        rids = [srv.submit(*vols[i]) for i in order]
         done = srv.drain(60)
         srv.close()
-        assert [c["id"] for c in done] == [rid]
-        assert done[0]["status"] == 0 and same(done[0], alone[k % 2]), k
+        assert sorted(c["id"] for c in done) == sorted(rids)
+        for c in done:
+            i = order[rids.index(c["id"])]
+            assert c["status"] == 0 and same(c, alone[i]), k
+
+
+@pytest.mark.parametrize("robust,conn", [(1.0, 6), (0.9, 6), (1.0, 18)])
+def test_lone_requests_equal_alone(robust, conn):
+    """A request that finds nothing in flight and nothing else waiting runs as
+    the one-pair call on the server's stream and completes in the same step:
+    its result equals the pair computed alone (device and host pointers,
+    int32 volumes, a robust cut, the 18 element), and an empty mask completes
+    it with status -2."""
+    vols = mixed_pairs(3, seed=41)
+    alone = [hdm.hausdorff(g, s, robustness=robust, connectivity=conn) for g, s in vols]
+    srv = serving.Server(hbm_budget=1 << 30, max_slots=4, robustness=robust, connectivity=conn,
+                         max_dims=(256, 128, 100))
+    for i, (g, s) in enumerate(vols):
+        for kind in ("device", "host", "i32"):
+            if kind == "host":
+                gg, ss = g.cpu().numpy(), s.cpu().numpy()
+            elif kind == "i32":
+                gg, ss = g.int(), s.int()
+            else:
+                gg, ss = g, s
+            rid = srv.submit(gg, ss)
+            done = srv.step()
+            assert [c["id"] for c in done] == [rid], (i, kind)
+            assert done[0]["status"] == 0 and same(done[0], alone[i]), (i, kind)
+    g0 = torch.zeros((20, 30, 40), dtype=torch.uint8, device="cuda")
+    s0 = g0.clone()
+    s0[5, 5, 5] = 1
+    rid = srv.submit(g0, s0)
+    done = srv.step()
+    assert [(c["id"], c["status"]) for c in done] == [(rid, -2)]
+    srv.close()
 
 
 def test_server_errors_budget_and_capacity():
