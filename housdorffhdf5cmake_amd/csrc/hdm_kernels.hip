@@ -767,7 +767,8 @@ __device__ __forceinline__ uint32_t flush_local_queue(LocalQueue &lq, uint64_t *
 // ---------------------------------------------------------------- seed
 // Bit m of the result: an empty tile is reached in one step of the element
 // from a non-empty neighbour of mask m, judged from the neighbours' tile
-// flags (flags(nb) loads them).  CONN 6: a face neighbour has voxels on its
+// flags (flags(j, nb) gives those of neighbour nb at offset index
+// j = (ox+1) + 3(oy+1) + 9(oz+1)).  CONN 6: a face neighbour has voxels on its
 // face toward the tile (hello.cu:1649-1663).  CONN 18 / 26: a neighbour at
 // offset o among the 26 has voxels on each of its faces that o crosses (the
 // per-face flags make this a superset of the tiles reached; a tile queued
@@ -790,7 +791,7 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
       if (ux < b.lo[0] || ux > b.hi[0] || uy < b.lo[1] || uy > b.hi[1] || uz < b.lo[2] ||
           uz > b.hi[2])
         continue;
-      const uint32_t f = flags((uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy));
+      const uint32_t f = flags(j, (uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy));
       // the neighbour's faces toward this tile: one layer (bits 2..7) or two (8..13)
       const int base = nzc == 1 ? 8 : 2;
       const uint32_t need = (ox ? 1u << (base + (ox > 0 ? 0 : 1)) : 0u) |
@@ -803,11 +804,12 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
     // bit of the neighbour's flags that marks its face toward this tile, per
     // direction (0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z)
     constexpr int kFacing[6] = {3, 2, 5, 4, 7, 6};
+    constexpr int kJ[6] = {12, 14, 10, 16, 4, 22};  // offset index of each direction
 #pragma unroll
     for (int dir = 0; dir < 6; ++dir) {
       const uint32_t nb = neighbour(d, b, tile, tx, ty, tz, dir);
       if (nb == kNoTile) continue;
-      const uint32_t f = flags(nb);
+      const uint32_t f = flags(kJ[dir], nb);
 #pragma unroll
       for (int m = 0; m < 2; ++m) act |= ((f >> (16 * m + kFacing[dir])) & 1u) << m;
     }
@@ -821,7 +823,7 @@ __device__ __forceinline__ uint32_t seed_reach(const G &d, const Box &b, uint32_
           uz > b.hi[2])
         continue;
       const uint32_t nb = (uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy);
-      const uint32_t f = flags(nb);
+      const uint32_t f = flags(j, nb);
       // the neighbour's faces toward this tile: -x (bit 2) when it lies at +x
       const uint32_t need = (ox ? 1u << (ox > 0 ? 2 : 3) : 0u) | (oy ? 1u << (oy > 0 ? 4 : 5) : 0u) |
                             (oz ? 1u << (oz > 0 ? 6 : 7) : 0u);
@@ -904,7 +906,7 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
     uint32_t own = 0, reach = 0;
     if (in_box) {
       own = d.tflags[tile];
-      reach = seed_reach<CONN, DS>(d, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
+      reach = seed_reach<CONN, DS>(d, b, tile, tx, ty, tz, [&](int, uint32_t nb) -> uint32_t {
         return d.tflags[nb];
       });
     }
@@ -1527,6 +1529,30 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     if (run) {
       __shared__ unsigned long long s_tot[4];
       __shared__ int s_box[6];
+      // the flags of this thread's first tile and of its neighbours in the
+      // tile grid load together with the pack partials, one round trip for
+      // both; the box, known once the partials are summed, selects among them
+      constexpr bool kDiag = DS || CONN != 6;  // neighbours beyond the six faces
+      uint32_t pf_own = 0, pf[27];
+      {
+        const uint64_t g = (uint64_t)blockIdx.x * kWgThreads + threadIdx.x;
+        const bool in = g < a.total_tiles;
+        const uint32_t tile = in ? (uint32_t)g : 0u;
+        const int tx = (int)(tile % (uint32_t)d.gx);
+        const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+        const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+        if (in) pf_own = ld_sc(d.tflags + tile);
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+          pf[j] = 0;
+          const int ox = j % 3 - 1, oy = (j / 3) % 3 - 1, oz = j / 9 - 1;
+          const int nzc = (ox != 0) + (oy != 0) + (oz != 0);
+          if (nzc == 0 || (!kDiag && nzc > 1) || (DS && nzc == 3)) continue;
+          const int ux = tx + ox, uy = ty + oy, uz = tz + oz;
+          if (in && ux >= 0 && ux < d.gx && uy >= 0 && uy < d.gy && uz >= 0 && uz < d.gz)
+            pf[j] = ld_sc(d.tflags + (uint32_t)((int)tile + ox + oy * d.gx + oz * d.gx * d.gy));
+        }
+      }
       if (threadIdx.x < 64) {
         unsigned long long v[4];
 #pragma unroll
@@ -1578,12 +1604,14 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         const Box &b = v0.b;
         const bool in_box = in && tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] &&
                             ty <= b.hi[1] && tz >= b.lo[2] && tz <= b.hi[2];
+        const bool first = tb == (uint64_t)blockIdx.x * kWgThreads;  // prefetched
         uint32_t own = 0, reach = 0;
         if (in_box) {
-          own = ld_sc(d.tflags + tile);
-          reach = seed_reach<CONN, DS>(v0, b, tile, tx, ty, tz, [&](uint32_t nb) -> uint32_t {
-            return ld_sc(d.tflags + nb);
-          });
+          own = first ? pf_own : ld_sc(d.tflags + tile);
+          reach = seed_reach<CONN, DS>(v0, b, tile, tx, ty, tz,
+                                       [&](int j, uint32_t nb) -> uint32_t {
+                                         return first ? pf[j] : ld_sc(d.tflags + nb);
+                                       });
         }
         bool want[2];
         uint64_t ent[2];

@@ -42,6 +42,8 @@ class Server:
         self._done = (Completion * 256)()
 
     def submit(self, gold, segm, label=1) -> int:
+        """Queue a pair; device tensors must be complete (synchronize after
+        writing them): the server's stream does not wait for torch's."""
         gp, sp, nx, ny, nz, dt, pk, keep = _volume_args(gold, segm)
         rid = C.c_uint64()
         check(lib().hdm_server_submit(self.h, gp, sp, nx, ny, nz, dt, label, pk, C.byref(rid)),

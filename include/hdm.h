@@ -256,7 +256,8 @@ typedef struct hdm_completion {
 void hdm_server_opts_init(hdm_server_opts *o);
 int hdm_server_create(const hdm_server_opts *o, hdm_server **out);
 /* Queue a request.  With ptr_kind HDM_PTR_DEVICE, gold/segm are resident
- * device volumes (no copy); with HDM_PTR_HOST the server copies them on its
+ * device volumes (no copy), complete when submitted: the server's streams do
+ * not wait for the caller's; with HDM_PTR_HOST the server copies them on its
  * copy stream once admitted (pinned host memory lets the copy overlap the
  * engine).  The volumes must stay valid until the request completes.
  * HDM_E_BUDGET if the request alone exceeds the HBM budget, HDM_E_CAPACITY

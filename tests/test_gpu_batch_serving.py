@@ -203,6 +203,7 @@ def test_lone_requests_equal_alone(robust, conn):
                 gg, ss = g.cpu().numpy(), s.cpu().numpy()
             elif kind == "i32":
                 gg, ss = g.int(), s.int()
+                torch.cuda.synchronize()  # the server's stream does not wait for torch's
             else:
                 gg, ss = g, s
             rid = srv.submit(gg, ss)
@@ -212,6 +213,7 @@ def test_lone_requests_equal_alone(robust, conn):
     g0 = torch.zeros((20, 30, 40), dtype=torch.uint8, device="cuda")
     s0 = g0.clone()
     s0[5, 5, 5] = 1
+    torch.cuda.synchronize()
     rid = srv.submit(g0, s0)
     done = srv.step()
     assert [(c["id"], c["status"]) for c in done] == [(rid, -2)]
@@ -223,6 +225,7 @@ def test_server_errors_budget_and_capacity():
     g0 = torch.zeros((16, 16, 64), dtype=torch.uint8, device="cuda")
     s0 = g0.clone()
     s0[3, 3, 3] = 1
+    torch.cuda.synchronize()  # the server's stream does not wait for torch's
     srv = serving.Server(hbm_budget=64 << 20, max_slots=2, slice_iters=4, max_dims=(256, 128, 100))
     ids = {srv.submit(g, s): i for i, (g, s) in enumerate(vols)}
     bad = srv.submit(g0, s0)
