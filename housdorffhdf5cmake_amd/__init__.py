@@ -187,12 +187,21 @@ def _is_torch(x) -> bool:
 _OPTS_DEFAULT = None
 
 
+_RAW_STREAM = None
+
+
 def _current_stream_ptr():
-    import torch
-    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
-    if raw is not None:  # the same handle as current_stream().cuda_stream, without the object
-        return C.c_void_p(raw(torch.cuda.current_device()))
-    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """torch's current stream on the current device (a call's default)."""
+    global _RAW_STREAM
+    if _RAW_STREAM is None:
+        import torch
+        raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        if raw is not None:  # the handle of current_stream().cuda_stream, without the object
+            dev = torch.cuda.current_device
+            _RAW_STREAM = lambda: raw(dev())
+        else:
+            _RAW_STREAM = lambda: torch.cuda.current_stream().cuda_stream
+    return C.c_void_p(_RAW_STREAM())
 
 
 _TORCH_DTYPES = None
@@ -204,7 +213,7 @@ def _volume_args(gold, segm):
     shape = gold.shape
     if shape != segm.shape or len(shape) != 3:
         raise ValueError(f"need two equal 3-D volumes, got {tuple(gold.shape)} {tuple(segm.shape)}")
-    nz, ny, nx = (int(v) for v in shape)
+    nz, ny, nx = int(shape[0]), int(shape[1]), int(shape[2])
     if _is_torch(gold):
         if not (gold.is_cuda and segm.is_cuda):
             gold, segm = gold.numpy(), segm.numpy()

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -116,6 +117,9 @@ struct hdm_server {
   std::vector<hipEvent_t> event_pool;
   void *host = nullptr;  // pinned: outs + engine head + ready marker (zeroed at creation)
   uint32_t seq = 0;      // ready marker value of the last slice
+  // lone requests run as one-pair calls (off with HDM_SERVER_LONE=0, or for
+  // good once that call found no device memory for its workspace)
+  bool lone = true;
   uint64_t next_id = 1;
   uint32_t t_engine = 0;
 };
@@ -180,6 +184,10 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   hdm_server *s = new hdm_server();
   s->o = o;
   s->ctx = c;
+  {
+    const char *e = std::getenv("HDM_SERVER_LONE");
+    s->lone = !(e && std::atoi(e) == 0);
+  }
   s->tile = lid;
   s->conn = conn;
   s->tl = hdm_tile_launch(s->tile);
@@ -355,12 +363,19 @@ static int run_lone(hdm_server *s, hdm_completion *done, int *n_done) {
 extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done) {
   if (!s || (!done && cap > 0) || !n_done) return HDM_E_INVALID;
   *n_done = 0;
-  if (s->pending.size() == 1 && cap > 0 &&
+  if (s->lone && s->pending.size() == 1 && cap > 0 &&
       (s->pending.front().dtype != HDM_DTYPE_U8 ||
        (s->pending.front().label >= 0 && s->pending.front().label <= 255))) {
     bool idle = true;
     for (auto &r : s->slots) idle = idle && r.id == 0;
-    if (idle) return run_lone(s, done, n_done);
+    if (idle) {
+      const int rc = run_lone(s, done, n_done);
+      if (rc != HDM_E_NOMEM) return rc;
+      // no device memory for the one-pair workspace (the arena may hold most
+      // of it): this request and every later one go through the slots
+      s->lone = false;
+      (void)hipGetLastError();
+    }
   }
   // admit in arrival order while a slot and arena room exist
   int admitted = 0;

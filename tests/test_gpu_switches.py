@@ -1,8 +1,10 @@
 """The comparison switches: HDM_DS=0 (one dilation step per grid barrier) and
 HDM_FUSED=0 (a single pair as four kernels) give the same results as the
 default two-step fused launch, bit for bit, for single pairs, batches and
-robust cuts.  The switches are read once per process, so the switched runs
-happen in a child process on the same GPU."""
+robust cuts; HDM_SERVER_LONE=0 (a lone server request through a slot) gives
+the same results as the lone one-pair launch.  The switches are read once per
+process (or server), so the switched runs happen in a child process on the
+same GPU."""
 from __future__ import annotations
 
 import json
@@ -116,3 +118,41 @@ def test_barrier_timeout_is_reported_and_the_next_calls_are_exact():
     assert out["batch_timeout"] == -5
     assert out["after"] == out["before"]
     assert out["batch_after"] == out["batch_before"]
+
+
+CHILD_SERVE = r"""
+import json, sys
+sys.path.insert(0, sys.argv[1])
+import torch
+import housdorffhdf5cmake_amd as hdm
+from housdorffhdf5cmake_amd import serving
+specs = json.loads(sys.argv[2])
+keys = json.loads(sys.argv[3])
+vols = [hdm.synth_device(hdm.synth_spec(s[0], *s[1:])) for s in specs]
+torch.cuda.synchronize()
+srv = serving.Server(hbm_budget=1 << 30, max_slots=4, max_dims=(160, 128, 128))
+out = []
+for g, s in vols:
+    rid = srv.submit(g, s)
+    done = srv.drain(60)
+    out.append([done[0]["id"] == rid, len(done), done[0]["status"]] + [done[0][k] for k in keys])
+srv.close()
+print(json.dumps(out))
+"""
+
+
+def test_server_lone_switch_gives_the_same_results():
+    """Requests submitted one at a time, each finding the server idle: by
+    default each runs as a one-pair launch; with HDM_SERVER_LONE=0 each goes
+    through a slot and the several-pair kernel.  Both equal the pair computed
+    alone."""
+    keys = [k for k in KEYS if k != "iterations"]
+    vols = [hdm.synth_device(hdm.synth_spec(s[0], *s[1:])) for s in SPECS]
+    torch.cuda.synchronize()
+    want = [[True, 1, 0] + [hdm.hausdorff(g, s)[k] for k in keys] for g, s in vols]
+    for lone in ("1", "0"):
+        env = dict(os.environ, HDM_SERVER_LONE=lone)
+        r = subprocess.run([sys.executable, "-c", CHILD_SERVE, ROOT, json.dumps(SPECS),
+                            json.dumps(keys)], capture_output=True, text=True, timeout=240, env=env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        assert json.loads(r.stdout.strip().splitlines()[-1]) == want, lone
