@@ -26,6 +26,7 @@
 // sub-queues, and the grid barrier uses one arrival flag per workgroup.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <type_traits>
 
 #include "hdm_plan.h"
@@ -93,23 +94,64 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// sum over the aligned group of R lanes (R a power of two <= 64)
+// Wave-wide reductions through DPP moves (VALU latency, where a butterfly of
+// ds_bpermute shuffles waits on the LDS pipe at every level): an inclusive
+// scan inside each row of 16 lanes (row_shr 1, 2, 4, 8), then the total of
+// row 0 into row 1 and of row 2 into row 3 (row_bcast 15), then rows 0-1
+// into rows 2-3 (row_bcast 31); lane 63 holds the result, which every lane
+// reads back (readlane).  A source lane outside the row, or a row outside the
+// step's row mask, contributes the identity (bound_ctrl off: the move keeps
+// its old value).  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp32(int v, int identity) {
+  return __builtin_amdgcn_update_dpp(identity, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = (uint32_t)dpp32<CTRL, ROWS>((int)(uint32_t)v, 0);
+  const uint32_t hi = (uint32_t)dpp32<CTRL, ROWS>((int)(uint32_t)(v >> 32), 0);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+template <typename Op>
+__device__ __forceinline__ int wave_reduce32(int v, int identity, Op op) {
+  v = op(v, dpp32<0x111, 0xF>(v, identity));  // row_shr:1
+  v = op(v, dpp32<0x112, 0xF>(v, identity));  // row_shr:2
+  v = op(v, dpp32<0x114, 0xF>(v, identity));  // row_shr:4
+  v = op(v, dpp32<0x118, 0xF>(v, identity));  // row_shr:8
+  v = op(v, dpp32<0x142, 0xA>(v, identity));  // row_bcast:15 into rows 1, 3
+  v = op(v, dpp32<0x143, 0xC>(v, identity));  // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
+  v += dpp64<0x111, 0xF>(v);
+  v += dpp64<0x112, 0xF>(v);
+  v += dpp64<0x114, 0xF>(v);
+  v += dpp64<0x118, 0xF>(v);
+  v += dpp64<0x142, 0xA>(v);
+  v += dpp64<0x143, 0xC>(v);
+  return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63) |
+         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32);
+}
+
+// sum over the aligned group of R lanes (R a power of two <= 64), in every
+// lane of the group; a whole wave (R = 64) must be active
 template <int R>
 __device__ __forceinline__ uint32_t group_sum(uint32_t v, int lane) {
+  if constexpr (R == 64) {
+    return (uint32_t)wave_reduce32((int)v, 0,
+                                   [](int a, int b) { return (int)((uint32_t)a + (uint32_t)b); });
+  } else {
 #pragma unroll
-  for (int off = R / 2; off >= 1; off >>= 1) v += shfl32(v, lane ^ off);
-  return v;
+    for (int off = R / 2; off >= 1; off >>= 1) v += shfl32(v, lane ^ off);
+    return v;
+  }
 }
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v, int lane) { return group_sum<64>(v, lane); }
-__device__ __forceinline__ int wave_min(int v, int lane) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = min(v, (int)shfl32((uint32_t)v, lane ^ off));
-  return v;
+__device__ __forceinline__ int wave_min(int v, int) {
+  return wave_reduce32(v, INT_MAX, [](int a, int b) { return min(a, b); });
 }
-__device__ __forceinline__ int wave_max(int v, int lane) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v = max(v, (int)shfl32((uint32_t)v, lane ^ off));
-  return v;
+__device__ __forceinline__ int wave_max(int v, int) {
+  return wave_reduce32(v, INT_MIN, [](int a, int b) { return max(a, b); });
 }
 
 // bits of the R-lane group holding `lane` out of a wave ballot, shifted to bit 0
@@ -576,8 +618,7 @@ __global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     v[f] = c.sums[f][lane];
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+    v[f] = wave_sum64(v[f]);
   }
   if (lane == 0) c.n_gold = v[0], c.n_segm = v[1], c.n_fp = v[2], c.n_fn = v[3];
   int bx[6];
@@ -862,8 +903,7 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       v[f] = c.sums[f][lane];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+      v[f] = wave_sum64(v[f]);
     }
     int bx[6];
 #pragma unroll
@@ -1131,15 +1171,12 @@ __device__ __forceinline__ bool grid_barrier_single(
     // per sub-queue
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x < kSubQ) s_red[12 + threadIdx.x] = 0;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      h0 += shfl64(h0, lane ^ off);
-      h1 += shfl64(h1, lane ^ off);
-      nl += shfl64(nl, lane ^ off);
-      if constexpr (DS) {
-        g0 += shfl64(g0, lane ^ off);
-        g1 += shfl64(g1, lane ^ off);
-      }
+    h0 = wave_sum64(h0);
+    h1 = wave_sum64(h1);
+    nl = wave_sum64(nl);
+    if constexpr (DS) {
+      g0 = wave_sum64(g0);
+      g1 = wave_sum64(g1);
     }
     __syncthreads();
     if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1, s_red[8 + wave] = nl;
@@ -1558,8 +1595,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
           v[f] = ld_sc(&acc->sums[f][lane]);
-#pragma unroll
-          for (int off = 32; off >= 1; off >>= 1) v[f] += shfl64(v[f], lane ^ off);
+          v[f] = wave_sum64(v[f]);
         }
         int bx[6];
 #pragma unroll
