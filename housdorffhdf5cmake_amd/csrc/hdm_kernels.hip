@@ -122,6 +122,17 @@ __device__ __forceinline__ int wave_reduce32(int v, int identity, Op op) {
   v = op(v, dpp32<0x143, 0xC>(v, identity));  // row_bcast:31 into rows 2, 3
   return __builtin_amdgcn_readlane(v, 63);
 }
+// inclusive prefix sum over the 64 lanes (the same DPP steps, no read-back)
+__device__ __forceinline__ uint32_t wave_scan32(uint32_t x) {
+  int v = (int)x;
+  v += dpp32<0x111, 0xF>(v, 0);
+  v += dpp32<0x112, 0xF>(v, 0);
+  v += dpp32<0x114, 0xF>(v, 0);
+  v += dpp32<0x118, 0xF>(v, 0);
+  v += dpp32<0x142, 0xA>(v, 0);
+  v += dpp32<0x143, 0xC>(v, 0);
+  return (uint32_t)v;
+}
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
   v += dpp64<0x111, 0xF>(v);
   v += dpp64<0x112, 0xF>(v);
@@ -745,8 +756,8 @@ __device__ __forceinline__ void push_entries(bool want, uint64_t ent, LocalQueue
       cur = prev;
     }
   }
-  base = (uint32_t)__shfl((int)base, 0);
-  global = __shfl((int)global, 0) != 0;
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);  // lane 0's, as a scalar
+  global = __builtin_amdgcn_readlane((int)global, 0) != 0;
   if (want) {
     const uint32_t pos = base + lanes_below(m);
     if (global)
@@ -780,12 +791,12 @@ __device__ __forceinline__ void push_sticky(bool want, uint64_t ent, uint64_t *s
       cur = prev;
     }
   }
-  kept = __shfl((int)kept, 0) != 0;
+  kept = __builtin_amdgcn_readlane((int)kept, 0) != 0;  // lane 0's, as a scalar
   if (!kept) {  // staged; straight to gq (counted in *pushed) when lq is full too
     push_entries(want, ent, lq, gq, gq_len, lane, pushed);
     return;
   }
-  base = (uint32_t)__shfl((int)base, 0);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, 0);
   if (want) st[base + lanes_below(m)] = ent;
 }
 
@@ -1191,12 +1202,7 @@ __device__ __forceinline__ bool grid_barrier_single(
       const uint32_t wb = threadIdx.x * c, we = min(nwg, wb + c);
       uint32_t part = 0;
       for (uint32_t w = wb; w < we; ++w) part += slots - min(slots, (uint32_t)s_kept[w]);
-      uint32_t x = part;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off);
-        if (lane >= off) x += y;
-      }
+      const uint32_t x = wave_scan32(part);  // inclusive
       __shared__ uint32_t s_wsum[kWavesPerWg];
       if (lane == 63) s_wsum[wave] = x;
       __syncthreads();
