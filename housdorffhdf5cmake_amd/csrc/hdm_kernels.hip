@@ -144,6 +144,20 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t v) {
          ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63) << 32);
 }
 
+// A 64-bit row from lane - 1, + 1, - 2 or + 2 inside the same row of 16
+// lanes (DPP row_shr / row_shl; a source outside the row reads 0): for the y
+// neighbours of a 64x8x8 tile's rows, whose 8-lane y groups never straddle a
+// row of 16, where a ds_bpermute shuffle would wait on the LDS pipe.
+template <int CTRL>
+__device__ __forceinline__ uint64_t row_dpp64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, true);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, true);
+  return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+constexpr int kFromLeft1 = 0x111, kFromLeft2 = 0x112;    // row_shr: lane - 1, lane - 2
+constexpr int kFromRight1 = 0x101, kFromRight2 = 0x102;  // row_shl: lane + 1, lane + 2
+
 // sum over the aligned group of R lanes (R a power of two <= 64), in every
 // lane of the group; a whole wave (R = 64) must be active
 template <int R>
@@ -1911,19 +1925,21 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           // rows y+-1, z+-1; f2: rows y+-2, z+-2 and the four (y+-1, z+-1)
           Row f1, f2;
           {
-            Row t = shfl_row<Row>(own, lane - 1);
+            // y neighbours (lane +- 1, +- 2) through row DPP: each is used
+            // only where its source lane is in the same y group
+            Row t = row_dpp64<kFromLeft1>(own);
             f1 = yl > 0 ? t : ey1;
-            t = shfl_row<Row>(own, lane + 1);
+            t = row_dpp64<kFromRight1>(own);
             f1 |= yl < TY - 1 ? t : ey1;
             t = shfl_row<Row>(own, lane - TY);
             f1 |= zl > 0 ? t : ez1;
             t = shfl_row<Row>(own, lane + TY);
             f1 |= zl < TZ - 1 ? t : ez1;
-            Row e = shfl_row<Row>(ey1, lane - 1);
-            t = shfl_row<Row>(own, lane - 2);
+            Row e = row_dpp64<kFromLeft1>(ey1);
+            t = row_dpp64<kFromLeft2>(own);
             f2 = yl > 1 ? t : yl == 1 ? e : ey2;
-            e = shfl_row<Row>(ey1, lane + 1);
-            t = shfl_row<Row>(own, lane + 2);
+            e = row_dpp64<kFromRight1>(ey1);
+            t = row_dpp64<kFromRight2>(own);
             f2 |= yl < TY - 2 ? t : yl == TY - 2 ? e : ey2;
             e = shfl_row<Row>(ez1, lane - TY);
             t = shfl_row<Row>(own, lane - 2 * TY);
@@ -1934,7 +1950,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
             // (y-1, z-1), (y+1, z-1), (y-1, z+1), (y+1, z+1): inside the tile,
             // or an edge lane's neighbour in ey1 / ez1, or the corner's ed
             const Row eym8 = shfl_row<Row>(ey1, lane - TY), eyp8 = shfl_row<Row>(ey1, lane + TY);
-            const Row ezm1 = shfl_row<Row>(ez1, lane - 1), ezp1 = shfl_row<Row>(ez1, lane + 1);
+            const Row ezm1 = row_dpp64<kFromLeft1>(ez1), ezp1 = row_dpp64<kFromRight1>(ez1);
             t = shfl_row<Row>(own, lane - TY - 1);
             f2 |= yl > 0 ? (zl > 0 ? t : ezm1) : (zl > 0 ? eym8 : ed);
             t = shfl_row<Row>(own, lane - TY + 1);
