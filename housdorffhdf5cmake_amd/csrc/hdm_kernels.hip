@@ -2530,9 +2530,12 @@ __attribute__((amdgpu_waves_per_eu(!SINGLE && DS ? 4 : 1))) void dilate_kernel(D
   dilate_body<TXB, TY, TZ, SINGLE, FUSED, CONN, DS>(a);
 }
 
-// The fused launch with two dilation steps per barrier.  Its step needs
-// about 195 VGPRs (two resident workgroups per CU); held to three, it spills
-// in the step and measured slower, so it runs with two.
+// The fused launch with two dilation steps per barrier: about 185 VGPRs, two
+// resident workgroups per CU.  Held to three it spills in the step; with the
+// barrier master's state moved to LDS it fits three without spilling, but
+// every barrier period then grows with the 768 workgroups to poll and
+// release, and it measured no faster (the long pair slower), so it runs
+// with two.
 __global__ __launch_bounds__(kWgThreads) void dilate_ds_kernel(DilateArgs a) {
   dilate_body<64, 8, 8, true, true, 6, true>(a);
 }
