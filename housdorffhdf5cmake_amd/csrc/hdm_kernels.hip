@@ -2307,9 +2307,9 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       }
       return ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
     };
-    uint32_t gfirst = 0;  // global entries below this were taken in round 1
+    uint32_t gfirst = 0;  // global entries below this were taken in the first pass
     if constexpr (SINGLE) {
-      // round 1: slot s takes kept entry s, or else global entry
+      // first pass over the wave slots: slot s takes kept entry s, or else global entry
       // s_fs[0] + (s - kept) of this workgroup's free range
       const uint32_t nloc = s_stn[t & 1];
       const uint32_t s = (uint32_t)wave * TPW + sub;

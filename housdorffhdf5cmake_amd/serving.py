@@ -111,7 +111,7 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
                 per_class=4, max_slots=64, slice_iters=8, seed=11, return_pool=False,
                 grid_ctas=0):
     """verify (optional): callable(gold, segm, result) -> bool, an independent
-    check of each pool pair's result computed alone (the caller's checker).
+    check of each pool pair's result computed alone (the caller's own check).
     With return_pool, returns (summary, [(gold, segm, result alone)])."""
     import torch
     pool = make_pool(per_class)

@@ -1,8 +1,9 @@
 /*
  * hdm_oracle.c - CPU oracle for the Hausdorff dilation path.
  *
- * TEST INFRASTRUCTURE ONLY (checker for tests/, the smoke check, and the
- * benchmark's cpu_baseline leg).  Never linked into or called by the product.
+ * TEST INFRASTRUCTURE ONLY (the results tests/, the smoke check and the
+ * benchmark's cpu_baseline leg compare against).  Never linked into or called
+ * by the product.
  *
  * Follows the reference's algorithm (jakubMitura14/HousdorffHDF5cMake):
  *   - voxel on iff value == label ......................... hello.cu:453, 615

@@ -1,7 +1,7 @@
 /*
  * hdm_oracle.h - CPU oracle for the Hausdorff dilation path.  TEST
  * INFRASTRUCTURE ONLY: the test suite, the smoke check and the benchmark's CPU
- * baseline leg use it as the checker.  The product library never links or
+ * baseline leg compare results against it.  The product library never links or
  * calls it.
  *
  * Two restatements of the reference algorithm (hello.cu, mainPassKernel,

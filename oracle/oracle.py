@@ -1,7 +1,7 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-benchmark's cpu_baseline leg, as the checker.  The product package never
+benchmark's cpu_baseline leg, to compare results against.  The product package never
 imports it.
 """
 from __future__ import annotations
