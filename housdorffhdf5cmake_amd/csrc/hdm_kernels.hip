@@ -2297,15 +2297,20 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         claim_ent = new_ent;
     };
     // global entry ei of the queue of iteration t
+    // lane k holds the start of sub-queue k + 1 of this iteration's queue, read
+    // once per iteration (sub-queue 0 starts at 0)
+    const uint32_t pre_l = s_pre[1 + (lane & (kSubQ - 1))];
     auto global_entry = [&](uint32_t ei) {
       int sq = 0;
+      uint32_t start = 0;
       if constexpr (TPW == 1) {  // ei is wave-uniform: lane k < kSubQ - 1 compares start k + 1
-        const uint32_t pre = s_pre[1 + (lane & (kSubQ - 1))];
-        sq = __popcll(__ballot(lane < kSubQ - 1 && pre <= ei));
+        sq = __popcll(__ballot(lane < kSubQ - 1 && pre_l <= ei));
+        start = sq ? (uint32_t)__builtin_amdgcn_readlane((int)pre_l, sq - 1) : 0u;
       } else {
         while (sq + 1 < kSubQ && s_pre[sq + 1] <= ei) ++sq;
+        start = s_pre[sq];
       }
-      return ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - s_pre[sq]));
+      return ld_sc(sub_queue(q, qcap, t & 1, sq) + (ei - start));
     };
     uint32_t gfirst = 0;  // global entries below this were taken in the first pass
     if constexpr (SINGLE) {
