@@ -107,7 +107,7 @@ int barrier_mode();
 int wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq);
 bool fused_enabled();
 bool ds_enabled();
-int fused_grid(const Ctx *c, int tile, int requested, bool ds = false);
+int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles);
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,

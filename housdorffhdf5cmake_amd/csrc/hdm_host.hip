@@ -182,10 +182,16 @@ bool ds_enabled() {
   return on;
 }
 
-int fused_grid(const Ctx *c, int tile, int requested, bool ds) {
+// Default grid of the fused launch: one workgroup per two tiles, at least
+// 64, at most every resident one.  A small volume's iterations queue few
+// entries, and every workgroup costs the grid barriers an arrival to poll and
+// a release to read (a 64^3 pair measured 57 -> 54 us on 64 workgroups
+// instead of 512; pairs of 2048 tiles or more run fastest on all of them).
+int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles) {
   const int occ = ds ? c->occ_fused_ds[tile] : c->occ_fused[tile];
   const int maxres = std::min(std::max(1, occ) * c->cus, kMaxGrid);
-  return requested > 0 ? std::min(requested, maxres) : maxres;
+  if (requested > 0) return std::min(requested, maxres);
+  return (int)std::min<uint64_t>((uint64_t)maxres, std::max<uint64_t>(64, ntiles / 2));
 }
 
 int barrier_mode() {
@@ -524,7 +530,8 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     ds = tl->fused_ds && conn == 6 && ds_enabled();
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {
-      e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds)), s, a);
+      e = (ds ? tl->fused_ds : tl->fused)(
+          dim3(fused_grid(c, tile, opts.grid_ctas, ds, g.ntiles)), s, a);
       if (e == hipSuccess) c->acc_set ^= 1;  // the launch zeroes the other set first
     }
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
