@@ -106,6 +106,8 @@ int barrier_mode();
 // without it is an error, not a hang.
 int wait_ready(hipStream_t s, volatile uint32_t *ready, uint32_t seq);
 bool fused_enabled();
+// the fused single-pair launch is cooperative (HDM_COOP=1) or plain
+bool coop_enabled();
 bool ds_enabled();
 int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles);
 GroupPtrs carve_common(char *base, const CommonLayout &L);

@@ -171,6 +171,18 @@ bool fused_enabled() {
   return on;
 }
 
+// HDM_COOP=1 launches the fused single-pair kernel with
+// hipLaunchCooperativeKernel: the runtime then guarantees that the whole grid
+// is resident at once (or refuses the launch), where a plain launch relies on
+// the grid being sized within this process's occupancy.
+bool coop_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("HDM_COOP");
+    return e && std::atoi(e) == 1;
+  }();
+  return on;
+}
+
 // The dilation runs two steps per grid barrier (64x8x8 tiles, 6 element) --
 // the fused single-pair launch, batches and the server -- unless HDM_DS=0
 // asks for one.
@@ -400,6 +412,10 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const int tile = select_launch(opts.tile, opts.connectivity, &conn);
   if (tile < 0) return HDM_E_INVALID;
   if (!map_out && opts.map_out) map_out = opts.map_out;
+  // the map holds d(v) as uint16; the largest first hit in an nx x ny x nz
+  // box is nx + ny + nz - 3 steps (6 element; fewer for 18 and 26), so dims
+  // whose sum could exceed that are refused instead of wrapping
+  if (map_out && (int64_t)nx + ny + nz - 3 > 65535) return HDM_E_INVALID;
   Ctx *c = nullptr;
   int rc = get_ctx(&c);
   if (rc != HDM_OK) return rc;
@@ -527,6 +543,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.ready = cfg.ready;
     a.ready_seq = cfg.ready_seq;
     a.acc_set = c->acc_set;
+    a.coop = coop_enabled() ? 1 : 0;
     ds = tl->fused_ds && conn == 6 && ds_enabled();
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {

@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     for (int i = g; i < 4 * kLineWords; i += gs) es->qlen[0][i] = 0;
     for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel[0][i] = 0;
-    for (int i = g; i < kMaxGrid * 4; i += gs) es->arr[0][i] = 0;
+    constexpr int kArrWords = (int)(sizeof(es->arr) / sizeof(es->arr[0][0]));
+    for (int i = g; i < kArrWords; i += gs) es->arr[0][i] = 0;
     for (int i = g; i < kMaxGrid; i += gs) es->fslot[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel2[0][i] = 0;
   }
@@ -2571,16 +2572,27 @@ hipError_t launch_dilate(dim3 grid, hipStream_t s, const DilateArgs &a) {
   return hipGetLastError();
 }
 
+// The fused single-pair launch, plain or cooperative (a.coop): a cooperative
+// launch of a grid larger than can be resident at once fails with
+// hipErrorCooperativeLaunchTooLarge instead of starting.
+template <typename K>
+hipError_t launch_persistent(K kernel, dim3 grid, hipStream_t s, const DilateArgs &a) {
+  if (a.coop) {
+    DilateArgs copy = a;
+    void *args[] = {&copy};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(kernel), grid,
+                                      dim3(kWgThreads), args, 0, s);
+  }
+  hipLaunchKernelGGL(kernel, grid, dim3(kWgThreads), 0, s, a);
+  return hipGetLastError();
+}
 template <int TXB, int TY, int TZ, int CONN>
 hipError_t launch_fused(dim3 grid, hipStream_t s, const DilateArgs &a) {
-  hipLaunchKernelGGL((dilate_kernel<TXB, TY, TZ, true, true, CONN>), grid, dim3(kWgThreads), 0,
-                     s, a);
-  return hipGetLastError();
+  return launch_persistent(dilate_kernel<TXB, TY, TZ, true, true, CONN>, grid, s, a);
 }
 // two dilation steps per barrier (64x8x8, 6 element)
 hipError_t launch_fused_ds(dim3 grid, hipStream_t s, const DilateArgs &a) {
-  hipLaunchKernelGGL(dilate_ds_kernel, grid, dim3(kWgThreads), 0, s, a);
-  return hipGetLastError();
+  return launch_persistent(dilate_ds_kernel, grid, s, a);
 }
 
 // DS: every launch of the entry runs two dilation steps per engine iteration

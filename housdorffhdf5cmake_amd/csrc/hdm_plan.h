@@ -47,6 +47,10 @@ struct DilateArgs {
   uint32_t ready_seq;
   uint32_t *ready;
   int32_t acc_set;  // fused launch: the set of Engine::acc it packs into (0 or 1)
+  // fused launch: launch it cooperatively (hipLaunchCooperativeKernel), so the
+  // runtime refuses a grid that cannot be resident at once instead of letting
+  // the grid barrier wait for workgroups that never start (host choice only)
+  int32_t coop;
 };
 
 struct TileLaunch {

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/hdm.h"
@@ -122,6 +123,9 @@ struct hdm_server {
   bool lone = true;
   uint64_t next_id = 1;
   uint32_t t_engine = 0;
+  // completions of a failed slice beyond the caller's capacity, reported
+  // first by the next steps
+  std::deque<hdm_completion> backlog;
 };
 
 static hipEvent_t take_event(hdm_server *s) {
@@ -150,6 +154,13 @@ extern "C" double hdm_server_now(void) { return now_s(); }
 
 extern "C" void hdm_server_destroy(hdm_server *s) {
   if (!s) return;
+  if (s->ctx && s->s) {
+    // a lone request ran as a one-pair call on this stream and may have left
+    // it as the context's pending stream: settle it before the stream goes,
+    // so no later call synchronizes a destroyed (or reused) handle
+    std::lock_guard<std::mutex> lk(s->ctx->mu);
+    if (s->ctx->has_pending && s->ctx->pending == s->s) (void)s->ctx->settle();
+  }
   if (s->s) (void)hipStreamSynchronize(s->s);
   if (s->s_copy) (void)hipStreamSynchronize(s->s_copy);
   for (auto &r : s->slots)
@@ -363,7 +374,12 @@ static int run_lone(hdm_server *s, hdm_completion *done, int *n_done) {
 extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done) {
   if (!s || (!done && cap > 0) || !n_done) return HDM_E_INVALID;
   *n_done = 0;
-  if (s->lone && s->pending.size() == 1 && cap > 0 &&
+  while (!s->backlog.empty() && *n_done < cap) {  // a failed slice's leftovers
+    done[(*n_done)++] = s->backlog.front();
+    s->backlog.pop_front();
+  }
+  if (!s->backlog.empty()) return HDM_OK;
+  if (s->lone && s->pending.size() == 1 && cap > *n_done &&
       (s->pending.front().dtype != HDM_DTYPE_U8 ||
        (s->pending.front().label >= 0 && s->pending.front().label <= 255))) {
     bool idle = true;
@@ -425,7 +441,7 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   a.q = s->P.q;
   a.qcap = s->qcap;
   a.chunk = 1;
-  a.spin = s->cfg.spin;
+  a.spin = spin_ticks(s->ctx);  // read per slice (HDM_SPIN_TICKS)
   a.flags = barrier_mode();
   a.outs = (PairOut *)h;
   a.head = hhead;
@@ -448,6 +464,7 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   if (head.nwg == 0) return HDM_E_HIP;  // the launch did not complete
   s->t_engine = head.t;
   const double t_now = now_s();
+  static const uint32_t kUnused = 0xFFFFFFFFu;
   for (int p = 0; p < nslots; ++p) {
     Request &r = s->slots[p];
     if (r.id == 0) continue;
@@ -464,22 +481,40 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
       finished = true;
     }
     if (!finished) continue;
-    if (*n_done < cap) {
-      hdm_completion &c = done[(*n_done)++];
-      std::memset(&c, 0, sizeof(c));
-      c.id = r.id;
-      c.status = st;
-      fill_result(&c.result, o, nullptr, r.geom.ntiles, std::max(o.hd_fp, o.hd_fn));
-      c.t_submit = r.t_submit;
-      c.t_admit = r.t_admit;
-      c.t_done = t_now;
-    } else {
+    hdm_completion c;
+    std::memset(&c, 0, sizeof(c));
+    c.id = r.id;
+    c.status = st;
+    fill_result(&c.result, o, nullptr, r.geom.ntiles, std::max(o.hd_fp, o.hd_fn));
+    c.t_submit = r.t_submit;
+    c.t_admit = r.t_admit;
+    c.t_done = t_now;
+    if (*n_done < cap)
+      done[(*n_done)++] = c;
+    else if (head.error)
+      s->backlog.push_back(c);  // the engine restarts below: report it later
+    else
       continue;  // report it at the next step
-    }
+    // an unfinished pair leaves the engine's bookkeeping for good: with its
+    // start beyond every iteration, no later fold reads its counters or
+    // writes its (released) histogram
+    if (!o.done &&
+        hipMemcpyAsync(&s->P.descs[p].t_start, &kUnused, sizeof(kUnused), hipMemcpyHostToDevice,
+                       s->s) != hipSuccess)
+      return HDM_E_HIP;
     if (r.copied) s->event_pool.push_back(r.copied);
     s->arena.release(r.off, r.bytes);
     r = Request();
     s->freed_after[p] = s->slices;
+  }
+  if (head.error) {
+    // a barrier wait expired and every request in flight failed with it: the
+    // engine restarts at iteration 0 with an empty queue and a clear error
+    // flag (otherwise every later slice would fail at its first barrier)
+    const hipError_t e2 = hdm_launch_reset(s->s, s->P.descs, nullptr, nullptr, s->P.idx,
+                                           s->P.ctrs, 0, 0, s->P.engine, 1, s->qcap);
+    if (e2 != hipSuccess) return map_hip(e2);
+    s->t_engine = 0;
   }
   (void)admitted;
   return HDM_OK;

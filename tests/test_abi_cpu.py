@@ -101,6 +101,25 @@ def test_invalid_arguments_are_rejected_before_any_device_work():
     assert L.hdm_server_create(C.byref(so), C.byref(C.c_void_p())) == -1
 
 
+def test_first_hit_map_refuses_dims_whose_distances_overflow_uint16():
+    """d(v) <= nx + ny + nz - 3 in a box; the map stores uint16, so dims whose
+    sum minus 3 exceeds 65535 are refused (HDM_E_INVALID) on the host before
+    any device work, instead of wrapping.  The pointers are never read."""
+    L = hdm.lib()
+    r = hdm.Result()
+    buf = np.zeros(16, np.uint8)
+    p = C.c_void_p(buf.ctypes.data)
+    for dims in ((65535, 2, 3), (30000, 30000, 5539), (1, 65535, 4)):
+        assert sum(dims) - 3 > 65535
+        assert L.hdm_first_hit_map_u8(p, p, *dims, 1, None, p, C.byref(r)) == -1, dims
+    # without a map the same dims are not refused for that reason (the call
+    # then needs a device; only the argument check is exercised here)
+    o = hdm.Opts()
+    L.hdm_opts_init(C.byref(o))
+    o.map_out = p
+    assert L.hdm_hausdorff_u8(p, p, 65535, 2, 3, 1, C.byref(o), C.byref(r)) == -1
+
+
 def test_cpp_compat_header_compiles_and_reports_errors(tmp_path):
     """include/hdm_compat.hpp: the reference's template signature over the C
     ABI.  Built with g++ and linked against libhdm.so; the null-pointer call

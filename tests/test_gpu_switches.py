@@ -61,10 +61,12 @@ def run_here():
     return out
 
 
-@pytest.mark.parametrize("switch", ["HDM_DS", "HDM_FUSED"])
-def test_switched_off_paths_give_the_same_results(switch):
+@pytest.mark.parametrize("switch,value", [("HDM_DS", "0"), ("HDM_FUSED", "0"), ("HDM_COOP", "1")])
+def test_switched_off_paths_give_the_same_results(switch, value):
+    """HDM_COOP=1 (the fused launch as a cooperative launch) is compared the
+    same way."""
     here = run_here()
-    env = dict(os.environ, **{switch: "0"})
+    env = dict(os.environ, **{switch: value})
     r = subprocess.run([sys.executable, "-c", CHILD, ROOT, json.dumps(SPECS), json.dumps(KEYS)],
                        capture_output=True, text=True, timeout=240, env=env)
     assert r.returncode == 0, r.stderr[-2000:]
@@ -156,3 +158,65 @@ def test_server_lone_switch_gives_the_same_results():
                             json.dumps(keys)], capture_output=True, text=True, timeout=240, env=env)
         assert r.returncode == 0, r.stderr[-2000:]
         assert json.loads(r.stdout.strip().splitlines()[-1]) == want, lone
+
+
+CHILD_SERVER_TIMEOUT = r"""
+import json, os, sys
+sys.path.insert(0, sys.argv[1])
+import torch
+import housdorffhdf5cmake_amd as hdm
+from housdorffhdf5cmake_amd import serving
+keys = json.loads(sys.argv[2])
+vols = [hdm.synth_device(hdm.synth_spec("ellipsoids", 160, 128, 96, 8, 6, 24, 2, 0.005, 41)),
+        hdm.synth_device(hdm.synth_spec("ellipsoids", 67, 45, 31, 6, 4, 20, 2, 0.01, 7))]
+torch.cuda.synchronize()
+alone = [{k: hdm.hausdorff(g, s)[k] for k in keys} for g, s in vols]
+srv = serving.Server(hbm_budget=1 << 30, max_slots=4, max_dims=(160, 128, 128))
+def round_trip():
+    ids = [srv.submit(g, s) for g, s in vols]  # two at once: the slot path
+    done = srv.drain(60)
+    by = {d["id"]: d for d in done}
+    return [[by[i]["status"]] + [by[i][k] for k in keys] for i in ids]
+os.environ["HDM_SPIN_TICKS"] = "0"  # every barrier wait of the slice expires at once
+failed = round_trip()
+del os.environ["HDM_SPIN_TICKS"]
+after = [round_trip() for _ in range(2)]
+srv.close()
+print(json.dumps({"failed": [f[0] for f in failed], "after": after,
+                  "want": [[0] + [a[k] for k in keys] for a in alone]}))
+"""
+
+
+def test_server_recovers_after_a_barrier_timeout():
+    """A slice whose grid-barrier wait expires (HDM_SPIN_TICKS=0, read per
+    slice) fails every request in flight with HDM_E_TIMEOUT (-5); the engine
+    then restarts (iteration 0, empty queue, error flag cleared), so the
+    requests submitted after it, with the normal bound, give exactly the
+    results of the pairs computed alone."""
+    keys = [k for k in KEYS if k != "iterations"]
+    r = subprocess.run([sys.executable, "-c", CHILD_SERVER_TIMEOUT, ROOT, json.dumps(keys)],
+                       capture_output=True, text=True, timeout=240,
+                       env={k: v for k, v in os.environ.items() if k != "HDM_SPIN_TICKS"})
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["failed"] == [-5, -5]
+    assert out["after"] == [out["want"], out["want"]]
+
+
+def test_lone_request_then_server_destroyed_then_single_call():
+    """A lone request runs as a one-pair call on the server's stream; after
+    the server (and that stream) is destroyed, a one-pair call on torch's
+    stream succeeds and is exact (the library no longer waits for the
+    destroyed stream)."""
+    from housdorffhdf5cmake_amd import serving
+    g, s = hdm.synth_device(hdm.synth_spec("ellipsoids", 96, 80, 64, 6, 4, 20, 2, 0.01, 5))
+    torch.cuda.synchronize()
+    want = hdm.hausdorff(g, s)
+    for _ in range(3):
+        srv = serving.Server(hbm_budget=1 << 30, max_slots=4, max_dims=(128, 128, 128))
+        rid = srv.submit(g, s)
+        done = srv.drain(60)
+        srv.close()
+        assert len(done) == 1 and done[0]["id"] == rid and done[0]["status"] == 0
+        got = hdm.hausdorff(g, s)  # on torch's current stream
+        assert all(got[k] == want[k] for k in KEYS)
