@@ -140,6 +140,9 @@ def lib() -> C.CDLL:
         "hdm_hausdorff_bool": ([vp, vp, i32, i32, i32, C.POINTER(Opts), C.POINTER(Result)], i32),
         "hdm_first_hit_map_u8": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts), vp,
                                   C.POINTER(Result)], i32),
+        "hdm_hausdorff_u8_noalloc": ([vp, vp, i32, i32, i32, C.c_uint8, C.POINTER(Opts),
+                                      C.POINTER(Result)], i32),
+        "hdm_noalloc_max_bytes": ([], u64),
         "hdm_batch_create": ([C.POINTER(Pair), i32, C.POINTER(Opts), i32, C.POINTER(vp)], i32),
         "hdm_batch_run": ([vp], i32), "hdm_batch_sync": ([vp, C.POINTER(Result)], i32),
         "hdm_batch_last_ms": ([vp], C.c_float), "hdm_batch_destroy": ([vp], None),

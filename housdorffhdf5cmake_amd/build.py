@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhdm.so")
-SOURCES = ["hdm_kernels.hip", "hdm_host.hip", "hdm_server.hip", "hdm_synth.hip", "hdm_h5.cpp"]
+SOURCES = ["hdm_kernels.hip", "hdm_host.hip", "hdm_server.hip", "hdm_noalloc.hip", "hdm_synth.hip",
+           "hdm_h5.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

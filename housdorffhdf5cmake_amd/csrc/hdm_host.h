@@ -110,6 +110,8 @@ bool fused_enabled();
 bool coop_enabled();
 bool ds_enabled();
 int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles);
+// the same from the kernel's resident workgroups per CU and the CU count
+int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles);
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,

@@ -199,11 +199,14 @@ bool ds_enabled() {
 // entries, and every workgroup costs the grid barriers an arrival to poll and
 // a release to read (a 64^3 pair measured 57 -> 54 us on 64 workgroups
 // instead of 512; pairs of 2048 tiles or more run fastest on all of them).
-int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles) {
-  const int occ = ds ? c->occ_fused_ds[tile] : c->occ_fused[tile];
-  const int maxres = std::min(std::max(1, occ) * c->cus, kMaxGrid);
+int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles) {
+  const int maxres = std::min(std::max(1, occ) * cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
   return (int)std::min<uint64_t>((uint64_t)maxres, std::max<uint64_t>(64, ntiles / 2));
+}
+int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles) {
+  return fused_grid_of(ds ? c->occ_fused_ds[tile] : c->occ_fused[tile], c->cus, requested,
+                       ntiles);
 }
 
 int barrier_mode() {

@@ -185,6 +185,24 @@ int hdm_first_hit_map_u8(const uint8_t *gold, const uint8_t *segm, int nx, int n
                          uint8_t label, const hdm_opts *opts, uint16_t *map,
                          hdm_result *out);
 
+/* ---- allocation-free single pair ----------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * or host memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same computation and results as
+ * hdm_hausdorff_u8, with its whole workspace in a fixed device array of
+ * hdm_noalloc_max_bytes() bytes (64 MiB: volumes up to 512x512x256 with
+ * 64x8x8 tiles) that each GPU holds from the first call until the process
+ * ends.  It calls no allocating routine.  Restrictions: device pointers only
+ * (ptr_kind HDM_PTR_DEVICE), timing 0; map_out, when set, is a device
+ * pointer; hist_out is host memory.  Returns HDM_E_CAPACITY when the pair's
+ * workspace exceeds the array.  Calls on one GPU share the array: within a
+ * process they are serialized (each returns after its stream is done); calls
+ * that overlapped on one GPU would race. */
+int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm, int nx, int ny, int nz,
+                             uint8_t label, const hdm_opts *opts, hdm_result *out);
+/* Size of the fixed device array of hdm_hausdorff_u8_noalloc, in bytes. */
+uint64_t hdm_noalloc_max_bytes(void);
+
 /* ---- batched pairs -----------------------------------------------------
  * No single counterpart in the reference (it has one pair per call,
  * hello.cu:2025): many pairs of uneven size go through one persistent

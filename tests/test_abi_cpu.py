@@ -120,6 +120,27 @@ def test_first_hit_map_refuses_dims_whose_distances_overflow_uint16():
     assert L.hdm_hausdorff_u8(p, p, 65535, 2, 3, 1, C.byref(o), C.byref(r)) == -1
 
 
+def test_noalloc_entry_rejects_sizes_beyond_its_array_on_the_host():
+    """hdm_hausdorff_u8_noalloc: the fixed array is 64 MiB; a 512^3 pair's
+    workspace (about 119 MiB) is refused with HDM_E_CAPACITY, host inputs and
+    event timing with HDM_E_INVALID, all before any device work."""
+    L = hdm.lib()
+    assert L.hdm_noalloc_max_bytes() == 64 << 20
+    assert L.hdm_workspace_bytes(512, 512, 256, 1) < 64 << 20  # the CT-sized pair fits
+    r = hdm.Result()
+    buf = np.zeros(16, np.uint8)
+    p = C.c_void_p(buf.ctypes.data)
+    assert L.hdm_hausdorff_u8_noalloc(p, p, 512, 512, 512, 1, None, C.byref(r)) == -9
+    o = hdm.Opts()
+    L.hdm_opts_init(C.byref(o))
+    o.ptr_kind = hdm.PTR_HOST
+    assert L.hdm_hausdorff_u8_noalloc(p, p, 8, 8, 8, 1, C.byref(o), C.byref(r)) == -1
+    L.hdm_opts_init(C.byref(o))
+    o.timing = 3
+    assert L.hdm_hausdorff_u8_noalloc(p, p, 8, 8, 8, 1, C.byref(o), C.byref(r)) == -1
+    assert L.hdm_hausdorff_u8_noalloc(None, p, 8, 8, 8, 1, None, C.byref(r)) == -1
+
+
 def test_cpp_compat_header_compiles_and_reports_errors(tmp_path):
     """include/hdm_compat.hpp: the reference's template signature over the C
     ABI.  Built with g++ and linked against libhdm.so; the null-pointer call
