@@ -302,7 +302,10 @@ def last_trace(max_iters: int = 4096):
         nxt = buf[8 * (i + 1)] if i + 1 < n else 0
         out.append({"t": i, "queued": int(r[5]), "all_done_us": us(1), "wg0_done_us": us(2),
                     "arrived_us": us(3), "folded_us": us(4),
-                    "next_start_us": round((nxt - r[0]) * tick.value / 1e3, 2) if nxt else None})
+                    "next_start_us": round((nxt - r[0]) * tick.value / 1e3, 2) if nxt else None,
+                    # region kernel: entries stepped, mean step phase of one
+                    "entries": int(r[6]),
+                    "entry_steps_us_mean": round(r[7] / r[6] * tick.value / 1e3, 2) if r[6] else None})
     return out
 
 

@@ -41,6 +41,11 @@ constexpr int xf_words(int txb, int ty, int tz) {
   return (txb == 64 && ty == 8 && tz == 8) ? 4 : 2;
 }
 constexpr int kReleaseCopies = 32; // replicas of the barrier release line
+// dilation steps per grid barrier of the single-pair region kernel, and the
+// words of a workgroup's barrier arrival record (hits per pass and step,
+// entries pushed and kept)
+constexpr int kTbSteps = 8;
+constexpr int kArrWords = 2 + 2 * kTbSteps;
 
 constexpr uint32_t kFull = 0x80000000u;
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
@@ -76,9 +81,13 @@ struct PairDesc {
   uint64_t *xf_orig[2];
   uint64_t *xf_buf[2][2];
   uint32_t *state[2];
+  // per region of 1 x 2 x 2 tiles (the eight-step launch): 0 idle (data =
+  // orig), kFull, else 1 + the period at which it was first queued
+  uint32_t *rstate[2];
   // per tile, low 16 bits gold / high 16 bits segm: bit 0 non-empty, 1 full,
   // 2..7 voxels on the -x, +x, -y, +y, -z, +z face, 8..13 voxels in the two
-  // outermost layers toward -x .. +z (written by the pack pass)
+  // outermost layers toward -x .. +z, 14 / 15 voxels in the eight outermost
+  // columns toward -x / +x (written by the pack pass; 8..15 on 64x8x8 tiles)
   uint32_t *tflags;
   uint32_t *hist[2];
   uint16_t *map;
@@ -115,6 +124,8 @@ struct PairOut {
   unsigned long long n_gold, n_segm, n_fp, n_fn;
   int32_t done;  // both passes finished at the engine's current iteration
   int32_t iters; // dilation iterations: the last one a pass needed, or all run
+  int32_t steps; // dilation steps per engine iteration of the body that ran
+  int32_t pad;
 };
 
 // The engine fields the host reads after a launch; the dilation kernel writes
@@ -129,7 +140,9 @@ __host__ __device__ inline int32_t box_dec_lo(int32_t e) { return 0x7FFFFFFF - e
 __host__ __device__ inline int32_t box_dec_hi(int32_t e) { return e - 1; }
 struct PackSums {
   unsigned long long sums[4][kSumSlots];
-  int32_t box[6][kSumSlots];
+  // rows 0..5 the box of the tiles with voxels of either mask (lo x, y, z,
+  // hi x, y, z), 6..11 of gold, 12..17 of segm
+  int32_t box[18][kSumSlots];
 };
 
 struct EngineHead {
@@ -167,11 +180,12 @@ struct Engine {
   // Single-pair barrier: every word carries its epoch in the top 24 bits
   // (tag()), so a reader that sees the right tag has the value that goes with
   // it and no ordering between words is needed.  arr[wg]: this iteration's
-  // hits of pass 0 and 1, the entries the workgroup pushed and kept, and (two
-  // dilation steps per barrier) the second step's hits of pass 0 and 1; rel2[copy]:
+  // hits of pass 0 and 1 in its first dilation step, the entries the
+  // workgroup pushed and kept, then the hits of pass 0 and 1 in each further
+  // step of the iteration (two or eight steps per barrier); rel2[copy]:
   // word 0 flags (bit m: pass m done, bit 2: error), words 1..kSubQ+1 prefix
   // offsets of the next queue.
-  unsigned long long arr[kMaxGrid][8];
+  unsigned long long arr[kMaxGrid][kArrWords];
   unsigned long long rel2[kReleaseCopies][kLineWords];
   // tag(epoch, first global entry) of each workgroup's free wave slots: the
   // master's prefix over workgroups of (slots - entries the workgroup kept)
@@ -180,6 +194,10 @@ struct Engine {
   // a launch packs into one and zeroes the other for the next launch, so no
   // barrier has to separate the zeroing from the pack
   PackSums acc[2];
+  // the two-step launch hands a packed pair to the region launch queued
+  // behind it: word 0 tag(epoch base, 2), words 1 and 2 its clock stamps of
+  // the start and the end of the pack
+  unsigned long long handoff[3];
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

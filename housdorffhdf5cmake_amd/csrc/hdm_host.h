@@ -30,6 +30,7 @@ struct Ctx {
   int occ_pack[kLaunchIds] = {};  // resident pack workgroups per CU
   int occ_fused[kLaunchIds] = {};  // resident fused single-pair workgroups per CU
   int occ_fused_ds[kLaunchIds] = {};  // the same, two dilation steps per barrier
+  int occ_fused_tb[kLaunchIds] = {};  // the same, regions of eight steps per barrier
   double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
   void *host_out = nullptr;  // pinned, 4 KiB
@@ -109,9 +110,17 @@ bool fused_enabled();
 // the fused single-pair launch is cooperative (HDM_COOP=1) or plain
 bool coop_enabled();
 bool ds_enabled();
+// the two-step fused single-pair launch (64x8x8 tiles, 6 element) is
+// followed by the region launch, to which it hands long pairs, unless HDM_TB=0
+bool tb_enabled();
+// DilateArgs::tb_mode of the two-step launch: 0, 1 (by the HD bound) or 2
+// (every pair), from HDM_TB
+int tb_mode();
 int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles);
 // the same from the kernel's resident workgroups per CU and the CU count
 int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles);
+// the grid of the region launch (regions of 1 x 2 x 2 tiles)
+int tb_grid(int occ, int cus, int requested, uint64_t ntiles, uint32_t nregions);
 GroupPtrs carve_common(char *base, const CommonLayout &L);
 hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P, int npairs,
                          const PairDesc *single, const PackIndex *single_idx,

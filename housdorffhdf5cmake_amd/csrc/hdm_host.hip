@@ -115,6 +115,12 @@ int get_ctx(Ctx **out) {
          nb < 1))
       rc = HDM_E_HIP;
     c->occ_fused_ds[k] = ds_fn ? nb : 0;
+    const void *tb_fn = hdm_tile_launch(k)->fused_tb_fn;
+    if (rc == HDM_OK && tb_fn &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tb_fn, kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ_fused_tb[k] = tb_fn ? nb : 0;
   }
   // the dilation kernel stores the result and the engine head here directly:
   // coherent (fine-grained) host memory, so the stores are seen by the host
@@ -194,6 +200,21 @@ bool ds_enabled() {
   return on;
 }
 
+// HDM_TB: 0 runs every single pair in the two-step fused launch; 1 (the
+// default) queues the region launch behind it, and the two-step launch hands
+// over the pairs whose HD lower bound reaches 16 steps; 2 hands over every pair.
+static int tb_setting() {
+  static const int v = [] {
+    const char *e = std::getenv("HDM_TB");
+    if (!e || !*e) return 1;
+    const int x = std::atoi(e);
+    return x == 0 ? 0 : x == 2 ? 2 : 1;
+  }();
+  return v;
+}
+bool tb_enabled() { return tb_setting() != 0; }
+int tb_mode() { return tb_setting(); }
+
 // Default grid of the fused launch: one workgroup per two tiles, at least
 // 64, at most every resident one.  A small volume's iterations queue few
 // entries, and every workgroup costs the grid barriers an arrival to poll and
@@ -203,6 +224,16 @@ int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles) {
   const int maxres = std::min(std::max(1, occ) * cus, kMaxGrid);
   if (requested > 0) return std::min(requested, maxres);
   return (int)std::min<uint64_t>((uint64_t)maxres, std::max<uint64_t>(64, ntiles / 2));
+}
+// Grid of the region launch: the fused launch's rule over tiles (the launch
+// may run the two-step body), at most four workgroups per region and at most
+// 2 x kWgThreads in all (its seed period publishes one count per workgroup
+// into a table of that size and gives each a segment of the queue area sized
+// by regions per workgroup)
+int tb_grid(int occ, int cus, int requested, uint64_t ntiles, uint32_t nregions) {
+  int g = fused_grid_of(occ, cus, requested, ntiles);
+  g = (int)std::min<uint64_t>((uint64_t)g, 4ull * std::max<uint32_t>(1u, nregions));
+  return std::min(g, 2 * kWgThreads);
 }
 int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles) {
   return fused_grid_of(ds ? c->occ_fused_ds[tile] : c->occ_fused[tile], c->cus, requested,
@@ -514,6 +545,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const bool fused = fused_enabled();
   const uint32_t ebase = c->ebase;
   bool ds = false;  // two dilation steps per barrier
+  bool tb = false;  // then the region launch (eight steps per barrier)
   if (fused) {
     // one launch: reset, pack, seed and dilation.  Its barrier tags use
     // epochs above every earlier launch's on this workspace; a new workspace
@@ -547,13 +579,22 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     a.ready_seq = cfg.ready_seq;
     a.acc_set = c->acc_set;
     a.coop = coop_enabled() ? 1 : 0;
+    a.tb_active = c->cus;
     ds = tl->fused_ds && conn == 6 && ds_enabled();
+    tb = ds && tl->fused_tb && tb_enabled();
+    a.tb_mode = tb ? tb_mode() : 0;
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {
-      e = (ds ? tl->fused_ds : tl->fused)(
-          dim3(fused_grid(c, tile, opts.grid_ctas, ds, g.ntiles)), s, a);
+      e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds, g.ntiles)),
+                                          s, a);
       if (e == hipSuccess) c->acc_set ^= 1;  // the launch zeroes the other set first
     }
+    // the region launch, which runs only when the two-step launch hands the
+    // pair over (and otherwise leaves at once)
+    if (e == hipSuccess && tb)
+      e = tl->fused_tb(
+          dim3(tb_grid(c->occ_fused_tb[tile], c->cus, opts.grid_ctas, g.ntiles, g.nregions)), s,
+          a);
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
     // until the result is read, assume the launch used every epoch it could
     c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;
@@ -584,7 +625,8 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   std::memcpy(&o, hout, sizeof(o));
   std::memcpy(&es, hhead, sizeof(es));
   if (o.status == 1) return HDM_E_HIP;
-  c->last_ms[6] = ds ? 2.f : 1.f;
+  // the body that ran (the region launch picks it per pair)
+  c->last_ms[6] = (float)std::max(1, o.steps);
   if (fused) {
     c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used
     c->last_ms[5] = 1.f;

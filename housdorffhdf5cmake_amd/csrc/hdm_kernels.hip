@@ -336,35 +336,47 @@ __device__ __forceinline__ Row load_row_bits(const PairDesc &d, const void *vol,
 // Per-wave accumulator of the pack statistics of one pair.
 struct PackAcc {
   uint32_t c[4];  // |gold|, |segm|, |FP|, |FN| bits seen by this lane
-  int mn[3], mx[3];
+  // tile box of the tiles with voxels of either mask (0..2), of gold (3..5)
+  // and of segm (6..8); the per-mask boxes only where the partials have room
+  // for them (PackSums, the fused single-pair launch)
+  int mn[9], mx[9];
 };
 __device__ __forceinline__ void acc_clear(PackAcc &a) {
   a.c[0] = a.c[1] = a.c[2] = a.c[3] = 0;
-  a.mn[0] = a.mn[1] = a.mn[2] = 0x7FFFFFFF;
-  a.mx[0] = a.mx[1] = a.mx[2] = -1;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) a.mn[k] = 0x7FFFFFFF, a.mx[k] = -1;
+}
+template <typename S>
+constexpr int box_rows() {
+  return (int)(sizeof(S::box) / sizeof(S::box[0]));
 }
 // into the pack partials (sums, box) of a PairCtr or a PackSums
 template <typename S>
 __device__ __forceinline__ void acc_publish(S &c, const PackAcc &a, int slot) {
+  constexpr int kBoxes = box_rows<S>() / 6;  // 1 (either mask) or 3 (also per mask)
 #pragma unroll
   for (int f = 0; f < 4; ++f)
     if (a.c[f]) atomicAdd(&c.sums[f][slot], (unsigned long long)a.c[f]);
-  if (a.mx[0] >= 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      atomicMax(&c.box[k][slot], box_enc_lo(a.mn[k]));
-      atomicMax(&c.box[3 + k][slot], box_enc_hi(a.mx[k]));
+  for (int bx = 0; bx < kBoxes; ++bx) {
+    if (a.mx[3 * bx] >= 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        atomicMax(&c.box[6 * bx + k][slot], box_enc_lo(a.mn[3 * bx + k]));
+        atomicMax(&c.box[6 * bx + 3 + k][slot], box_enc_hi(a.mx[3 * bx + k]));
+      }
     }
   }
 }
 // wave-reduce the accumulator, lane 0 publishes
 template <typename S>
 __device__ __forceinline__ void acc_flush(S &c, PackAcc &a, int lane, int slot) {
+  constexpr int kVals = box_rows<S>() / 2;  // 3 or 9 box coordinates
   PackAcc r;
 #pragma unroll
   for (int f = 0; f < 4; ++f) r.c[f] = wave_sum(a.c[f], lane);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) r.mn[k] = wave_min(a.mn[k], lane), r.mx[k] = wave_max(a.mx[k], lane);
+  for (int k = 0; k < kVals; ++k) r.mn[k] = wave_min(a.mn[k], lane), r.mx[k] = wave_max(a.mx[k], lane);
   if (lane == 0) acc_publish(c, r, slot);
   acc_clear(a);
 }
@@ -556,6 +568,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         // 64x8x8): the second column from each x face as face words, and flags
         constexpr bool kDeep = TXB == 64 && TY == 8 && TZ == 8;
         bool gxl = false, gxh = false, gyl = false, gyh = false, gzl = false, gzh = false;
+        bool sxl = false, sxh = false;  // voxels in the eight columns toward -x / +x
         uint64_t fx2l = 0, fx2h = 0;
         if constexpr (kDeep) {
           fx2l = group_bits<R>(__ballot(rowok && ((w >> 1) & 1)), lane);
@@ -566,6 +579,8 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
           gyh = group_bits<R>(__ballot(yl >= TY - 2 && w != 0), lane) != 0;
           gzl = group_bits<R>(__ballot(zl <= 1 && w != 0), lane) != 0;
           gzh = group_bits<R>(__ballot(zl >= TZ - 2 && w != 0), lane) != 0;
+          sxl = group_bits<R>(__ballot(rowok && (w & 0xFFu) != 0), lane) != 0;
+          sxh = group_bits<R>(__ballot(rowok && (w >> (TXB - 8)) != 0), lane) != 0;
         }
         if (in && r == 0) {
           put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile, fxl);
@@ -579,7 +594,8 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                   (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
                   (uint32_t)fzl << 6 | (uint32_t)fzh << 7 | (uint32_t)gxl << 8 |
                   (uint32_t)gxh << 9 | (uint32_t)gyl << 10 | (uint32_t)gyh << 11 |
-                  (uint32_t)gzl << 12 | (uint32_t)gzh << 13)
+                  (uint32_t)gzl << 12 | (uint32_t)gzh << 13 | (uint32_t)sxl << 14 |
+                  (uint32_t)sxh << 15)
                  << (16 * m);
       }
       if (in && r == 0) put<COH>(d.tflags + tile, flags);
@@ -597,6 +613,14 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
           acc.mn[0] = min(acc.mn[0], txU[u]), acc.mx[0] = max(acc.mx[0], txU[u]);
           acc.mn[1] = min(acc.mn[1], tyU[u]), acc.mx[1] = max(acc.mx[1], tyU[u]);
           acc.mn[2] = min(acc.mn[2], tzU[u]), acc.mx[2] = max(acc.mx[2], tzU[u]);
+        }
+        if constexpr (COH) {  // per-mask boxes (the fused launch's PackSums)
+          const int tc[3] = {txU[u], tyU[u], tzU[u]};
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            if (a != 0) acc.mn[3 + k] = min(acc.mn[3 + k], tc[k]), acc.mx[3 + k] = max(acc.mx[3 + k], tc[k]);
+            if (b != 0) acc.mn[6 + k] = min(acc.mn[6 + k], tc[k]), acc.mx[6 + k] = max(acc.mx[6 + k], tc[k]);
+          }
         }
       } else if (in) {  // a lane past a pair boundary publishes its own row
         PairCtr &cp = ctrs[p];
@@ -691,6 +715,7 @@ struct PairView {
   const uint64_t *xf_orig[2];
   uint64_t *xf_buf[2][2];
   uint32_t *state[2];
+  uint32_t *rstate[2];
   uint32_t *hist0, *hist1;
   uint16_t *map;
   int nx, ny, nz, gx, gy, hist_len;
@@ -709,6 +734,7 @@ __device__ __forceinline__ PairView make_view(const PairDesc &d, const int *lo, 
     v.xf_orig[m] = d.xf_orig[m];
     v.xf_buf[m][0] = d.xf_buf[m][0], v.xf_buf[m][1] = d.xf_buf[m][1];
     v.state[m] = d.state[m];
+    v.rstate[m] = d.rstate[m];
   }
   v.hist0 = d.hist[0], v.hist1 = d.hist[1];
   v.map = d.map;
@@ -1128,16 +1154,27 @@ struct PassState {
 // counters to zero.  Waits are bounded as in grid_barrier.  s_cnt[0..3] holds the four counts
 // and is zeroed here, as is *s_zero (the kept count of the queue that becomes
 // the next iteration's push target).  Returns false on a timeout.
-// DS: each engine iteration is two dilation steps; the arrival carries the
-// second step's hits too (s_cnt[4], s_cnt[5]) and the fold records them as
-// the next histogram bin, stopping a pass at the bin that reaches its cut.
-template <bool DS = false>
+// KS: dilation steps per engine iteration (1, 2 or 8).  With KS > 1 the
+// arrival carries the hits of every step (s_cnt[m] for the first, s_cnt[2 +
+// 2k + m] for step k >= 1), and the fold records each step as its own
+// histogram bin, stopping a pass at the bin that reaches its cut.
+template <int KS = 1>
 __device__ __forceinline__ bool grid_barrier_single(
     Engine *es, uint32_t epoch, uint64_t limit, int *s_ok, uint32_t *s_cnt, uint32_t *s_zero,
     PassState &ps, const PairView &d, uint32_t tdone, uint32_t *qstats, int qstats_len,
     uint64_t *tr, uint32_t *s_pre, uint32_t *s_flags, unsigned long long *s_red, uint32_t slots,
-    uint8_t *s_kept, uint32_t *s_fs, int qz) {
-  constexpr int kArr = DS ? 6 : 4;  // arrival words
+    uint8_t *s_kept, uint32_t *s_fs, int qz, uint32_t narr = 0, uint32_t nnext = 0) {
+  static_assert(KS == 1 || KS == 2 || KS == kTbSteps, "steps per barrier");
+  // narr: the workgroups that arrive (0: the grid); nnext: those that take
+  // part in the next iteration, among which the free slots are numbered (0:
+  // narr).  Workgroups 0 .. narr-1 arrive; workgroup 0 is always one of them.
+  if (narr == 0) narr = gridDim.x;
+  if (nnext == 0) nnext = narr;
+  constexpr int kArr = KS == 1 ? 4 : 2 + 2 * KS;  // arrival words
+  constexpr int kH = 2 * KS;                      // hit words, index m + 2k
+  static_assert(kArr <= kArrWords, "arrival record");
+  // arrival word of hit index i (pass i % 2, step i / 2)
+  auto hw = [](int i) { return i < 2 ? i : i + 2; };
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1149,20 +1186,22 @@ __device__ __forceinline__ bool grid_barrier_single(
     *s_zero = 0;
   }
   constexpr int kWords = kSubQ + 3;  // flags, kSubQ + 1 prefix offsets, free slots
-  __shared__ unsigned long long s_red2[2 * kWavesPerWg];  // second-step hits per wave
+  __shared__ unsigned long long s_hw[kH][kWavesPerWg];  // hits per (pass, step) per wave
   if (blockIdx.x == 0) {
-    unsigned long long h0 = 0, h1 = 0, np = 0, nl = 0, g0 = 0, g1 = 0;
+    unsigned long long h[kH], np = 0, nl = 0;
+#pragma unroll
+    for (int i = 0; i < kH; ++i) h[i] = 0;
     int ok = 1;
     const uint64_t t0 = wall_clock64();
     // up to kPoll records per thread are polled together: all their loads are
     // in flight at once, so a grid of kPoll * 256 workgroups costs one round
     // trip per poll, not one per workgroup
-    constexpr int kPoll = 4;
-    for (uint32_t i0 = threadIdx.x; i0 < gridDim.x && ok; i0 += kPoll * blockDim.x) {
+    constexpr int kPoll = KS > 2 ? 2 : 4;
+    for (uint32_t i0 = threadIdx.x; i0 < narr && ok; i0 += kPoll * blockDim.x) {
       uint32_t pending = 0;
 #pragma unroll
       for (int k = 0; k < kPoll; ++k)
-        if (i0 + k * blockDim.x < gridDim.x) pending |= 1u << k;
+        if (i0 + k * blockDim.x < narr) pending |= 1u << k;
       while (pending) {
         unsigned long long w[kPoll][kArr];
 #pragma unroll
@@ -1178,9 +1217,9 @@ __device__ __forceinline__ bool grid_barrier_single(
 #pragma unroll
           for (int j = 0; j < kArr; ++j) all = all && tagged(w[k][j], epoch);
           if (all) {
-            h0 += w[k][0] & kTagValue, h1 += w[k][1] & kTagValue;
+#pragma unroll
+            for (int i = 0; i < kH; ++i) h[i] += w[k][hw(i)] & kTagValue;
             np += w[k][2] & kTagValue, nl += w[k][3] & kTagValue;
-            if constexpr (DS) g0 += w[k][4] & kTagValue, g1 += w[k][5] & kTagValue;
             s_kept[i0 + k * blockDim.x] = (uint8_t)(w[k][3] & kTagValue);
             pending &= ~(1u << k);
           }
@@ -1193,27 +1232,26 @@ __device__ __forceinline__ bool grid_barrier_single(
         }
       }
     }
-    // sums: hits per pass and kept entries over all workgroups, global pushes
-    // per sub-queue
+    // sums: hits per pass and step and kept entries over all workgroups,
+    // global pushes per sub-queue
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (threadIdx.x < kSubQ) s_red[12 + threadIdx.x] = 0;
-    h0 = wave_sum64(h0);
-    h1 = wave_sum64(h1);
+#pragma unroll
+    for (int i = 0; i < kH; ++i) h[i] = wave_sum64(h[i]);
     nl = wave_sum64(nl);
-    if constexpr (DS) {
-      g0 = wave_sum64(g0);
-      g1 = wave_sum64(g1);
-    }
     __syncthreads();
-    if (lane == 0) s_red[wave] = h0, s_red[4 + wave] = h1, s_red[8 + wave] = nl;
-    if (DS && lane == 0) s_red2[wave] = g0, s_red2[kWavesPerWg + wave] = g1;
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < kH; ++i) s_hw[i][wave] = h[i];
+      s_red[8 + wave] = nl;
+    }
     if (!ok) *s_ok = 0;
     atomicAdd(&s_red[12 + threadIdx.x % kSubQ], np);
     __syncthreads();
     if (tr && threadIdx.x == 0) tr[3] = wall_clock64();
     const bool good = *s_ok != 0;
     {  // free slots: thread i owns workgroups [i*c, i*c + c); wave scan, then waves
-      const uint32_t nwg = gridDim.x, c = (nwg + blockDim.x - 1) / blockDim.x;
+      const uint32_t nwg = nnext, c = (nwg + blockDim.x - 1) / blockDim.x;
       const uint32_t wb = threadIdx.x * c, we = min(nwg, wb + c);
       uint32_t part = 0;
       for (uint32_t w = wb; w < we; ++w) part += slots - min(slots, (uint32_t)s_kept[w]);
@@ -1242,26 +1280,18 @@ __device__ __forceinline__ bool grid_barrier_single(
       }
       s_pre[kSubQ] = acc;
       if (good) {
-        const unsigned long long hits[2] = {s_red[0] + s_red[1] + s_red[2] + s_red[3],
-                                            s_red[4] + s_red[5] + s_red[6] + s_red[7]};
         const uint32_t kept = (uint32_t)(s_red[8] + s_red[9] + s_red[10] + s_red[11]);
         const int tp = (int)(tdone - d.t_start);
-        // DS: this engine iteration covered dilation steps 2tp+1 and 2tp+2
-        constexpr int kSteps = DS ? 2 : 1;
-        unsigned long long bins[2][2] = {{hits[0], 0ull}, {hits[1], 0ull}};
-        if constexpr (DS) {
-          bins[0][1] = s_red2[0] + s_red2[1] + s_red2[2] + s_red2[3];
-          bins[1][1] = s_red2[4] + s_red2[5] + s_red2[6] + s_red2[7];
-        }
+        // this engine iteration covered dilation steps KS*tp+1 .. KS*tp+KS
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
-#pragma unroll
-          for (int k = 0; k < kSteps; ++k) {
-            const int it = kSteps * tp + 1 + k;  // the dilation iteration of this bin
-            const unsigned long long hm = bins[m][k];
+          for (int k = 0; k < KS; ++k) {
+            const int it = KS * tp + 1 + k;  // the dilation step of this bin
+            const int i = m + 2 * k;
+            const unsigned long long hm = s_hw[i][0] + s_hw[i][1] + s_hw[i][2] + s_hw[i][3];
             if (ps.cut[m] == 0 || ps.done_from[m] < it || hm == 0) continue;
-            uint32_t *h = m ? d.hist1 : d.hist0;
-            if (it < d.hist_len) h[it] = (uint32_t)hm;
+            uint32_t *hp = m ? d.hist1 : d.hist0;
+            if (it < d.hist_len) hp[it] = (uint32_t)hm;
             ps.cov[m] += hm;
             ps.sum[m] += hm * (unsigned long long)it;
             ps.hd[m] = (uint32_t)it;
@@ -1270,7 +1300,7 @@ __device__ __forceinline__ bool grid_barrier_single(
         }
 #pragma unroll
         for (int m = 0; m < 2; ++m)
-          if (ps.cut[m] == 0 || ps.done_from[m] <= kSteps * tp + kSteps) flags |= 1u << m;
+          if (ps.cut[m] == 0 || ps.done_from[m] <= KS * tp + KS) flags |= 1u << m;
         ps.qtotal += ps.cur_total;
         ps.qmax = max(ps.qmax, ps.cur_total);
         if (qstats && tdone < (uint32_t)qstats_len) qstats[tdone] = ps.cur_total;
@@ -1294,7 +1324,10 @@ __device__ __forceinline__ bool grid_barrier_single(
     }
   } else {
     __shared__ unsigned long long s_rel[kWords + 1];
-    if (threadIdx.x <= kWords) {  // the replica's words, then this workgroup's first slot
+    // a workgroup that takes no part in the next iteration has no first slot
+    if (threadIdx.x == kWords && blockIdx.x >= nnext) s_rel[kWords] = 0;
+    if (threadIdx.x < kWords || (threadIdx.x == kWords && blockIdx.x < nnext)) {
+      // the replica's words, then this workgroup's first slot
       const unsigned long long *src = threadIdx.x < kWords
                                           ? &es->rel2[blockIdx.x % kReleaseCopies][threadIdx.x]
                                           : &es->fslot[blockIdx.x];
@@ -1452,6 +1485,8 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
     reached = reached && cnt[m] >= cut;
   }
   o.iters = done ? last : tl;
+  o.steps = steps;
+  o.pad = 0;
   o.hd_fp = hd[0], o.hd_fn = hd[1];
   o.hd = max(hd[0], hd[1]);
   o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
@@ -1467,6 +1502,30 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
   o.status = st;
   return o;
 }
+
+// A lower bound of the HD in dilation steps (6 element, 64x8x8 tiles) from
+// the tile boxes of gold (g: lo x, y, z, hi x, y, z) and segm (s): a segm
+// voxel outside gold's box is an FP voxel, at least as far from gold as from
+// gold's box, and a gold voxel outside segm's box is an FN voxel.  A box that
+// reaches d tiles beyond the other's along an axis holds a voxel at least
+// (d - 1) * tile size + 1 voxels beyond it.  0 when a mask is empty.
+__device__ __forceinline__ int hd_lower_bound(const int *g, const int *s) {
+  if (g[3] < 0 || s[3] < 0) return 0;
+  constexpr int kT[3] = {64, 8, 8};
+  int lb = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int d[4] = {s[3 + k] - g[3 + k], g[k] - s[k], g[3 + k] - s[3 + k], s[k] - g[k]};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (d[j] >= 1) lb = max(lb, (d[j] - 1) * kT[k] + 1);
+  }
+  return lb;
+}
+// Pairs whose HD bound reaches this many steps go to the region launch
+// (eight steps per barrier); the others stay with the two-step launch, which
+// keeps more waves busy in the dense first iterations of a short run.
+constexpr int kTbMinSteps = 16;
 
 // ---------------------------------------------------------------- dilate
 // SINGLE: the launch holds one pair, so its descriptor and its per-iteration
@@ -1570,7 +1629,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       }
       PackSums *next = &es->acc[(a.acc_set & 1) ^ 1];
       for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&next->sums[0][0] + i, 0ull);
-      for (int i = tid; i < 6 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
+      for (int i = tid; i < 18 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
       for (int i = tid; i < d.hist_len; i += blockDim.x)
         st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
       for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
@@ -1587,6 +1646,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     if (run) {
       __shared__ unsigned long long s_tot[4];
       __shared__ int s_box[6];
+      __shared__ int s_hand;  // 1: the pair goes to the region launch
       // the flags of this thread's first tile and of its neighbours in the
       // tile grid load together with the pack partials, one round trip for
       // both; the box, known once the partials are summed, selects among them
@@ -1624,11 +1684,29 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           const int x = wave_max(ld_sc(&acc->box[k][lane]), lane);
           bx[k] = k < 3 ? box_dec_lo(x) : box_dec_hi(x);
         }
+        // hand the pair to the region launch (two-step launch only): always
+        // (tb_mode 2), or when the HD bound from the gold and segm boxes
+        // (rows 6..17) reaches kTbMinSteps (tb_mode 1)
+        bool hand = false;
+        if constexpr (DS) {
+          if (a.tb_mode == 2) {
+            hand = true;
+          } else if (a.tb_mode == 1) {
+            int gs[12];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+              const int x = wave_max(ld_sc(&acc->box[6 + k][lane]), lane);
+              gs[k] = k % 6 < 3 ? box_dec_lo(x) : box_dec_hi(x);
+            }
+            hand = hd_lower_bound(gs, gs + 6) >= kTbMinSteps;
+          }
+        }
         if (lane == 0) {
 #pragma unroll
           for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
 #pragma unroll
           for (int k = 0; k < 6; ++k) s_box[k] = bx[k];
+          s_hand = hand ? 1 : 0;
           if (blockIdx.x == 0) {
             st_sc(&c0.n_gold, v[0]), st_sc(&c0.n_segm, v[1]);
             st_sc(&c0.n_fp, v[2]), st_sc(&c0.n_fn, v[3]);
@@ -1638,6 +1716,23 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         }
       }
       __syncthreads();
+      if (DS && s_hand) {
+        // the region launch takes over from the packed pair: its region
+        // states start idle, and workgroup 0 leaves the tagged hand-off word
+        // (with the clock stamps of the start and the pack) instead of the
+        // results; the region launch starts once every workgroup here is done
+        const uint32_t nreg = (uint32_t)d.gx * (uint32_t)((d.gy + 1) / 2) *
+                              (uint32_t)((d.gz + 1) / 2);
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nreg;
+             i += gridDim.x * blockDim.x)
+          st_sc(d.rstate[0] + i, 0u), st_sc(d.rstate[1] + i, 0u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+          st_sc(&es->handoff[1], s_phase[0]);
+          st_sc(&es->handoff[2], s_phase[1]);
+          st_sc(&es->handoff[0], tag(a.ebase, 2ull));
+        }
+        return;
+      }
       v0 = make_view(d, s_box, s_box + 3, s_tot[2], s_tot[3]);
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int m = 0; m < 2; ++m) {
@@ -1690,7 +1785,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       if (threadIdx.x == 0) s_cnt[2] += nflushed, s_cnt[3] = s_stn[0];
       if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[2] = wall_clock64();
       // hands iteration 0 its flags, global prefix and free slots
-      run = grid_barrier_single<DS>(es, a.ebase + 3, spin_limit, &s_ok, s_cnt, &s_stn[1], ps, v0,
+      run = grid_barrier_single<DS ? 2 : 1>(es, a.ebase + 3, spin_limit, &s_ok, s_cnt, &s_stn[1], ps, v0,
                                 ~0u, qstats, qstats_len, nullptr, s_pre, s_flags, s_red, kSlots,
                                 s_kept, s_fs, 2);
     }
@@ -2465,7 +2560,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       }
     };
     if (SINGLE) {
-      if (!grid_barrier_single<DS>(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
+      if (!grid_barrier_single<DS ? 2 : 1>(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
                                v0, tdone, qstats, qstats_len, tr, s_pre, s_flags, s_red, kSlots,
                                s_kept, s_fs, (t + 2) % 4))
         break;
@@ -2546,6 +2641,759 @@ __global__ __launch_bounds__(kWgThreads) void dilate_ds_kernel(DilateArgs a) {
   dilate_body<64, 8, 8, true, true, 6, true>(a);
 }
 
+// ---------------------------------------------------------------- regions
+// The single-pair launch with eight dilation steps per grid barrier
+// (dilate_tb_kernel; 64x8x8 tiles, 6 element).  A region is 1 x 2 x 2 tiles
+// (64 x 16 x 16 voxels), region (rx, ry, rz) = tiles (rx, 2ry + {0,1},
+// 2rz + {0,1}); the queue holds (region, mask) entries and one workgroup
+// steps one entry at a time.  It holds the region's rows plus an eight-voxel
+// halo -- a window of 32 x 32 rows in (y, z), each row 80 bits in x (eight
+// columns of the x neighbours on each side) -- in registers, four rows of
+// consecutive z per thread, and runs the eight 6-neighbour dilation steps
+// there: x within the row, y from the lanes beside (DPP), z from the
+// thread's own rows or, for its first and last row, from the neighbouring
+// threads' rows through LDS, one workgroup barrier per step
+// (hello.cu:373-376, 1310-1542 per step).  After step k the window is exact
+// wherever the rows and columns it did not hold are more than k voxels away,
+// so after eight steps the region itself is exact: the overlapped-tiling form
+// of the reference's shared-memory block plus halo (hello.cu:1086, 1310-1542),
+// taken eight steps deep.  New hits of the other mask in the region count per
+// step (bins 8tp+1 .. 8tp+8, hello.cu:1590-1626); the barrier master folds
+// them bin by bin.  Region states (rstate): 0 idle (data = orig), kFull, or
+// 1 + the period at which the region was first queued (its data then sits in
+// buf[(tp + 1) & 1] at period tp).  A region stays queued while its input is
+// not full, and claims (compare-and-swap 0 -> tp + 2) every idle neighbour
+// region whose core its output reaches within eight steps (L1 distance).
+constexpr int kTbWin = 32;  // window rows per axis: 16 + 2 * 8
+
+// A window row: o = the region's 64 voxels of the row; h = its x halo, bits
+// 0..7 the voxels x = -8 .. -1 (the left tile's last eight columns), bits
+// 16..23 the voxels x = 64 .. 71 (the right tile's first eight); bits 8..15
+// stay empty, so a shift of h by one never carries one side into the other.
+struct TbRow {
+  uint64_t o;
+  uint32_t h;
+};
+constexpr uint32_t kTbHalo = 0x00FF00FFu;
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));  // one LDS row
+__device__ __forceinline__ TbRow tb_make(uint64_t left, uint64_t own, uint64_t right) {
+  return TbRow{own, (uint32_t)(left >> 56) | ((uint32_t)(right & 0xFFull) << 16)};
+}
+// one 6-neighbour step of a row in x (bitDilatate, hello.cu:373-376), across
+// the halo on both sides
+__device__ __forceinline__ TbRow tb_xdil(const TbRow &r) {
+  const uint64_t o = r.o | (r.o << 1) | (r.o >> 1) | (uint64_t)((r.h >> 7) & 1u) |
+                     ((uint64_t)((r.h >> 16) & 1u) << 63);
+  const uint32_t h = ((r.h | (r.h << 1) | (r.h >> 1)) & kTbHalo) | ((uint32_t)(r.o & 1u) << 7) |
+                     ((uint32_t)(r.o >> 63) << 16);
+  return TbRow{o, h};
+}
+// the row of lane - 1 (FROM_LOWER) or lane + 1 in the wave, 0 at the wave's ends
+// (DPP wave_shr:1 / wave_shl:1)
+template <bool FROM_LOWER>
+__device__ __forceinline__ TbRow tb_lane_nb(const TbRow &r) {
+  constexpr int kCtrl = FROM_LOWER ? 0x138 : 0x130;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)r.o, kCtrl, 0xF, 0xF, true);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(r.o >> 32), kCtrl, 0xF, 0xF, true);
+  const uint32_t h = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)r.h, kCtrl, 0xF, 0xF, true);
+  return TbRow{(uint64_t)lo | ((uint64_t)hi << 32), h};
+}
+
+__device__ __forceinline__ bool tb_region_in_box(const Box &b, int rx, int ry, int rz) {
+  return rx >= b.lo[0] && rx <= b.hi[0] && 2 * ry + 1 >= b.lo[1] && 2 * ry <= b.hi[1] &&
+         2 * rz + 1 >= b.lo[2] && 2 * rz <= b.hi[2];
+}
+__device__ __forceinline__ bool tb_tile_in_box(const Box &b, int tx, int ty, int tz) {
+  return tx >= b.lo[0] && tx <= b.hi[0] && ty >= b.lo[1] && ty <= b.hi[1] && tz >= b.lo[2] &&
+         tz <= b.hi[2];
+}
+
+// The single-pair dilation of a pair the two-step launch handed over after
+// its pack: region seed merged with the first eight steps, then periods of
+// eight steps between grid barriers, then the results.  Held to two
+// resident workgroups per CU (240 VGPRs, no spill).
+__global__ __launch_bounds__(kWgThreads)
+__attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
+  constexpr int KS = kTbSteps;
+  constexpr int R = 64;  // rows per tile
+  PairCtr *ctrs = a.ctrs;
+  Engine *es = a.es;
+  uint64_t *q = a.q;
+  const uint32_t qcap = a.qcap;
+  const uint64_t spin_limit = a.spin;
+  uint32_t *qstats = a.qstats;
+  const int qstats_len = a.qstats_len;
+  uint64_t *trace = a.trace;
+  const PairDesc &d = a.desc;
+
+  // the first and last of each thread's four rows, per step parity (the
+  // rows other threads read in z), y fastest so that the lanes of a wave
+  // touch consecutive 16-byte words (no bank conflicts)
+  __shared__ u64x2 s_win[2][8][2][kTbWin];
+  __shared__ LocalQueue lq;
+  __shared__ uint32_t s_pre[kSubQ + 1];
+  __shared__ int s_ok;
+  __shared__ uint32_t s_cnt[kArrWords];
+  __shared__ uint32_t s_flags[1];
+  __shared__ unsigned long long s_red[12 + kSubQ];
+  __shared__ uint64_t s_st[2][1];  // the entry a workgroup keeps for the next period
+  __shared__ uint32_t s_stn[2];
+  __shared__ uint8_t s_kept[kMaxGrid];
+  __shared__ uint32_t s_fs[2];
+  __shared__ uint32_t s_src[27];    // per neighbour region: 0 none, 1 orig, 2 buf, 3 full
+  __shared__ uint32_t s_or[3];      // per entry: input not full, reach bits, other left
+  __shared__ uint64_t s_ent;        // entry taken from the global queue
+  __shared__ uint32_t s_seed[2 * kWgThreads];
+  __shared__ uint32_t s_nseed;
+  __shared__ unsigned long long s_phase[3];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const uint32_t kq = blockIdx.x % kSubQ;
+  // the step mapping: thread t holds rows (yy, zz0 .. zz0 + 3) of the window
+  const int yy = threadIdx.x & 31;
+  const int seg = threadIdx.x >> 5;
+  const int zz0 = 4 * seg;
+  const int tyi = yy / 8 - 1, tzi = seg / 2 - 1;  // tile offsets in the region, -1 .. 2
+  const bool core = yy >= 8 && yy < 24 && seg >= 2 && seg < 6;
+  const int cy = yy - 8;  // core coordinates of the thread's rows (core threads)
+
+  if (threadIdx.x == 0) lq.n = 0;
+  if (threadIdx.x < kArrWords) s_cnt[threadIdx.x] = 0;
+  if (threadIdx.x < 2) s_stn[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {  // no kept entries yet: every slot is free
+    s_fs[0] = blockIdx.x;
+    s_fs[1] = gridDim.x;
+  }
+  PassState ps{};
+  uint32_t t = 0;
+  bool run = true;
+  const uint32_t eoff = a.ebase + 3;  // the barrier closing period tp has epoch eoff + tp + 1
+  PackSums *acc = &es->acc[a.acc_set & 1];
+  PairCtr &c0 = ctrs[0];
+  // The two-step launch before this one on the stream reset and packed the
+  // pair and zeroed its region states; it hands the pair over with a word
+  // tagged with this launch's epoch base.  Without it (the two-step launch
+  // computed the pair, or failed) every workgroup leaves at once.
+  {
+    const unsigned long long h = ld_sc(&es->handoff[0]);
+    if (!(tagged(h, a.ebase) && (h & kTagValue) == 2ull)) return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s_phase[0] = ld_sc(&es->handoff[1]);
+    s_phase[1] = ld_sc(&es->handoff[2]);
+  }
+  const int rgx = d.gx, rgy = (d.gy + 1) / 2, rgz = (d.gz + 1) / 2;
+  const uint32_t nreg = (uint32_t)rgx * (uint32_t)rgy * (uint32_t)rgz;
+  // totals and box from the pack partials
+  __shared__ unsigned long long s_tot[4];
+  __shared__ int s_box[6];
+  if (threadIdx.x < 64) {
+    unsigned long long v[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) v[f] = wave_sum64(ld_sc(&acc->sums[f][lane]));
+    int bx[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const int x = wave_max(ld_sc(&acc->box[k][lane]), lane);
+      bx[k] = k < 3 ? box_dec_lo(x) : box_dec_hi(x);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) s_box[k] = bx[k];
+    }
+  }
+  __syncthreads();
+  const PairView v0 = make_view(d, s_box, s_box + 3, s_tot[2], s_tot[3]);
+  const Box &bb = v0.b;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int m = 0; m < 2; ++m) {
+      ps.cut[m] = v0.cut[m];
+      ps.cov[m] = ps.sum[m] = 0;
+      ps.hd[m] = 0;
+      ps.done_from[m] = 0x7FFFFFFF;
+    }
+  }
+  bool pdone[2] = {v0.cut[0] == 0, v0.cut[1] == 0};
+  // The seed period runs on every workgroup (its queue is as dense as the
+  // volume's surface); the later periods, whose queue holds the moving fronts
+  // only, on the first `active` (one per CU by default), because every
+  // workgroup adds an arrival to the barrier.  The others keep no entry for
+  // the next period and leave after the first barrier.
+  const uint32_t active =
+      a.tb_active > 0 ? min(gridDim.x, (uint32_t)a.tb_active) : gridDim.x;
+  const bool leaving = blockIdx.x >= active;
+  const uint32_t kcap = leaving ? 0u : 1u;  // entries kept for the next period
+
+  // One queued (region, mask) by the whole workgroup at period tp; requeue:
+  // whether it may queue itself for the next period (not when another
+  // workgroup's claim already did).
+  uint64_t *trp = nullptr;  // trace record of the running period (timing 2) or null
+  auto step_region = [&](uint64_t ent, uint32_t tp, bool requeue) {
+    const uint64_t ts1 = trp ? wall_clock64() : 0ull;
+    const uint32_t rg = (uint32_t)ent;
+    const int m = (int)((ent >> 48) & 1);
+    const bool check = ((ent >> 50) & 1) != 0;
+    const int rx = (int)(rg % (uint32_t)rgx);
+    const int ry = (int)((rg / (uint32_t)rgx) % (uint32_t)rgy);
+    const int rz = (int)(rg / ((uint32_t)rgx * (uint32_t)rgy));
+    const int tx = rx, ty0 = 2 * ry, tz0 = 2 * rz;
+    uint32_t *rst = m ? v0.rstate[1] : v0.rstate[0];
+    const uint64_t *orig_m = (const uint64_t *)(m ? v0.orig[1] : v0.orig[0]);
+    const uint64_t *orig_o = (const uint64_t *)(m ? v0.orig[0] : v0.orig[1]);
+    // pointer selects, not indexing: indexing the view with run-time values
+    // would put it in scratch memory
+    uint64_t *buf0 = (uint64_t *)(m ? v0.buf[1][0] : v0.buf[0][0]);
+    uint64_t *buf1 = (uint64_t *)(m ? v0.buf[1][1] : v0.buf[0][1]);
+    const uint64_t *bufr = (tp & 1) ? buf0 : buf1;  // slot (tp + 1) & 1
+    uint64_t *bufw = (tp & 1) ? buf1 : buf0;        // slot tp & 1
+    // 1. in one round trip: the state of every region of the 3 x 3 x 3
+    // neighbourhood (threads 0..26), and both candidates (orig, buf) of the
+    // thread's four rows in the tiles at x-1, x, x+1; the states then pick
+    if (threadIdx.x < 27) {
+      const int j = threadIdx.x;
+      const int nx_ = rx + j % 3 - 1, ny_ = ry + (j / 3) % 3 - 1, nz_ = rz + j / 9 - 1;
+      uint32_t src = 0;
+      if (tb_region_in_box(bb, nx_, ny_, nz_)) {
+        const uint32_t st = ld_sc(rst + (uint32_t)nx_ + (uint32_t)rgx * ((uint32_t)ny_ + (uint32_t)rgy * (uint32_t)nz_));
+        src = (st & kFull) ? 3u : (st != 0 && st - 1 < tp) ? 2u : 1u;
+      }
+      s_src[j] = src;
+    }
+    if (threadIdx.x < 3) s_or[threadIdx.x] = 0;
+    const int ty = ty0 + tyi, tz = tz0 + tzi;
+    const int yl = yy & 7;
+    const bool yok = ty * 8 + yl < v0.ny;
+    bool tin[3];
+    uint64_t vb[3];
+    uint32_t tix[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ux = tx + c - 1;
+      tin[c] = tb_tile_in_box(bb, ux, ty, tz);
+      vb[c] = tin[c] ? valid_bits<uint64_t>(ux, v0.nx) : 0ull;
+      tix[c] = tin[c] ? (uint32_t)ux + (uint32_t)v0.gx * ((uint32_t)ty + (uint32_t)v0.gy * (uint32_t)tz) : 0u;
+    }
+    bool rok[4];
+    uint64_t wo[4][3], wb[4][3], oth[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int zl = (zz0 + i) & 7;
+      rok[i] = yok && tz * 8 + zl < v0.nz;
+      const uint32_t r = (uint32_t)(yl + 8 * zl);
+      // every load is issued, at a valid address (tile 0 for a tile outside
+      // the box), and dropped afterwards where it does not count: no branch
+      // splits them into several round trips
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const uint64_t o = (uint64_t)tix[c] * R + r;
+        wo[i][c] = ldg(orig_m + o);
+        wb[i][c] = ld_sc(bufr + o);
+      }
+      oth[i] = ldg(orig_o + (uint64_t)tix[1] * R + r);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (!(core && check && rok[i] && tin[1])) oth[i] = 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (!(rok[i] && tin[c])) wo[i][c] = wb[i][c] = 0;
+    }
+    __syncthreads();
+    const int oyr = tyi < 0 ? -1 : tyi > 1 ? 1 : 0, ozr = tzi < 0 ? -1 : tzi > 1 ? 1 : 0;
+    uint32_t srcx[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) srcx[c] = tin[c] ? s_src[c + 3 * (oyr + 1) + 9 * (ozr + 1)] : 0u;
+    TbRow X[4], V[4];
+    bool notfull = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      uint64_t w[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        w[c] = srcx[c] == 1 ? wo[i][c] : srcx[c] == 2 ? wb[i][c] : srcx[c] == 3 ? vb[c] : 0ull;
+      V[i] = rok[i] ? tb_make(vb[0], vb[1], vb[2]) : TbRow{0, 0};
+      X[i] = tb_make(w[0], w[1], w[2]);
+      X[i].o &= V[i].o, X[i].h &= V[i].h;
+      if (core && X[i].o != V[i].o) notfull = true;
+    }
+    if (__ballot(notfull) != 0 && lane == 0) s_or[0] = 1;
+    __syncthreads();
+    if (s_or[0] == 0) {  // input full: the region leaves the queue (hello.cu:1319, 1634-1643)
+      if (threadIdx.x == 0) or_g(rst + rg, kFull);
+      __syncthreads();
+      return;
+    }
+    // 2. eight steps in the window
+    // this thread's new hits per step (at most 4 x 64): step k < 7 in bits
+    // 9k .. 9k + 8 of cntp, step 7 in cnt7 -- no array indexed by the step,
+    // so the loop need not be unrolled
+    static_assert(KS == 8, "eight steps: seven packed counts and one more");
+    uint64_t cntp = 0;
+    uint32_t cnt7 = 0;
+#pragma unroll 1
+    for (int k = 0; k < KS; ++k) {
+      // rows z-1 and z+1 beyond the thread's four: the last row of the
+      // segment below and the first of the segment above, through LDS.  At the
+      // window's edges (z = 0 and 31 here, y = 0 and 31 below) the neighbour
+      // read is some other row: a wrong voxel there moves inward one row per
+      // step, so after eight steps it is still one row short of the region's
+      // own rows (8 rows in) -- the edges need no masking
+      s_win[k & 1][seg][0][yy] = u64x2{X[0].o, X[0].h};
+      s_win[k & 1][seg][1][yy] = u64x2{X[3].o, X[3].h};
+      __syncthreads();
+      const u64x2 ezm = s_win[k & 1][seg > 0 ? seg - 1 : 0][1][yy];
+      const u64x2 ezp = s_win[k & 1][seg < 7 ? seg + 1 : 7][0][yy];
+      TbRow N[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // rows y-1 and y+1 are the same row of the lanes beside (lane = y in
+        // each half wave)
+        const TbRow ym = tb_lane_nb<true>(X[i]), yp = tb_lane_nb<false>(X[i]);
+        const TbRow zm = i > 0 ? X[i - 1] : TbRow{ezm.x, (uint32_t)ezm.y};
+        const TbRow zp = i < 3 ? X[i + 1] : TbRow{ezp.x, (uint32_t)ezp.y};
+        const TbRow xd = tb_xdil(X[i]);
+        N[i] = TbRow{(xd.o | ym.o | yp.o | zm.o | zp.o) & V[i].o,
+                     (xd.h | ym.h | yp.h | zm.h | zp.h) & V[i].h};
+      }
+      // new hits: res & ~src & ref (hello.cu:1590-1626), step 8tp + k + 1
+      // (oth is zero outside the core)
+      uint64_t nw[4];
+      uint32_t c = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        nw[i] = N[i].o & ~X[i].o & oth[i];
+        c += (uint32_t)__popcll(nw[i]);
+      }
+      if (k < KS - 1)
+        cntp += (uint64_t)c << (9 * k);
+      else
+        cnt7 = c;
+      if (v0.map) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          uint64_t bits = nw[i];
+          const int zl = (zz0 + i) & 7;
+          const uint64_t vrow = (uint64_t)tx * 64 + (uint64_t)v0.nx *
+                                ((uint64_t)(ty * 8 + yl) + (uint64_t)v0.ny * (uint64_t)(tz * 8 + zl));
+          const uint16_t it = (uint16_t)(KS * tp + k + 1);
+          while (bits) {
+            const int kb = (int)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            v0.map[vrow + kb] = it;
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) X[i] = N[i];
+    }
+    // 3. the neighbour regions the output reaches within eight steps: a
+    // voxel at (cx, cy, cz) of the region's 64 x 16 x 16 core is dy + dz away
+    // in y and z from the neighbour at (ox, oy, oz) and, with ox != 0, cx + 1
+    // or 64 - cx in x; the lowest and highest voxel of a row bound the x part
+    uint32_t reach = 0;
+    bool left = false;
+    if (core) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint64_t y = X[i].o;
+        if (oth[i] & ~y) left = true;
+        if (y == 0) continue;
+        const int cz = zz0 + i - 8;
+        const int xlo = __builtin_ctzll(y) + 1, xhi = __builtin_clzll(y) + 1;  // x distances
+#pragma unroll
+        for (int j = 0; j < 27; ++j) {
+          if (j == 13) continue;
+          const int ox = j % 3 - 1, oy = (j / 3) % 3 - 1, oz = j / 9 - 1;
+          const int dy = oy < 0 ? cy + 1 : oy > 0 ? 16 - cy : 0;
+          const int dz = oz < 0 ? cz + 1 : oz > 0 ? 16 - cz : 0;
+          const int dx = ox < 0 ? xlo : ox > 0 ? xhi : 0;
+          if (dx + dy + dz <= KS) reach |= 1u << j;
+        }
+      }
+    }
+    reach = (uint32_t)wave_reduce32((int)reach, 0, [](int x, int y) { return x | y; });
+    const bool anyleft = __ballot(left) != 0;
+    if (lane == 0) {
+      if (reach) atomicOr(&s_or[1], reach);
+      if (anyleft) s_or[2] = 1;
+    }
+    __syncthreads();
+    // 4. claims first (their compare-and-swap travels while the output is
+    // written and the hits are summed), then the pushes (wave 0): the region
+    // itself for the next period, then the idle neighbours it won
+    bool won = false;
+    uint32_t nrg = 0;
+    if (wave == 0 && lane < 27 && lane != 13 && ((s_or[1] >> lane) & 1) && s_src[lane] == 1) {
+      const int j = lane;
+      const int nx_ = rx + j % 3 - 1, ny_ = ry + (j / 3) % 3 - 1, nz_ = rz + j / 9 - 1;
+      nrg = (uint32_t)nx_ + (uint32_t)rgx * ((uint32_t)ny_ + (uint32_t)rgy * (uint32_t)nz_);
+      won = cas_g(rst + nrg, 0u, tp + 2) == 0u;
+    }
+    if (core && srcx[1] != 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int zl = (zz0 + i) & 7;
+        if (rok[i]) st_sc(bufw + (uint64_t)tix[1] * R + (uint32_t)(yl + 8 * zl), X[i].o);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const uint32_t ck = k < KS - 1 ? (uint32_t)(cntp >> (9 * k)) & 511u : cnt7;
+      const uint32_t sum = (uint32_t)wave_reduce32((int)ck, 0, [](int x, int y) {
+        return (int)((uint32_t)x + (uint32_t)y);
+      });
+      if (lane == 0 && sum) atomicAdd(&s_cnt[k == 0 ? m : 2 + 2 * k + m], sum);
+    }
+    if (wave == 0) {
+      uint64_t *Qout = sub_queue(q, qcap, (tp + 1) & 1, kq);
+      uint32_t *qout_len = &es->qlen[(tp + 1) % 4][kq];
+      const uint32_t par = (tp + 1) & 1;
+      push_sticky(requeue && lane == 0, make_entry(rg, 0u, (uint32_t)m, 0, s_or[2] != 0, v0.gen),
+                  s_st[par], &s_stn[par], kcap, lq, Qout, qout_len, lane, &s_cnt[2]);
+      push_sticky(won, make_entry(nrg, 0u, (uint32_t)m, 1, 1, v0.gen), s_st[par], &s_stn[par],
+                  kcap, lq, Qout, qout_len, lane, &s_cnt[2]);
+    }
+    __syncthreads();
+    // trace (timing 2): the period's entries stepped and the sum of their
+    // durations in ticks
+    if (trp && threadIdx.x == 0) {
+      atomicAdd((unsigned long long *)&trp[6], 1ull);
+      atomicAdd((unsigned long long *)&trp[7], (unsigned long long)(wall_clock64() - ts1));
+    }
+  };
+
+  // Periods.  Period 0 is the seed: each workgroup scans its regions (region
+  // i = blockIdx.x + gridDim.x * i, 256 at a time), queues as seed_kernel's
+  // rule does -- a region that has voxels and is not full, or an empty one
+  // whose neighbours hold voxels within eight steps of it (hello.cu:762-877,
+  // 1649-1663) -- and steps the regions it queued itself at once; a full
+  // region becomes kFull.  Claims of period-0 steps (0 -> 2) can reach a
+  // region before its own scan: the scan then takes it over (2 -> 1) and steps
+  // it without queueing it again.  Later periods take the workgroup's kept
+  // entry or the first of its free range of the global queue, then the global
+  // queue round robin.  Every entry goes through the one call of step_region.
+  const uint32_t per = gridDim.x * blockDim.x;
+  // the seed queue: parity 0 of the queue array, free during the first period
+  uint64_t *seedq = sub_queue(q, qcap, 0, 0);
+  const uint32_t seed_epoch = a.ebase + 3;
+  // a seed word carries this launch's epoch and bit 39 (kSeedMark), which no
+  // queue entry of a single pair has (its pair field is 0), so a slot still
+  // holding an older entry never reads as a seed
+  constexpr unsigned long long kSeedMark = 1ull << 39;
+  auto is_seed = [&](uint64_t w) { return tagged(w, seed_epoch) && (w & kSeedMark) != 0; };
+  bool scanned = false;
+  uint32_t sj = 0, se = 0;  // this workgroup's share of the seeds: [sj, se)
+  uint32_t chunk_end = nreg;
+  // each workgroup's segment of the seed area: two seeds per region it scans
+  const uint32_t seg_cap = 2 * ((nreg + gridDim.x - 1) / gridDim.x);
+  while (run) {
+    const uint32_t tp = t;
+    if (tp > 0) {
+      if ((s_flags[0] & 8u) == 0) break;  // no entries anywhere
+      pdone[0] = (s_flags[0] & 1u) != 0;
+      pdone[1] = (s_flags[0] & 2u) != 0;
+      if (pdone[0] && pdone[1]) break;
+    }
+    uint64_t *tr = (trace && (int)t < qstats_len) ? trace + 8 * (uint64_t)t : nullptr;
+    if (tr && blockIdx.x == 0 && threadIdx.x == 0) {
+      tr[0] = wall_clock64();
+      if (tp > 0) tr[5] = ps.cur_total;  // period 0: the seeds, once counted
+    }
+    trp = tr;
+    const uint32_t L = tp > 0 ? s_pre[kSubQ] : 0u;
+    const uint32_t nloc = tp > 0 ? s_stn[t & 1] : 0u;
+    int phase = 0;             // later periods: 0 the first slot, 1 round robin, 2 done
+    uint32_t ei = min(L, s_fs[1]) + blockIdx.x;  // next round-robin global entry
+    for (;;) {
+      uint64_t ent = 0;
+      bool have = false, requeue = true, global = false;
+      uint32_t gi = 0;
+      if (tp == 0) {
+        if (!scanned) {
+          // this workgroup's regions, 256 at a time: seeds go to the
+          // workgroup's own segment of the seed area as tagged words
+          if (threadIdx.x == 0) s_nseed = 0;
+          __syncthreads();
+          // the segments fit the parity-0 queue area (16 sub-queues of qcap =
+          // 2 x tiles entries) when the grid has at most one workgroup per
+          // region and 2 x kWgThreads in all (the host sizes it so); a grid
+          // that does not is refused here rather than written past
+          if ((uint64_t)gridDim.x * seg_cap > (uint64_t)kSubQ * qcap ||
+              gridDim.x > 2u * kWgThreads) {
+            if (threadIdx.x == 0) atomicOr(&es->error, 1u);
+            chunk_end = 0;
+          }
+          for (uint32_t chunk = 0; chunk < chunk_end; chunk += per) {
+            const uint32_t rg = blockIdx.x + gridDim.x * (chunk / gridDim.x + threadIdx.x);
+            if (rg >= nreg) continue;
+            const int rx = (int)(rg % (uint32_t)rgx);
+            const int ry = (int)((rg / (uint32_t)rgx) % (uint32_t)rgy);
+            const int rz = (int)(rg / ((uint32_t)rgx * (uint32_t)rgy));
+            if (!tb_region_in_box(bb, rx, ry, rz)) continue;
+            // the flags of the region's tiles and of the ring of tiles
+            // around them (x +-1, y and z +-1 tile), all loads at once
+            uint32_t fl[3][4][4];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  const int ux = rx + c - 1, ty = 2 * ry + b - 1, tz = 2 * rz + e - 1;
+                  const bool in = tb_tile_in_box(bb, ux, ty, tz);
+                  // tile 0 stands in for a tile outside the box: every
+                  // load issued at once, the stand-ins dropped below
+                  const uint32_t ti = in ? (uint32_t)ux + (uint32_t)v0.gx * ((uint32_t)ty + (uint32_t)v0.gy * (uint32_t)tz) : 0u;
+                  fl[c][b][e] = ld_sc(d.tflags + ti);
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                  if (!tb_tile_in_box(bb, rx + c - 1, 2 * ry + b - 1, 2 * rz + e - 1)) fl[c][b][e] = 0;
+            uint32_t any = 0, allfull = 3, reach = 0;  // bit m
+#pragma unroll
+            for (int b = 1; b <= 2; ++b)
+#pragma unroll
+              for (int e = 1; e <= 2; ++e) {
+                const int ty = 2 * ry + b - 1, tz = 2 * rz + e - 1;
+                if (!tb_tile_in_box(bb, rx, ty, tz)) continue;
+                const uint32_t f = fl[1][b][e];
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                  const uint32_t fm = f >> (16 * m);
+                  any |= (fm & 1u) << m;
+                  if (!((fm & 1u) && (fm & 2u))) allfull &= ~(1u << m);
+                }
+              }
+            // voxels of the ring within eight steps of an empty mask's
+            // region: any voxel of a y / z ring tile, and of an x
+            // neighbour its eight columns toward the region
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                  if (c == 1 && b >= 1 && b <= 2 && e >= 1 && e <= 2) continue;
+                  const uint32_t f = fl[c][b][e];
+                  const uint32_t bit = c == 1 ? 0u : c == 0 ? 15u : 14u;
+                  reach |= ((f >> bit) & 1u) | (((f >> (16 + bit)) & 1u) << 1);
+                }
+            for (int m = 0; m < 2; ++m) {
+              if (m ? pdone[1] : pdone[0]) continue;
+              uint32_t *rst = m ? v0.rstate[1] : v0.rstate[0];
+              const bool an = (any >> m) & 1u;
+              if (an && ((allfull >> m) & 1u)) {
+                or_g(rst + rg, kFull);
+              } else if (an || ((reach >> m) & 1u)) {
+                uint32_t old = cas_g(rst + rg, 0u, 1u);
+                bool rq = true;
+                if (old == 2u) {  // claimed for period 1 by a period-0 step
+                  old = cas_g(rst + rg, 2u, 1u) == 2u ? 0u : 1u;
+                  rq = false;
+                }
+                if (old == 0u) {
+                  const uint32_t pos = atomicAdd(&s_nseed, 1u);  // LDS: no contended counter
+                  st_sc(seedq + (uint64_t)blockIdx.x * seg_cap + pos,
+                        (uint64_t)tag(seed_epoch, kSeedMark | (rg << 2) | ((uint32_t)m << 1) | (rq ? 1u : 0u)));
+                }
+              }
+            }
+          }
+          __syncthreads();  // every seed of this workgroup is counted in s_nseed
+          // publish the count (tagged, in this workgroup's free-slot word,
+          // which the first barrier overwrites later), then gather every
+          // workgroup's count and take an equal contiguous share of all seeds
+          if (threadIdx.x == 0) st_sc(&es->fslot[blockIdx.x], tag(seed_epoch, s_nseed));
+          uint32_t part = 0;
+          for (uint32_t w2 = 2 * threadIdx.x; w2 < 2 * threadIdx.x + 2; ++w2) {
+            uint32_t cw = 0;
+            if (w2 < gridDim.x) {
+              const uint64_t t0 = wall_clock64();
+              for (;;) {
+                const unsigned long long x = ld_sc(&es->fslot[w2]);
+                if (tagged(x, seed_epoch)) {
+                  cw = (uint32_t)(x & kTagValue);
+                  break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (wall_clock64() - t0 > spin_limit) {
+                  atomicOr(&es->error, 1u);
+                  break;
+                }
+              }
+            }
+            s_seed[w2] = cw;
+            part += cw;
+          }
+          // exclusive prefix over the workgroups' counts (two per thread)
+          const uint32_t incl = wave_scan32(part);
+          __shared__ uint32_t s_psum[kWavesPerWg];
+          if (lane == 63) s_psum[wave] = incl;
+          __syncthreads();
+          uint32_t before = incl - part, total = 0;
+#pragma unroll
+          for (int k2 = 0; k2 < kWavesPerWg; ++k2) {
+            if (k2 < wave) before += s_psum[k2];
+            total += s_psum[k2];
+          }
+          const uint32_t c0w = s_seed[2 * threadIdx.x], c1w = s_seed[2 * threadIdx.x + 1];
+          __syncthreads();
+          s_seed[2 * threadIdx.x] = before;
+          s_seed[2 * threadIdx.x + 1] = before + c0w;
+          (void)c1w;
+          __syncthreads();
+          sj = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
+          se = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
+          scanned = true;
+          if (tr && blockIdx.x == 0 && threadIdx.x == 0) tr[5] = total;
+        }
+        // the next seed of this workgroup's share: global index sj, in the
+        // segment of the workgroup whose prefix range holds it
+        if (sj < se && threadIdx.x == 0) {
+          uint32_t lo = 0, hi = gridDim.x - 1;  // last w with prefix[w] <= sj
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s_seed[mid] <= sj)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          const uint64_t *src = seedq + (uint64_t)lo * seg_cap + (sj - s_seed[lo]);
+          const uint64_t t0 = wall_clock64();
+          uint64_t w = 0;
+          for (;;) {  // the count was published after the seed, but may be seen first
+            w = ld_sc(src);
+            if (is_seed(w)) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (wall_clock64() - t0 > spin_limit) {
+              atomicOr(&es->error, 1u);
+              w = 0;
+              break;
+            }
+          }
+          s_ent = w;
+        }
+        if (sj < se) {
+          ++sj;
+        __syncthreads();
+        const uint64_t w = s_ent;
+        __syncthreads();
+        if (is_seed(w)) {
+          const uint32_t e = (uint32_t)(w & 0xFFFFFFFFull);
+          ent = make_entry(e >> 2, 0u, (e >> 1) & 1u, 1, 1, v0.gen);
+          requeue = (e & 1u) != 0;
+          have = true;
+        }
+        }
+      } else {
+        if (phase == 0) {
+          phase = 1;
+          if (nloc) {
+            ent = s_st[t & 1][0];
+            have = true;
+          } else if (s_fs[0] < L && s_fs[0] < s_fs[1]) {
+            gi = s_fs[0];
+            have = global = true;
+          }
+        }
+        if (!have && phase == 1) {
+          if (ei < L) {
+            gi = ei;
+            ei += gridDim.x;
+            have = global = true;
+          } else {
+            phase = 2;
+          }
+        }
+      }
+      if (!have) break;
+      if (global) {  // global entry gi of this period's queue
+        if (threadIdx.x == 0) {
+          int sq = 0;
+          while (sq + 1 < kSubQ && s_pre[sq + 1] <= gi) ++sq;
+          s_ent = ld_sc(sub_queue(q, qcap, tp & 1, sq) + (gi - s_pre[sq]));
+        }
+        __syncthreads();
+        ent = s_ent;
+        __syncthreads();
+      }
+      const int m = (int)((ent >> 48) & 1);
+      if (m ? pdone[1] : pdone[0]) continue;  // a pass whose cut was reached stops (hello.cu:1748-1759)
+      if (KS * tp + 1 >= (uint32_t)v0.hist_len) {  // beyond any distance in the volume
+        if (threadIdx.x == 0) st_sc(&c0.status, -8);
+        continue;
+      }
+      step_region(ent, tp, requeue);
+    }
+    if (tr) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const uint64_t now = wall_clock64();
+        atomicMax((unsigned long long *)&tr[1], (unsigned long long)now);
+        if (blockIdx.x == 0) tr[2] = now;
+      }
+    }
+    const uint32_t nflushed =
+        flush_local_queue(lq, sub_queue(q, qcap, (t + 1) & 1, kq), &es->qlen[(t + 1) % 4][kq]);
+    if (threadIdx.x == 0) s_cnt[2] += nflushed, s_cnt[3] = s_stn[(t + 1) & 1];
+    if (tp == 0 && blockIdx.x == 0 && threadIdx.x == 0) s_phase[2] = wall_clock64();
+    const uint32_t tdone = t;
+    ++t;
+    if (!grid_barrier_single<KS>(es, eoff + t, spin_limit, &s_ok, s_cnt, &s_stn[tdone & 1], ps,
+                                 v0, tdone, qstats, qstats_len, tr, s_pre, s_flags, s_red, 1,
+                                 s_kept, s_fs, (t + 2) % 4, tdone == 0 ? gridDim.x : active,
+                                 active))
+      break;
+    if (leaving) break;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the master's state, for finalize
+    for (int m = 0; m < 2; ++m) {
+      st_sc(&c0.cov[m], ps.cov[m]);
+      st_sc(&c0.sum_d[m], ps.sum[m]);
+      st_sc(&c0.hd_last[m], ps.hd[m]);
+      st_sc(&c0.done_from[m], ps.done_from[m]);
+    }
+  }
+  if (blockIdx.x == 0) {
+    const bool err = ld_sc(&es->error) != 0;
+    if (threadIdx.x == 0) a.outs[0] = finalize_pair(d, c0, t, err, KS);
+    if (a.ready) __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t qn = ps.next_total;
+      st_sc(&es->t, t);
+      st_sc(&es->qnext, qn);
+      st_sc(&es->qtotal, ps.qtotal), st_sc(&es->qmax, ps.qmax);
+      if (a.head) {
+        EngineHead h;
+        h.t = t;
+        h.error = err ? 1u : 0u;
+        h.qmax = ps.qmax;
+        h.nwg = gridDim.x;
+        h.qtotal = ps.qtotal;
+        h.qcap = es->qcap;
+        h.qnext = qn;
+        h.phase[0] = s_phase[0];
+        h.phase[1] = s_phase[1];
+        h.phase[2] = s_phase[2];
+        h.phase[3] = wall_clock64();
+        *a.head = h;
+      }
+      if (a.ready) {  // the head too reaches the host before the marker
+        __threadfence_system();
+        __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- launchers
 template <int TXB, int TY, int TZ>
 hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
@@ -2594,6 +3442,10 @@ hipError_t launch_fused(dim3 grid, hipStream_t s, const DilateArgs &a) {
 hipError_t launch_fused_ds(dim3 grid, hipStream_t s, const DilateArgs &a) {
   return launch_persistent(dilate_ds_kernel, grid, s, a);
 }
+// regions of 1 x 2 x 2 tiles, eight dilation steps per barrier (64x8x8, 6 element)
+hipError_t launch_fused_tb(dim3 grid, hipStream_t s, const DilateArgs &a) {
+  return launch_persistent(dilate_tb_kernel, grid, s, a);
+}
 
 // DS: every launch of the entry runs two dilation steps per engine iteration
 // (its fused launch is dilate_ds_kernel)
@@ -2606,11 +3458,13 @@ TileLaunch make_launch() {
                     &launch_dilate<TXB, TY, TZ, CONN, DS>,
                     DS ? &launch_fused_ds : &launch_fused<TXB, TY, TZ, CONN>,
                     kDS ? &launch_fused_ds : nullptr,
+                    kDS ? &launch_fused_tb : nullptr,
                     (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN, DS>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN, DS>,
                     DS ? (const void *)&dilate_ds_kernel
                        : (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
                     kDS ? (const void *)&dilate_ds_kernel : nullptr,
+                    kDS ? (const void *)&dilate_tb_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     TXB,
                     TY,

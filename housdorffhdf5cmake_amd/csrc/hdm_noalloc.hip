@@ -49,6 +49,7 @@ struct NoAllocDev {
   int cus = 0;
   int occ_fused[kLaunchIds] = {};
   int occ_fused_ds[kLaunchIds] = {};
+  int occ_fused_tb[kLaunchIds] = {};
   double clock_khz = 0;
   uint32_t ebase = 0;  // epoch base of the next launch's barrier tags
   int acc_set = 0;     // the pack-partial set the next launch packs into
@@ -85,6 +86,13 @@ int setup(NoAllocDev &d, int dev) {
           nb < 1)
         return HDM_E_HIP;
       d.occ_fused_ds[k] = nb;
+    }
+    if (tl->fused_tb_fn) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tl->fused_tb_fn, kWgThreads, 0) !=
+              hipSuccess ||
+          nb < 1)
+        return HDM_E_HIP;
+      d.occ_fused_tb[k] = nb;
     }
   }
   d.ptr = (unsigned char *)p;
@@ -180,14 +188,22 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
   a.ebase = D.ebase;
   a.acc_set = D.acc_set;
   a.coop = coop_enabled() ? 1 : 0;
+  a.tb_active = D.cus;
+  // the launches hdm_hausdorff_u8 makes: two steps per barrier (else one),
+  // then the region launch, to which the first hands long pairs
   const bool ds = tl->fused_ds && conn == 6 && ds_enabled();
+  const bool tb = ds && tl->fused_tb && tb_enabled();
+  a.tb_mode = tb ? tb_mode() : 0;
   const int grid = fused_grid_of(ds ? D.occ_fused_ds[tile] : D.occ_fused[tile], D.cus,
                                  opts.grid_ctas, g.ntiles);
   hipError_t e = (ds ? tl->fused_ds : tl->fused)(dim3(grid), s, a);
-  // until the head is read, assume the launch used every epoch it could
+  if (e == hipSuccess) D.acc_set ^= 1;  // the launch zeroes the other set for the next one
+  if (e == hipSuccess && tb)
+    e = tl->fused_tb(
+        dim3(tb_grid(D.occ_fused_tb[tile], D.cus, opts.grid_ctas, g.ntiles, g.nregions)), s, a);
+  // until the head is read, assume the launches used every epoch they could
   D.ebase = a.ebase + 8 + (uint32_t)g.hist_len;
   if (e != hipSuccess) return map_hip(e);
-  D.acc_set ^= 1;  // the launch zeroed the other set for the next one
   PairOut o;
   EngineHead h;
   if (hipMemcpyAsync(&o, dout, sizeof(o), hipMemcpyDeviceToHost, s) != hipSuccess ||

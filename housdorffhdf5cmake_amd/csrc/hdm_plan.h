@@ -51,6 +51,13 @@ struct DilateArgs {
   // runtime refuses a grid that cannot be resident at once instead of letting
   // the grid barrier wait for workgroups that never start (host choice only)
   int32_t coop;
+  // the region launch: workgroups that stay for the periods after the first
+  // (the others leave once the seed period is over); 0 = all
+  int32_t tb_active;
+  // the two-step fused launch: 0 computes every pair itself; 1 hands a pair
+  // whose HD lower bound reaches kTbMinSteps, after its pack, to the region
+  // launch queued behind it; 2 hands every pair over
+  int32_t tb_mode;
 };
 
 struct TileLaunch {
@@ -62,10 +69,14 @@ struct TileLaunch {
   hipError_t (*fused)(dim3, hipStream_t, const DilateArgs &);  // one pair, one launch
   // the fused launch with two dilation steps per barrier (null: not built)
   hipError_t (*fused_ds)(dim3, hipStream_t, const DilateArgs &);
+  // the fused launch over regions of 1 x 2 x 2 tiles staged in LDS, eight
+  // dilation steps per barrier (null: not built)
+  hipError_t (*fused_tb)(dim3, hipStream_t, const DilateArgs &);
   const void *dilate_fn;         // several pairs
   const void *dilate_single_fn;  // one pair
   const void *fused_fn;          // one pair, reset + pack + seed + dilation
   const void *fused_ds_fn;
+  const void *fused_tb_fn;
   const void *pack_fn;
   int txb, ty, tz;
   int conn;   // structuring element of the seed and dilation kernels
@@ -105,7 +116,9 @@ struct PairGeom {
   uint64_t words;      // ntiles * rows
   uint64_t row_words;  // wxn * ny * nz, the pack grid
   int hist_len;
-  uint64_t buf_bytes, xf_bytes, st_bytes, tf_bytes, hist_bytes, total;
+  // regions of 1 x 2 x 2 tiles (the eight-step single-pair launch)
+  uint32_t nregions;
+  uint64_t buf_bytes, xf_bytes, st_bytes, rs_bytes, tf_bytes, hist_bytes, total;
 };
 
 inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
@@ -123,8 +136,11 @@ inline PairGeom pair_geom(int nx, int ny, int nz, const TileShape &ts) {
   g.xf_bytes = align256((uint64_t)g.ntiles * 8 * xf_words(ts.txb, ts.ty, ts.tz));
   g.st_bytes = align256((uint64_t)g.ntiles * 4);
   g.tf_bytes = align256((uint64_t)g.ntiles * 4);
+  g.nregions = (uint32_t)((uint64_t)g.gx * ((g.gy + 1) / 2) * ((g.gz + 1) / 2));
+  g.rs_bytes = align256((uint64_t)g.nregions * 4);
   g.hist_bytes = align256((uint64_t)g.hist_len * 4);
-  g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + g.tf_bytes + 2 * g.hist_bytes;
+  g.total = 6 * g.buf_bytes + 6 * g.xf_bytes + 2 * g.st_bytes + 2 * g.rs_bytes + g.tf_bytes +
+            2 * g.hist_bytes;
   return g;
 }
 
@@ -146,6 +162,8 @@ inline void carve_pair(PairDesc &d, const PairGeom &g, char *base) {
     for (int s = 0; s < 2; ++s) d.xf_buf[m][s] = (uint64_t *)take(g.xf_bytes);
   d.state[0] = (uint32_t *)take(g.st_bytes);
   d.state[1] = (uint32_t *)take(g.st_bytes);
+  d.rstate[0] = (uint32_t *)take(g.rs_bytes);
+  d.rstate[1] = (uint32_t *)take(g.rs_bytes);
   d.tflags = (uint32_t *)take(g.tf_bytes);
   d.hist[0] = (uint32_t *)take(g.hist_bytes);
   d.hist[1] = (uint32_t *)take(g.hist_bytes);

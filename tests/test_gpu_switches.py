@@ -1,6 +1,6 @@
 """The comparison switches: HDM_DS=0 (one dilation step per grid barrier) and
 HDM_FUSED=0 (a single pair as four kernels) give the same results as the
-default two-step fused launch, bit for bit, for single pairs, batches and
+default fused launch (the region launch), bit for bit, for single pairs, batches and
 robust cuts; HDM_SERVER_LONE=0 (a lone server request through a slot) gives
 the same results as the lone one-pair launch.  The switches are read once per
 process (or server), so the switched runs happen in a child process on the
@@ -61,10 +61,13 @@ def run_here():
     return out
 
 
-@pytest.mark.parametrize("switch,value", [("HDM_DS", "0"), ("HDM_FUSED", "0"), ("HDM_COOP", "1")])
+@pytest.mark.parametrize("switch,value", [("HDM_DS", "0"), ("HDM_FUSED", "0"), ("HDM_COOP", "1"),
+                                          ("HDM_TB", "0"), ("HDM_TB", "2")])
 def test_switched_off_paths_give_the_same_results(switch, value):
-    """HDM_COOP=1 (the fused launch as a cooperative launch) is compared the
-    same way."""
+    """HDM_COOP=1 (the fused launch as a cooperative launch), HDM_TB=0 (the
+    two-step fused kernel instead of the region launch) and HDM_TB=2 (the
+    region body for every pair, not only for pairs whose HD bound reaches 16
+    steps) are compared the same way."""
     here = run_here()
     env = dict(os.environ, **{switch: value})
     r = subprocess.run([sys.executable, "-c", CHILD, ROOT, json.dumps(SPECS), json.dumps(KEYS)],
