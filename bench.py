@@ -31,12 +31,24 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 CT = dict(dims=(512, 512, 256), n=8, amin=10, amax=60, dmax=2, p=0.005, seed=2024)
 BATCH_VOLUMES = [(64, 64, 64), (128, 128, 128), (256, 256, 128), (384, 384, 256)]
 TILE_BYTES = {  # per queued (tile, mask): read source + write target + read the
-    # other mask's packed original (3 V/8) plus the six halo faces
+    # other mask's packed original (3 V/8) plus the six halo faces.  An entry
+    # moves these bytes whether it runs one dilation step or two.
     1: 3 * 64 * 8 * 8 // 8 + (2 * 8 * 8 + 4 * 64 * 8) // 8,
     2: 3 * 64 * 4 * 4 // 8 + (2 * 4 * 4 + 4 * 64 * 4) // 8,
     3: 3 * 64 * 8 * 4 // 8 + (2 * 8 * 4 + 2 * 64 * 4 + 2 * 64 * 8) // 8,
     4: 3 * 32 * 8 * 8 // 8 + (2 * 8 * 8 + 4 * 32 * 8) // 8,
 }
+# per queued (region, mask) of the region launch (64x16x16 voxels, eight steps):
+# read the 32x32-row window of the source (the region's rows and an 8-row ring,
+# 8 B each, plus 8 columns on each x side, 2 B per row), read the other mask's
+# 256 core rows and write the 256 core rows of the result
+REGION_BYTES = 32 * 32 * (8 + 2) + 2 * 16 * 16 * 8
+
+
+def entry_bytes(steps, tile=1):
+    """Algorithmic bytes one queued entry moves: a region entry (eight steps
+    per barrier) or a tile entry (one or two)."""
+    return REGION_BYTES if steps == 8 else TILE_BYTES[tile]
 
 
 def log(*a):
@@ -109,14 +121,33 @@ def copy_bandwidth(torch, mib=1024, reps=5):
     return round(2 * (mib << 20) / (ms * 1e-3) / 1e9, 1)
 
 
+def roofline_obj(kernel, algo_bytes, ms, traffic_name, **extra):
+    """The roofline object of one workload: algorithmic bytes per launch over
+    the launch's mean HIP-event time, against MI355X's 8 TB/s; traffic is the
+    counted HBM bytes per launch from profiles/traffic_<traffic_name>.json
+    (two rocprofv3 counter passes, scaled by a copy of known size), or None."""
+    gbs = algo_bytes / (ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+           "algo_bytes": int(algo_bytes), "kernel_ms": round(ms, 4)}
+    tf = os.path.join(ROOT, "profiles", f"traffic_{traffic_name}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            out["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    out.update(extra)
+    return out
+
+
 def roofline(res, avg, nvox, tile=1):
-    """Algorithmic bytes per launch of each phase over its mean event time."""
-    # a queued (tile, mask) entry does the work of steps_per_barrier dilation
-    # steps on its tile (two when the launch runs two steps per barrier)
+    """Algorithmic bytes per call of each phase over its mean event time; the
+    dominant kernel is the fused launch (with the region launch queued behind
+    it, which leaves at once unless the pair was handed over)."""
+    steps = avg.get("steps_per_barrier", 1)
+    moved = float(res["queue_total"]) * entry_bytes(steps, tile)
     algo = {
         "pack": 2.25 * nvox,  # read 2 uint8 masks, write 2 bits per voxel
         "meta": 0.25 * nvox,  # read both packed masks
-        "dilate": float(res["queue_total"]) * TILE_BYTES[tile] * avg.get("steps_per_barrier", 1),
+        "dilate": moved,      # counted as moved: one entry's bytes per entry
     }
     rows = {}
     for k, b in algo.items():
@@ -125,23 +156,16 @@ def roofline(res, avg, nvox, tile=1):
                    "gbs": round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
     if avg.get("fused"):
         # one launch does it all: its algorithmic bytes are the sum of the
-        # phases' (the seed reads 2 flag bytes per tile and is left out)
-        dom = "fused"
-        b = algo["pack"] + algo["dilate"]
-        ms = avg["dominant_timed_steps"]
-    else:
-        dom = max(("pack", "meta", "dilate"), key=lambda k: avg[k])
-        b = algo[dom]
-        ms = avg["dominant_timed_steps"] if dom == "dilate" else avg[dom]
-    r = {"ms": round(ms, 4), "algo_bytes": int(b), "gbs": round(b / (ms * 1e-3) / 1e9, 1)}
-    out = {"bound": "hbm", "kernel": dom, "achieved": r["gbs"], "peak": HBM_PEAK_GBS,
-           "unit": "GB/s", "frac": round(r["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-           "kernel_ms": r["ms"], "per_phase": rows}
-    tf = os.path.join(ROOT, "profiles", f"traffic_ct512_{dom}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            out["traffic"] = json.load(f).get("hbm_bytes_per_launch")
-    return out
+        # phases' (the seed reads 2 flag bytes per tile and is left out).  The
+        # second count charges a two-step tile entry one tile's bytes per step.
+        extra = {"per_phase": rows, "dilation_steps_per_entry": steps}
+        if steps in (1, 2):
+            extra["algo_bytes_counting_each_step"] = int(algo["pack"] + moved * steps)
+        return roofline_obj("fused", algo["pack"] + algo["dilate"], avg["dominant_timed_steps"],
+                            "ct512", **extra)
+    dom = max(("pack", "meta", "dilate"), key=lambda k: avg[k])
+    ms = avg["dominant_timed_steps"] if dom == "dilate" else avg[dom]
+    return roofline_obj(dom, algo[dom], ms, f"ct512_{dom}", per_phase=rows)
 
 
 def _oracle_times(oracle, g, s, threads, seconds, min_runs=5):
@@ -216,29 +240,36 @@ def leg_long(hdm, torch, reps):
     sp = hdm.synth_spec("long", 512, 512, 512, 100, 256, 256, 80, 60, 50, 300, 5)
     G, S = hdm.synth_device(sp)
     torch.cuda.synchronize()
-    res = hdm.hausdorff(G, S, timing=True)
+    nvox = G.numel()
+    res = hdm.hausdorff(G, S, timing=3)
     torch.cuda.synchronize()
-    phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
+    hdm.timing_log()
     t0 = time.perf_counter()
     for _ in range(reps):
-        res = hdm.hausdorff(G, S, timing=True)
+        res = hdm.hausdorff(G, S, timing=3)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    dom = hdm.timing_log()  # HIP events around the call's launches
+    phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
+    for _ in range(reps):
+        hdm.hausdorff(G, S, timing=True)
         for k, v in hdm.last_timings().items():
             if k in phases:
                 phases[k].append(v)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
     avg = {k: sum(v) / len(v) for k, v in phases.items()}
     hdm.hausdorff(G, S, timing=2)
     tr = hdm.last_trace()
     steps = hdm.last_timings()["steps"]  # dilation steps per grid barrier
     occ = [r["queued"] for r in tr]
     it_us = [r["next_start_us"] for r in tr if r["next_start_us"]]
-    dil_bytes = float(res["queue_total"]) * TILE_BYTES[1] * steps
+    dil_bytes = float(res["queue_total"]) * entry_bytes(steps)
+    dom_ms = sum(dom) / len(dom)
     return G, S, res, {
         "workload": "long_512x512x512_ellipsoid_80_60_50_cube5_gap300", "hd": res["hd"],
         "avg_hd": res["avg_hd"], "iterations": res["iterations"],
         "dilation_steps_per_barrier": steps,
-        "ms_per_pair": round(dt * 1e3, 3), "mask_voxels_per_s": round(2 * G.numel() / dt, 1),
+        "ms_per_pair": round(dt * 1e3, 3), "mask_voxels_per_s": round(2 * nvox / dt, 1),
+        "device_ms_per_pair": round(dom_ms, 4),
         "phase_ms": {k: round(v, 4) for k, v in avg.items()},
         "dilate_us_per_step": round(avg["dilate"] * 1e3 / max(1, res["iterations"]), 2),
         "queue_total": res["queue_total"], "queue_max": res["queue_max"],
@@ -246,7 +277,12 @@ def leg_long(hdm, torch, reps):
                                      "max": max(occ), "first8": occ[:8], "last8": occ[-8:]},
         "us_per_barrier_period_median_traced": round(statistics.median(it_us), 2) if it_us else None,
         "dilate_algo_bytes": int(dil_bytes),
-        "dilate_gbs": round(dil_bytes / (avg["dilate"] * 1e-3) / 1e9, 1)}
+        "dilate_gbs": round(dil_bytes / (avg["dilate"] * 1e-3) / 1e9, 1),
+        # the call's launches (two-step launch: reset, pack, hand-off; region
+        # launch: seed and dilation) as one: pack bytes plus the moved bytes
+        # of every queued entry
+        "roofline": roofline_obj("fused+regions", 2.25 * nvox + dil_bytes, dom_ms, "long512",
+                                 dilation_steps_per_entry=steps)}
 
 
 def leg_h5(hdm, torch, reps):
@@ -329,6 +365,11 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
     b.close()
     keys = ("hd", "sum", "count", "sum_fp", "sum_fn", "n_fp", "n_fn")
     same = all(all(o[k] == p[k] for k in keys) for o, p in zip(out, plain))
+    # the batch's launches (reset, pack, totals, seed, dilation) per replay:
+    # pack bytes of every pair plus the moved bytes of every queued entry
+    # (queue_total is the batch's, reported with each pair)
+    moved = float(out[0]["queue_total"]) * TILE_BYTES[best["tile_id"]]
+    dev_ms = statistics.median(ms)
     return {"pairs": n_pairs, "mask_voxels": 2 * nvox,
             "plain_pairs_per_s": round(n_pairs / t_plain, 1),
             "batched_pairs_per_s": round(n_pairs / t_batch, 1),
@@ -338,6 +379,9 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
             "queue_total": int(out[0]["queue_total"]), "queue_max": int(out[0]["queue_max"]),
             "tile": best["tile"], "chunk": best["chunk"], "graph": True,
             "identical_to_plain": bool(same),
+            "roofline": roofline_obj("batch graph replay", 2.25 * nvox + moved, dev_ms,
+                                     "batch256", dilation_algo_bytes=int(moved),
+                                     pack_algo_bytes=int(2.25 * nvox)),
             "autotune": [{k: e[k] for k in ("tile", "chunk", "ms", "status")}
                          for e in (table or [])][:12]}
 
@@ -372,6 +416,9 @@ def main():
                     help="size of the device-to-device copy timed for the attainable "
                          "bandwidth (default 1024); a run's copies all have this size, the "
                          "known-size kernel that scales HBM byte counters")
+    ap.add_argument("--only", choices=("small", "long", "batched"),
+                    help="run this one leg (and the copy probe) and print its object alone: "
+                         "a profile of one workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the engine is single-device (a pair or a batch per "
@@ -380,6 +427,19 @@ def main():
     import torch
 
     import housdorffhdf5cmake_amd as hdm
+    if args.only:
+        one = {"only": args.only}
+        if args.only == "small":
+            one["small_64"] = leg_small(hdm, torch, 200)[3]
+        elif args.only == "long":
+            one["long_512"] = leg_long(hdm, torch, args.long_reps)[3]
+        else:
+            one["batched"] = leg_batched(hdm, torch, args.batch_pairs, args.batch_reps,
+                                         not args.no_tune)
+        one["attainable_copy_gbs"] = copy_bandwidth(
+            torch, mib=args.copy_probe if args.copy_probe > 0 else 1024)
+        print(json.dumps(one), flush=True)
+        return
     log("single pair 512x512x256 ...")
     G, S, res, dt, nvox, avg = leg_single(hdm, torch, args.steps, args.warmup)
     value = 2 * nvox * args.steps / dt

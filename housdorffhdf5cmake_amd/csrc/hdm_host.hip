@@ -589,13 +589,28 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
                                           s, a);
       if (e == hipSuccess) c->acc_set ^= 1;  // the launch zeroes the other set first
     }
-    // the region launch, which runs only when the two-step launch hands the
-    // pair over (and otherwise leaves at once)
-    if (e == hipSuccess && tb)
-      e = tl->fused_tb(
-          dim3(tb_grid(c->occ_fused_tb[tile], c->cus, opts.grid_ctas, g.ntiles, g.nregions)), s,
-          a);
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
+    if (e == hipSuccess && tb) {
+      // the two-step launch either computes the pair or, after its pack,
+      // hands it over (status 2): then the region launch follows on the
+      // stream, and the end event is recorded again behind it
+      if (by_marker) {
+        if ((rc = wait_ready(s, ready, c->seq)) != HDM_OK) return rc;
+      } else if ((e = hipStreamSynchronize(s)) != hipSuccess) {
+        return map_hip(e);
+      }
+      if (__atomic_load_n(&hout->status, __ATOMIC_ACQUIRE) == 2) {
+        hout->status = 1;
+        if (by_marker) {
+          if (++c->seq == 0) ++c->seq;
+          a.ready_seq = c->seq;
+        }
+        e = tl->fused_tb(
+            dim3(tb_grid(c->occ_fused_tb[tile], c->cus, opts.grid_ctas, g.ntiles, g.nregions)),
+            s, a);
+        if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
+      }
+    }
     // until the result is read, assume the launch used every epoch it could
     c->ebase = a.ebase + 8 + (uint32_t)g.hist_len;
   } else {
@@ -625,7 +640,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   std::memcpy(&o, hout, sizeof(o));
   std::memcpy(&es, hhead, sizeof(es));
   if (o.status == 1) return HDM_E_HIP;
-  // the body that ran (the region launch picks it per pair)
+  // the body that ran (eight steps: the pair was handed to the region launch)
   c->last_ms[6] = (float)std::max(1, o.steps);
   if (fused) {
     c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used

@@ -1719,8 +1719,9 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       if (DS && s_hand) {
         // the region launch takes over from the packed pair: its region
         // states start idle, and workgroup 0 leaves the tagged hand-off word
-        // (with the clock stamps of the start and the pack) instead of the
-        // results; the region launch starts once every workgroup here is done
+        // (with the clock stamps of the start and the pack) and, in place of
+        // the results, status 2 and the ready marker: the host then queues
+        // the region launch, which starts once every workgroup here is done
         const uint32_t nreg = (uint32_t)d.gx * (uint32_t)((d.gy + 1) / 2) *
                               (uint32_t)((d.gz + 1) / 2);
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nreg;
@@ -1730,6 +1731,12 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           st_sc(&es->handoff[1], s_phase[0]);
           st_sc(&es->handoff[2], s_phase[1]);
           st_sc(&es->handoff[0], tag(a.ebase, 2ull));
+          __hip_atomic_store(&a.outs[0].status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          if (a.ready) {
+            __threadfence_system();
+            __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+          }
         }
         return;
       }
@@ -2773,8 +2780,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
   PairCtr &c0 = ctrs[0];
   // The two-step launch before this one on the stream reset and packed the
   // pair and zeroed its region states; it hands the pair over with a word
-  // tagged with this launch's epoch base.  Without it (the two-step launch
-  // computed the pair, or failed) every workgroup leaves at once.
+  // tagged with this launch's epoch base.  Without that word (a launch the
+  // host queued by mistake) every workgroup leaves at once.
   {
     const unsigned long long h = ld_sc(&es->handoff[0]);
     if (!(tagged(h, a.ebase) && (h & kTagValue) == 2ull)) return;

@@ -198,14 +198,24 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
                                  opts.grid_ctas, g.ntiles);
   hipError_t e = (ds ? tl->fused_ds : tl->fused)(dim3(grid), s, a);
   if (e == hipSuccess) D.acc_set ^= 1;  // the launch zeroes the other set for the next one
-  if (e == hipSuccess && tb)
-    e = tl->fused_tb(
-        dim3(tb_grid(D.occ_fused_tb[tile], D.cus, opts.grid_ctas, g.ntiles, g.nregions)), s, a);
   // until the head is read, assume the launches used every epoch they could
   D.ebase = a.ebase + 8 + (uint32_t)g.hist_len;
   if (e != hipSuccess) return map_hip(e);
   PairOut o;
   EngineHead h;
+  if (tb) {
+    // the two-step launch computes the pair or hands it over after its pack
+    // (status 2): then the region launch follows
+    if (hipMemcpyAsync(&o, dout, sizeof(o), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess)
+      return e != hipSuccess ? map_hip(e) : HDM_E_HIP;
+    if (o.status == 2) {
+      e = tl->fused_tb(
+          dim3(tb_grid(D.occ_fused_tb[tile], D.cus, opts.grid_ctas, g.ntiles, g.nregions)), s,
+          a);
+      if (e != hipSuccess) return map_hip(e);
+    }
+  }
   if (hipMemcpyAsync(&o, dout, sizeof(o), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipMemcpyAsync(&h, dhead, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess)
     return HDM_E_HIP;

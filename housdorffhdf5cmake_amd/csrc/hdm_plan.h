@@ -55,8 +55,8 @@ struct DilateArgs {
   // (the others leave once the seed period is over); 0 = all
   int32_t tb_active;
   // the two-step fused launch: 0 computes every pair itself; 1 hands a pair
-  // whose HD lower bound reaches kTbMinSteps, after its pack, to the region
-  // launch queued behind it; 2 hands every pair over
+  // whose HD lower bound reaches kTbMinSteps over after its pack (result
+  // status 2; the host then queues the region launch); 2 hands every pair over
   int32_t tb_mode;
 };
 
