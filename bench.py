@@ -381,7 +381,10 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
             "identical_to_plain": bool(same),
             "roofline": roofline_obj("batch graph replay", 2.25 * nvox + moved, dev_ms,
                                      "batch256", dilation_algo_bytes=int(moved),
-                                     pack_algo_bytes=int(2.25 * nvox)),
+                                     pack_algo_bytes=int(2.25 * nvox),
+                                     # 64x8x8 tiles run two steps per entry
+                                     algo_bytes_counting_each_step=int(2.25 * nvox + moved * (
+                                         2 if best["tile_id"] == hdm.TILE_64x8x8 else 1))),
             "autotune": [{k: e[k] for k in ("tile", "chunk", "ms", "status")}
                          for e in (table or [])][:12]}
 
