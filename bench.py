@@ -11,6 +11,9 @@ dilation, finalize, result copy).  Extra legs on the same JSON line:
            in a hipGraph (tile and chunk picked by the autotuner) versus 256
            plain per-pair calls, pairs/s and equality of every result;
   serving  Poisson arrivals against an HBM budget, p50/p99 latency, pairs/s;
+  serving_loaded  the same mix with resident packed masks at 200 and 5000
+           requests/s and in a closed loop of 64 requests (sustained pairs/s,
+           pairs per slice), and the closed loop with volumes;
   roofline achieved HBM GB/s of the dominant kernel from HIP-event phase times;
   cpu_baseline  the CPU oracle (same tiled algorithm, OpenMP) on the host cores,
            bit-exact against the GPU result.
@@ -140,8 +143,8 @@ def roofline_obj(kernel, algo_bytes, ms, traffic_name, **extra):
 
 def roofline(res, avg, nvox, tile=1):
     """Algorithmic bytes per call of each phase over its mean event time; the
-    dominant kernel is the fused launch (with the region launch queued behind
-    it, which leaves at once unless the pair was handed over)."""
+    dominant kernel is the fused launch (and the region launch after it when
+    the pair was handed over)."""
     steps = avg.get("steps_per_barrier", 1)
     moved = float(res["queue_total"]) * entry_bytes(steps, tile)
     algo = {
@@ -397,6 +400,27 @@ def leg_serving(hdm, torch, seconds, rate, budget_gb):
                                verify=None, return_pool=True)
 
 
+def leg_serving_loaded(hdm, torch, seconds, budget_gb):
+    """The serving engine under load: config 5's volume mix with its packed
+    masks resident (each pool volume registered once) at 200 requests/s (config
+    5's rate), at 5000 requests/s, and in a closed loop keeping 64 requests in
+    the server (the rate it sustains); the closed loop also with volumes
+    packed at every admission.  Every result is compared with the same pair
+    computed alone."""
+    from housdorffhdf5cmake_amd import serving
+    b = int(budget_gb * (1 << 30))
+    return {
+        "resident_poisson_200": serving.poisson_run(seconds=seconds, rate=200, budget_bytes=b,
+                                                    resident=True),
+        "resident_poisson_5000": serving.poisson_run(seconds=seconds, rate=5000, budget_bytes=b,
+                                                     resident=True),
+        "resident_closed_loop_64": serving.closed_loop_run(seconds=seconds, outstanding=64,
+                                                           budget_bytes=b, resident=True),
+        "volumes_closed_loop_64": serving.closed_loop_run(seconds=seconds, outstanding=64,
+                                                          budget_bytes=b, resident=False),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,6 +432,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-loaded-seconds", type=float, default=5.0,
+                    help="seconds of each loaded serving leg (0 skips them)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the 64^3 and the long-iteration 512^3 lines")
@@ -419,7 +445,7 @@ def main():
                     help="size of the device-to-device copy timed for the attainable "
                          "bandwidth (default 1024); a run's copies all have this size, the "
                          "known-size kernel that scales HBM byte counters")
-    ap.add_argument("--only", choices=("small", "long", "batched"),
+    ap.add_argument("--only", choices=("small", "long", "batched", "serving_loaded"),
                     help="run this one leg (and the copy probe) and print its object alone: "
                          "a profile of one workload")
     args = ap.parse_args()
@@ -436,6 +462,8 @@ def main():
             one["small_64"] = leg_small(hdm, torch, 200)[3]
         elif args.only == "long":
             one["long_512"] = leg_long(hdm, torch, args.long_reps)[3]
+        elif args.only == "serving_loaded":
+            one["serving_loaded"] = leg_serving_loaded(hdm, torch, args.serve_loaded_seconds, 64.0)
         else:
             one["batched"] = leg_batched(hdm, torch, args.batch_pairs, args.batch_reps,
                                          not args.no_tune)
@@ -488,6 +516,9 @@ def main():
         line["serving"], pool = leg_serving(hdm, torch, args.serve_seconds, args.serve_rate,
                                             64.0)
         extra += [("serving_pool", g, s, a) for g, s, a in pool]
+        if args.serve_loaded_seconds > 0:
+            log("serving under load ...")
+            line["serving_loaded"] = leg_serving_loaded(hdm, torch, args.serve_loaded_seconds, 64.0)
     log("cpu baseline ...")
     line["cpu_baseline"] = leg_cpu(hdm, G, S, res, args.cpu_seconds, extra)
     if "serving" in line:

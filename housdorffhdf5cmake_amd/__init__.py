@@ -82,6 +82,15 @@ class ServerOpts(C.Structure):
                 ("connectivity", C.c_int32)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("slices", C.c_uint64), ("pair_slices", C.c_uint64), ("max_in_flight", C.c_int32),
+                ("reserved", C.c_int32), ("lone", C.c_uint64), ("admitted", C.c_uint64),
+                ("registered_bytes", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}
+
+
 class H5Info(C.Structure):
     _fields_ = [("rank", C.c_int32), ("type_class", C.c_int32), ("elem_size", C.c_int32),
                 ("layout", C.c_int32), ("dims", C.c_uint64 * 8), ("n_elems", C.c_uint64),
@@ -157,6 +166,11 @@ def lib() -> C.CDLL:
         "hdm_server_load": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_uint64)], i32),
         "hdm_server_now": ([], C.c_double), "hdm_server_destroy": ([vp], None),
+        "hdm_server_register": ([vp, vp, i32, i32, i32, i32, i32, i32, C.POINTER(C.c_uint64)],
+                                i32),
+        "hdm_server_submit_masks": ([vp, u64, u64, C.POINTER(C.c_uint64)], i32),
+        "hdm_server_unregister": ([vp, u64], i32),
+        "hdm_server_stats_get": ([vp, C.POINTER(ServerStats)], i32),
         "hdm_synth_spec_sphere": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32, i32,
                                    i32, i32, C.c_double, u64], i32),
         "hdm_synth_spec_ellipsoids": ([C.POINTER(SynthSpec), i32, i32, i32, i32, i32, i32, i32,

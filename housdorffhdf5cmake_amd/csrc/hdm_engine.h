@@ -32,6 +32,7 @@ constexpr int kSubQ = 16;          // sub-queues: workgroup b flushes into b % k
 constexpr int kHitSlots = 16;      // per (pair, mask) per-iteration hit counters
 constexpr int kSumSlots = 64;      // per (pair, field) partial voxel counts
 constexpr int kMaxGrid = 4096;     // persistent workgroups (arrival flags)
+constexpr int kSlotListMax = 64;   // server slots per start-iteration update launch
 constexpr int kLineWords = 32;     // one 128-byte line
 // x-face words per tile: bit y + TY z of word [side] is the voxel of row (y, z)
 // in column 0 (side 0) or TXB-1 (side 1); 64x8x8 tiles also keep the second

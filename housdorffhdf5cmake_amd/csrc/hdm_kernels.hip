@@ -657,6 +657,67 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
   pack_tiles<TXB, TY, TZ, false>(descs, idx, ctrs, npairs, total_tiles, s_pieces);
 }
 
+// The pack pass of a pair whose two masks were packed before, each on its own
+// (the server's resident masks): d.orig and d.xf_orig point at them, and
+// gflags / sflags are their tile flags (the low 16 bits of each).  One wave
+// per tile step as in the pack: a lane reads its row of both masks (2 bits
+// per voxel instead of the pack's 2 bytes), counts |gold|, |segm|, |FP|, |FN|
+// and the non-empty tile box into the pair's partials, and writes the pair's
+// tile flags (gold low, segm high).  Tiles empty in both masks read nothing.
+template <int TXB, int TY, int TZ>
+__global__ __launch_bounds__(kWgThreads) void combine_kernel(const PairDesc *__restrict__ dp,
+                                                             PairCtr *ctr,
+                                                             const uint32_t *__restrict__ gflags,
+                                                             const uint32_t *__restrict__ sflags,
+                                                             uint64_t total_tiles) {
+  using Row = RowT<TXB>;
+  constexpr int R = TY * TZ;
+  constexpr int TPW = kWave / R;
+  const PairDesc &d = *dp;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / R, r = lane % R;
+  const int yl = r % TY, zl = r / TY;
+  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+  const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
+  const int slot = (int)(gw % kSumSlots);
+  PackAcc acc;
+  acc_clear(acc);
+  for (uint64_t base = gw * TPW; base < total_tiles; base += nw * TPW) {
+    const uint64_t g = base + sub;
+    const bool in = g < total_tiles;
+    const uint32_t tile = in ? (uint32_t)g : 0u;
+    const uint32_t gf = in ? gflags[tile] & 0xFFFFu : 0u, sf = in ? sflags[tile] & 0xFFFFu : 0u;
+    if (in && r == 0) d.tflags[tile] = gf | (sf << 16);
+    const int tx = (int)(tile % (uint32_t)d.gx);
+    const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+    const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+    // rows beyond the volume were never written by the masks' pack
+    const bool rowok = in && ty * TY + yl < d.ny && tz * TZ + zl < d.nz;
+    const uint64_t o = (uint64_t)tile * R + r;
+    const Row a = rowok && (gf & 1u) ? ((const Row *)d.orig[0])[o] : (Row)0;
+    const Row b = rowok && (sf & 1u) ? ((const Row *)d.orig[1])[o] : (Row)0;
+    acc.c[0] += popc_row(a), acc.c[1] += popc_row(b);
+    acc.c[2] += popc_row((Row)(b & ~a)), acc.c[3] += popc_row((Row)(a & ~b));
+    if ((a | b) != 0) {
+      acc.mn[0] = min(acc.mn[0], tx), acc.mx[0] = max(acc.mx[0], tx);
+      acc.mn[1] = min(acc.mn[1], ty), acc.mx[1] = max(acc.mx[1], ty);
+      acc.mn[2] = min(acc.mn[2], tz), acc.mx[2] = max(acc.mx[2], tz);
+    }
+  }
+  acc_flush(*ctr, acc, lane, slot);
+}
+
+// The engine iteration at which the pairs of some server slots start (a
+// staged pair joining, or kUnused for pairs that retire), written between two
+// slices on the engine's stream.
+struct SlotList {
+  int n;
+  uint16_t slot[kSlotListMax];
+};
+__global__ void set_start_kernel(PairDesc *descs, SlotList l, uint32_t t_start) {
+  if ((int)threadIdx.x < l.n) descs[l.slot[threadIdx.x]].t_start = t_start;
+}
+
 // |gold|, |segm|, |FP|, |FN| per pair from the spread partials (one wave per
 // pair); the single-pair seed pass folds them itself
 __global__ void totals_kernel(PairCtr *ctrs, int p0, int npairs) {
@@ -3408,6 +3469,13 @@ hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIn
   hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw);
   return hipGetLastError();
 }
+template <int TXB, int TY, int TZ>
+hipError_t launch_combine(dim3 grid, hipStream_t s, const PairDesc *d, PairCtr *c,
+                          const uint32_t *gf, const uint32_t *sf, uint64_t tiles) {
+  hipLaunchKernelGGL((combine_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, c, gf, sf,
+                     tiles);
+  return hipGetLastError();
+}
 template <int TXB, int TY, int TZ, int CONN, bool DS>
 hipError_t launch_seed(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
                        PairCtr *c, int np, Engine *es, uint64_t *q, uint32_t qcap, uint64_t tt,
@@ -3473,6 +3541,7 @@ TileLaunch make_launch() {
                     kDS ? (const void *)&dilate_ds_kernel : nullptr,
                     kDS ? (const void *)&dilate_tb_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
+                    &launch_combine<TXB, TY, TZ>,
                     TXB,
                     TY,
                     TZ,
@@ -3514,5 +3583,19 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
 extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs) {
   hipLaunchKernelGGL(hdm::totals_kernel, dim3(npairs), dim3(64), 0, s, ctrs, p0, npairs);
   return hipGetLastError();
+}
+
+extern "C" hipError_t hdm_launch_set_start(hipStream_t s, hdm::PairDesc *descs, const int *slots,
+                                           int n, uint32_t t_start) {
+  for (int i = 0; i < n; i += hdm::kSlotListMax) {
+    hdm::SlotList l{};
+    l.n = std::min(n - i, hdm::kSlotListMax);
+    for (int k = 0; k < l.n; ++k) l.slot[k] = (uint16_t)slots[i + k];
+    hipLaunchKernelGGL(hdm::set_start_kernel, dim3(1), dim3(hdm::kSlotListMax), 0, s, descs, l,
+                       t_start);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 

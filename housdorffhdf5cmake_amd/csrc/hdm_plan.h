@@ -78,6 +78,10 @@ struct TileLaunch {
   const void *fused_ds_fn;
   const void *fused_tb_fn;
   const void *pack_fn;
+  // the pack of a pair of masks packed before (combine_kernel): counts, box
+  // and the pair's tile flags from the masks' own flags
+  hipError_t (*combine)(dim3, hipStream_t, const PairDesc *, PairCtr *, const uint32_t *,
+                        const uint32_t *, uint64_t);
   int txb, ty, tz;
   int conn;   // structuring element of the seed and dilation kernels
   int steps;  // dilation steps per engine iteration of the seed and dilation kernels
@@ -224,3 +228,6 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
                                        hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
                                        int reset_engine, uint32_t qcap);
 extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs);
+// descs[slots[i]].t_start = t_start for i < n
+extern "C" hipError_t hdm_launch_set_start(hipStream_t s, hdm::PairDesc *descs, const int *slots,
+                                           int n, uint32_t t_start);

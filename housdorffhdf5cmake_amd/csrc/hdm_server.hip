@@ -6,12 +6,24 @@
 // join the running engine at an iteration boundary (their descriptor carries
 // the engine iteration at which they start, so every pair keeps its own
 // iteration count), and retire as soon as both of their passes are done.
-// One step = admit, one slice of slice_iters engine iterations, retire.
-// A lone request (nothing in flight, nothing else waiting) skips the slots:
-// it runs as the library's one-pair call on the server's stream, one fused
-// launch instead of the admission kernels and a slice.
-// Host-pointer inputs are copied on a separate copy stream; the engine stream
-// waits on an event per request, so copies overlap the dilation.
+//
+// Admission has two halves.  Staging -- the inputs' copy (host volumes), the
+// slot's reset and the pack (or, for registered masks, the combine pass) --
+// runs on the admission stream; it touches only the new pair's slot and
+// arena block, and the slot's descriptor keeps an unused start iteration.
+// Joining -- the start iteration and the seed into the engine's queue -- runs
+// on the engine stream after the staging's event.  One step = stage and join
+// what waits, launch one slice of slice_iters dilation steps, retire what it
+// finished.  Staging never runs beside a slice: a pack on the admission
+// stream during a slice held part of the slice's grid off the GPU, and its
+// grid barrier waited out its 2 s bound (measured: slices of two steps at
+// 5000 requests/s), so the kernels of both halves run between slices.
+// A lone request (nothing in flight, nothing else waiting) skips the
+// slots: it runs as the library's one-pair call on the engine stream.
+//
+// Registered masks: a volume packed once (hdm_server_register) stays in the
+// arena as packed bits, face words and tile flags; requests naming two of
+// them read those instead of re-packing their volumes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -79,15 +91,31 @@ struct Arena {
   }
 };
 
+constexpr uint32_t kUnused = 0xFFFFFFFFu;  // start iteration of a slot outside the engine
+
+// A registered mask: [packed bits][face words][tile flags][descriptor][index]
+// [counters] in one arena block; the last three serve its own pack only.
+struct MaskReg {
+  int nx = 0, ny = 0, nz = 0;
+  uint64_t off = 0, bytes = 0;
+  void *orig = nullptr;
+  uint64_t *xf = nullptr;
+  uint32_t *tflags = nullptr;
+  int refs = 0;          // queued, staged and in-flight requests naming it
+  bool released = false; // unregistered: freed when refs reaches 0
+};
+
 struct Request {
   uint64_t id = 0;
   const void *gold = nullptr, *segm = nullptr;
+  uint64_t mg = 0, ms = 0;  // registered masks (0: the volumes above)
   int nx = 0, ny = 0, nz = 0, dtype = 0, label = 1, ptr_kind = HDM_PTR_DEVICE;
   double t_submit = 0, t_admit = 0;
   int slot = -1;
+  bool active = false;  // joined the engine (else staged)
   uint64_t off = 0, bytes = 0;  // arena block: [inputs if copied][workspace]
   PairGeom geom{};
-  hipEvent_t copied = nullptr;
+  hipEvent_t staged = nullptr;  // the staging's end on the admission stream
 };
 
 }  // namespace
@@ -95,7 +123,8 @@ struct Request {
 struct hdm_server {
   hdm_server_opts o{};
   Ctx *ctx = nullptr;
-  hipStream_t s = nullptr, s_copy = nullptr;
+  hipStream_t s = nullptr;      // the engine: joins and slices
+  hipStream_t s_adm = nullptr;  // admissions: copies, resets, packs
   int tile = HDM_TILE_64x8x8;  // launch id
   int conn = 6;
   const TileLaunch *tl = nullptr;
@@ -109,10 +138,10 @@ struct hdm_server {
   std::deque<Request> pending;
   std::vector<Request> slots;  // slot index -> request (id 0 = free)
   std::vector<uint32_t> slot_gen;
-  // A slot freed after slice k is reused only after slice k+1 ran: that slice
-  // drops the queue entries its last owner left (stale generation), so one
-  // slot never holds a pair's entries and a predecessor's at once and the
-  // queue stays within 2 * tiles(max dims) per slot.
+  // A slot freed after slice k is reused only after slice k+1 has run: that
+  // slice drops the queue entries its last owner left (stale generation), so
+  // one slot never holds a pair's entries and a predecessor's at once and
+  // the queue stays within 2 * tiles(max dims) per slot.
   std::vector<uint64_t> freed_after;
   uint64_t slices = 0;
   std::vector<hipEvent_t> event_pool;
@@ -126,6 +155,9 @@ struct hdm_server {
   // completions of a failed slice beyond the caller's capacity, reported
   // first by the next steps
   std::deque<hdm_completion> backlog;
+  std::map<uint64_t, MaskReg> masks;
+  uint64_t next_mask = 1;
+  hdm_server_stats st{};
 };
 
 static hipEvent_t take_event(hdm_server *s) {
@@ -162,17 +194,15 @@ extern "C" void hdm_server_destroy(hdm_server *s) {
     if (s->ctx->has_pending && s->ctx->pending == s->s) (void)s->ctx->settle();
   }
   if (s->s) (void)hipStreamSynchronize(s->s);
-  if (s->s_copy) (void)hipStreamSynchronize(s->s_copy);
+  if (s->s_adm) (void)hipStreamSynchronize(s->s_adm);
   for (auto &r : s->slots)
-    if (r.copied) (void)hipEventDestroy(r.copied);
-  for (auto &r : s->pending)
-    if (r.copied) (void)hipEventDestroy(r.copied);
+    if (r.staged) (void)hipEventDestroy(r.staged);
   for (auto e : s->event_pool) (void)hipEventDestroy(e);
   if (s->common) (void)hipFree(s->common);
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   if (s->host) (void)hipHostFree(s->host);
   if (s->s) (void)hipStreamDestroy(s->s);
-  if (s->s_copy) (void)hipStreamDestroy(s->s_copy);
+  if (s->s_adm) (void)hipStreamDestroy(s->s_adm);
   (void)hipGetLastError();
   delete s;
 }
@@ -216,7 +246,7 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   uint64_t arena = o.hbm_budget;
   if (free_b && arena + L.total > (uint64_t)(free_b * 0.9)) arena = (uint64_t)(free_b * 0.9) - L.total;
   if (hipStreamCreateWithFlags(&s->s, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&s->s_copy, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->s_adm, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&s->common, L.total) != hipSuccess || hipMalloc(&s->arena_mem, arena) != hipSuccess ||
       hipHostMalloc(&s->host, align256(sizeof(PairOut) * o.max_slots) + 256,
                     hipHostMallocCoherent) !=
@@ -235,12 +265,12 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   s->freed_after.assign(o.max_slots, 0);
   s->slices = 1;
   s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1, o.max_slots);  // several pairs
-  // every slot starts unused: t_start beyond any engine iteration keeps it out
-  // of the per-iteration bookkeeping
+  // every slot starts unused: a start beyond any engine iteration keeps it
+  // out of the per-iteration bookkeeping
   std::vector<PairDesc> d(o.max_slots);
   for (auto &x : d) {
     std::memset(&x, 0, sizeof(x));
-    x.t_start = 0xFFFFFFFFu;
+    x.t_start = kUnused;
   }
   std::vector<PairCtr> cz(o.max_slots);
   std::memset(cz.data(), 0, sizeof(PairCtr) * o.max_slots);
@@ -264,9 +294,10 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
 }
 
 static uint64_t request_bytes(const hdm_server *s, const Request &r, uint64_t *in_bytes) {
+  (void)s;
   const uint64_t nvox = (uint64_t)r.nx * r.ny * r.nz;
   const uint64_t esz = r.dtype == HDM_DTYPE_U8 ? 1 : 4;
-  *in_bytes = r.ptr_kind == HDM_PTR_HOST ? 2 * align256(nvox * esz) : 0;
+  *in_bytes = r.ptr_kind == HDM_PTR_HOST && !r.mg ? 2 * align256(nvox * esz) : 0;
   return *in_bytes + r.geom.total;
 }
 
@@ -292,49 +323,223 @@ extern "C" int hdm_server_submit(hdm_server *s, const void *gold, const void *se
   return HDM_OK;
 }
 
-// Admit one request into slot p: inputs (copied or resident), descriptor,
-// counters, pack, totals, metadata -- the new pair's tiles join the queue of
-// the engine's current iteration.
-static int admit(hdm_server *s, Request &r, int p) {
+// Registration: the volume packed as both masks of a one-pair pack into the
+// mask's own block (both masks' stores go to the same words with the same
+// values), on the admission stream, waited for before returning.
+extern "C" int hdm_server_register(hdm_server *s, const void *vol, int nx, int ny, int nz,
+                                   int dtype, int label, int ptr_kind, uint64_t *handle_out) {
+  if (!s || !vol || !handle_out || nx < 1 || ny < 1 || nz < 1 ||
+      (dtype != HDM_DTYPE_U8 && dtype != HDM_DTYPE_I32) ||
+      (ptr_kind != HDM_PTR_DEVICE && ptr_kind != HDM_PTR_HOST))
+    return HDM_E_INVALID;
+  if (dtype == HDM_DTYPE_U8 && (label < 0 || label > 255)) return HDM_E_INVALID;
+  if (nx > s->o.max_nx || ny > s->o.max_ny || nz > s->o.max_nz) return HDM_E_CAPACITY;
+  const PairGeom g = pair_geom(nx, ny, nz, s->ts);
+  const uint64_t nvox = (uint64_t)nx * ny * nz;
+  const uint64_t esz = dtype == HDM_DTYPE_U8 ? 1 : 4;
+  const uint64_t b_desc = align256(sizeof(PairDesc)), b_idx = align256(sizeof(PackIndex)),
+                 b_ctr = align256(sizeof(PairCtr));
+  const uint64_t keep = g.buf_bytes + g.xf_bytes + g.tf_bytes;
+  const uint64_t bytes = keep + b_desc + b_idx + b_ctr;
+  const uint64_t in_bytes = ptr_kind == HDM_PTR_HOST ? align256(nvox * esz) : 0;
+  MaskReg m;
+  m.nx = nx, m.ny = ny, m.nz = nz;
+  if (!s->arena.alloc(bytes, &m.off)) return HDM_E_BUDGET;
+  m.bytes = bytes;
+  uint64_t in_off = 0;
+  if (in_bytes && !s->arena.alloc(in_bytes, &in_off)) {
+    s->arena.release(m.off, m.bytes);
+    return HDM_E_BUDGET;
+  }
+  char *base = (char *)s->arena_mem + m.off;
+  m.orig = base;
+  m.xf = (uint64_t *)(base + g.buf_bytes);
+  m.tflags = (uint32_t *)(base + g.buf_bytes + g.xf_bytes);
+  PairDesc *ddev = (PairDesc *)(base + keep);
+  PackIndex *idev = (PackIndex *)(base + keep + b_desc);
+  PairCtr *cdev = (PairCtr *)(base + keep + b_desc + b_idx);
+  const void *src = vol;
+  hipError_t e = hipSuccess;
+  if (in_bytes) {
+    src = (char *)s->arena_mem + in_off;
+    e = hipMemcpyAsync((void *)src, vol, nvox * esz, hipMemcpyHostToDevice, s->s_adm);
+  }
+  PairDesc d{};
+  fill_desc(d, g, src, src, dtype, label, s->o.robustness, 1, s->tile, kUnused, s->conn);
+  d.orig[0] = d.orig[1] = m.orig;
+  d.xf_orig[0] = d.xf_orig[1] = m.xf;
+  d.tflags = m.tflags;
+  d.hist_len = 0;  // the reset below clears no histogram
+  const PackIndex pi{0, 0};
+  if (e == hipSuccess)
+    e = hdm_launch_reset(s->s_adm, ddev, &d, &pi, idev, cdev, 0, 1, nullptr, 0, 0);
+  if (e == hipSuccess) {
+    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, g.row_words, g.ntiles, 0, 1, 1);
+    e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, ddev, idev, cdev, 1, g.ntiles);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->s_adm);
+  if (in_bytes) s->arena.release(in_off, in_bytes);
+  if (e != hipSuccess) {
+    s->arena.release(m.off, m.bytes);
+    return map_hip(e);
+  }
+  const uint64_t h = s->next_mask++;
+  s->masks[h] = m;
+  s->st.registered_bytes += align256(m.bytes);
+  *handle_out = h;
+  return HDM_OK;
+}
+
+static void mask_unref(hdm_server *s, uint64_t h) {
+  auto it = s->masks.find(h);
+  if (it == s->masks.end()) return;
+  MaskReg &m = it->second;
+  if (--m.refs <= 0 && m.released) {
+    s->arena.release(m.off, m.bytes);
+    s->st.registered_bytes -= align256(m.bytes);
+    s->masks.erase(it);
+  }
+}
+
+extern "C" int hdm_server_unregister(hdm_server *s, uint64_t handle) {
+  if (!s) return HDM_E_INVALID;
+  auto it = s->masks.find(handle);
+  if (it == s->masks.end() || it->second.released) return HDM_E_INVALID;
+  it->second.released = true;
+  ++it->second.refs;  // mask_unref below drops it now when nothing names it
+  mask_unref(s, handle);
+  return HDM_OK;
+}
+
+extern "C" int hdm_server_submit_masks(hdm_server *s, uint64_t gold, uint64_t segm,
+                                       uint64_t *id_out) {
+  if (!s) return HDM_E_INVALID;
+  auto ig = s->masks.find(gold), is = s->masks.find(segm);
+  if (ig == s->masks.end() || is == s->masks.end() || ig->second.released ||
+      is->second.released)
+    return HDM_E_INVALID;
+  const MaskReg &g = ig->second, &m = is->second;
+  if (g.nx != m.nx || g.ny != m.ny || g.nz != m.nz) return HDM_E_INVALID;
+  Request r;
+  r.id = s->next_id++;
+  r.mg = gold, r.ms = segm;
+  r.nx = g.nx, r.ny = g.ny, r.nz = g.nz, r.dtype = HDM_DTYPE_U8;
+  r.geom = pair_geom(r.nx, r.ny, r.nz, s->ts);
+  uint64_t inb = 0;
+  r.bytes = request_bytes(s, r, &inb);
+  if (align256(r.bytes) > s->arena.size) return HDM_E_BUDGET;
+  ++ig->second.refs, ++is->second.refs;
+  r.t_submit = now_s();
+  s->pending.push_back(r);
+  if (id_out) *id_out = r.id;
+  return HDM_OK;
+}
+
+// Staging of one request into slot p, on the admission stream: inputs
+// (copied, resident, or registered masks), descriptor with an unused start,
+// counters, pack or combine.  The running slice ignores the slot (its start
+// is unused), and nothing here touches the engine.
+static int stage(hdm_server *s, Request &r, int p) {
   uint64_t inb = 0;
   request_bytes(s, r, &inb);
   char *base = (char *)s->arena_mem + r.off;
   const void *gd = r.gold, *sd = r.segm;
-  if (r.ptr_kind == HDM_PTR_HOST) {
+  hipError_t e = hipSuccess;
+  if (r.ptr_kind == HDM_PTR_HOST && !r.mg) {
     const uint64_t nb = (uint64_t)r.nx * r.ny * r.nz * (r.dtype == HDM_DTYPE_U8 ? 1 : 4);
-    r.copied = take_event(s);
-    if (!r.copied ||
-        hipMemcpyAsync(base, r.gold, nb, hipMemcpyHostToDevice, s->s_copy) != hipSuccess ||
-        hipMemcpyAsync(base + inb / 2, r.segm, nb, hipMemcpyHostToDevice, s->s_copy) !=
-            hipSuccess ||
-        hipEventRecord(r.copied, s->s_copy) != hipSuccess ||
-        hipStreamWaitEvent(s->s, r.copied, 0) != hipSuccess)
+    if (hipMemcpyAsync(base, r.gold, nb, hipMemcpyHostToDevice, s->s_adm) != hipSuccess ||
+        hipMemcpyAsync(base + inb / 2, r.segm, nb, hipMemcpyHostToDevice, s->s_adm) !=
+            hipSuccess)
       return HDM_E_HIP;
     gd = base, sd = base + inb / 2;
   }
   PairDesc d{};
-  fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, s->t_engine,
-            s->conn);
+  fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, kUnused, s->conn);
   carve_pair(d, r.geom, base + inb);
+  const MaskReg *mg = nullptr, *ms = nullptr;
+  if (r.mg) {
+    mg = &s->masks.at(r.mg), ms = &s->masks.at(r.ms);
+    d.gold = d.segm = nullptr;
+    d.orig[0] = mg->orig, d.orig[1] = ms->orig;
+    d.xf_orig[0] = mg->xf, d.xf_orig[1] = ms->xf;
+  }
   d.gen = (++s->slot_gen[p]) & kGenMask;  // drops queue entries of the slot's last owner
   const PackIndex pi{0, 0};
-  hipError_t e = hdm_launch_reset(s->s, s->P.descs, &d, &pi, s->P.idx, s->P.ctrs, p, 1,
-                                  s->P.engine, 0, s->qcap);
+  e = hdm_launch_reset(s->s_adm, s->P.descs, &d, &pi, s->P.idx, s->P.ctrs, p, 1, nullptr, 0,
+                       s->qcap);
   if (e != hipSuccess) return map_hip(e);
   const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1, 1);
-  e = s->tl->pack(dim3(lc.pack_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
-                  r.geom.ntiles);
-  if (e == hipSuccess)  // one pair: the seed pass folds the totals itself
-    e = s->tl->seed(dim3(lc.seed_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
-                    s->P.engine, s->P.q, s->qcap, r.geom.ntiles, p);
+  if (r.mg)
+    e = s->tl->combine(dim3(lc.pack_blocks), s->s_adm, s->P.descs + p, s->P.ctrs + p,
+                       mg->tflags, ms->tflags, r.geom.ntiles);
+  else
+    e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, s->P.descs + p, s->P.idx + p,
+                    s->P.ctrs + p, 1, r.geom.ntiles);
   if (e != hipSuccess) return map_hip(e);
+  r.staged = take_event(s);
+  if (!r.staged || hipEventRecord(r.staged, s->s_adm) != hipSuccess) return HDM_E_HIP;
   r.slot = p;
+  r.active = false;
   r.t_admit = now_s();
   return HDM_OK;
 }
 
+// Joining the staged requests, on the engine stream between slices: after
+// their staging events, their start is the engine's current iteration (one
+// launch for all), and each one's seed (which folds its totals) puts its
+// first tiles in the current iteration's queue.  Returns the number joined.
+static int join_staged(hdm_server *s, int *joined) {
+  std::vector<int> list;
+  for (auto &r : s->slots) {
+    if (r.id == 0 || r.active) continue;
+    if (hipStreamWaitEvent(s->s, r.staged, 0) != hipSuccess) return HDM_E_HIP;
+    s->event_pool.push_back(r.staged);  // the wait holds the record it saw
+    r.staged = nullptr;
+    list.push_back(r.slot);
+  }
+  *joined = (int)list.size();
+  if (list.empty()) return HDM_OK;
+  hipError_t e = hdm_launch_set_start(s->s, s->P.descs, list.data(), (int)list.size(), s->t_engine);
+  for (int p : list) {
+    if (e != hipSuccess) break;
+    Request &r = s->slots[p];
+    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1, 1);
+    e = s->tl->seed(dim3(lc.seed_blocks), s->s, s->P.descs + p, s->P.idx + p, s->P.ctrs + p, 1,
+                    s->P.engine, s->P.q, s->qcap, r.geom.ntiles, p);
+    r.active = true;
+  }
+  return e == hipSuccess ? HDM_OK : map_hip(e);
+}
+
+// Stages pending requests in arrival order while a slot and arena room exist.
+static int stage_pending(hdm_server *s, bool *quarantined) {
+  while (!s->pending.empty()) {
+    int p = -1;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+      if (s->slots[i].id == 0) {
+        if (s->freed_after[i] < s->slices) {
+          p = i;
+          break;
+        }
+        *quarantined = true;
+      }
+    if (p < 0) break;
+    Request &r = s->pending.front();
+    if (!s->arena.alloc(r.bytes, &r.off)) break;
+    const int rc = stage(s, r, p);
+    if (rc != HDM_OK) {
+      s->arena.release(r.off, r.bytes);
+      return rc;
+    }
+    ++s->st.admitted;
+    s->slots[p] = r;
+    s->pending.pop_front();
+  }
+  return HDM_OK;
+}
+
 // The lone request at the head of the FIFO as a one-pair call (hdm_hausdorff_*)
-// on the server's stream: the same results as the pair computed alone, which
+// on the engine stream: the same results as the pair computed alone, which
 // every request's result equals.  Its workspace is the library context's (the
 // one the one-pair calls use, sized by the request), not the arena.  A
 // per-request outcome (done, empty mask, timeout, stuck) completes the
@@ -360,6 +565,7 @@ static int run_lone(hdm_server *s, hdm_completion *done, int *n_done) {
   if (rc != HDM_OK && rc != HDM_E_EMPTY_MASK && rc != HDM_E_TIMEOUT && rc != HDM_E_STUCK)
     return rc;
   s->pending.pop_front();
+  ++s->st.lone;
   hdm_completion &c = done[(*n_done)++];
   std::memset(&c, 0, sizeof(c));
   c.id = r.id;
@@ -379,53 +585,36 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
     s->backlog.pop_front();
   }
   if (!s->backlog.empty()) return HDM_OK;
-  if (s->lone && s->pending.size() == 1 && cap > *n_done &&
+  bool idle = true;  // no request staged or in flight
+  for (auto &r : s->slots) idle = idle && r.id == 0;
+  if (idle && s->lone && s->pending.size() == 1 && cap > *n_done && !s->pending.front().mg &&
       (s->pending.front().dtype != HDM_DTYPE_U8 ||
        (s->pending.front().label >= 0 && s->pending.front().label <= 255))) {
-    bool idle = true;
-    for (auto &r : s->slots) idle = idle && r.id == 0;
-    if (idle) {
-      const int rc = run_lone(s, done, n_done);
-      if (rc != HDM_E_NOMEM) return rc;
-      // no device memory for the one-pair workspace (the arena may hold most
-      // of it): this request and every later one go through the slots
-      s->lone = false;
-      (void)hipGetLastError();
-    }
+    const int rc = run_lone(s, done, n_done);
+    if (rc != HDM_E_NOMEM) return rc;
+    // no device memory for the one-pair workspace (the arena may hold most
+    // of it): this request and every later one go through the slots
+    s->lone = false;
+    (void)hipGetLastError();
   }
-  // admit in arrival order while a slot and arena room exist
-  int admitted = 0;
+  // join what the last step staged; with nothing in flight, stage and join
+  // now (there is no slice to overlap)
+  // stage what waits, then join it: between slices, never beside one
   bool quarantined = false;
-  while (!s->pending.empty()) {
-    int p = -1;
-    for (int i = 0; i < (int)s->slots.size(); ++i)
-      if (s->slots[i].id == 0) {
-        if (s->freed_after[i] < s->slices) {
-          p = i;
-          break;
-        }
-        quarantined = true;
-      }
-    if (p < 0) break;
-    Request &r = s->pending.front();
-    if (!s->arena.alloc(r.bytes, &r.off)) break;
-    const int rc = admit(s, r, p);
-    if (rc != HDM_OK) {
-      s->arena.release(r.off, r.bytes);
-      return rc;
-    }
-    s->slots[p] = r;
-    s->pending.pop_front();
-    ++admitted;
-  }
-  int busy = 0;
-  for (auto &r : s->slots) busy += r.id != 0;
+  int active = 0, joined = 0;
+  int rc = stage_pending(s, &quarantined);
+  if (rc != HDM_OK) return rc;
+  if ((rc = join_staged(s, &joined)) != HDM_OK) return rc;
+  for (auto &r : s->slots) active += r.id != 0 && r.active;
   // with nothing in flight a slice still runs when requests wait on
   // quarantined slots: it drops the stale entries and frees them
-  if (busy == 0 && !(quarantined && !s->pending.empty())) return HDM_OK;
+  if (active == 0 && !(quarantined && !s->pending.empty())) return HDM_OK;
   // one slice of the engine over every slot
   const int nslots = (int)s->slots.size();
   ++s->slices;
+  ++s->st.slices;
+  s->st.pair_slices += (uint64_t)active;
+  s->st.max_in_flight = std::max(s->st.max_in_flight, active);
   // the slice's workgroup 0 writes every slot's result and the engine head
   // into the pinned host block as the launch leaves
   char *h = (char *)s->host;
@@ -449,7 +638,7 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   // the step takes them from there instead of waiting for the stream (the
   // launch's workgroups touch no pair data after its last barrier, so the
   // slots and arena blocks freed below are free for the next admissions,
-  // whose work the stream orders after the launch)
+  // whose work the streams order after the launch)
   volatile uint32_t *ready =
       (volatile uint32_t *)(h + align256(sizeof(PairOut) * nslots) + 192);
   if (++s->seq == 0) ++s->seq;
@@ -464,10 +653,10 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   if (head.nwg == 0) return HDM_E_HIP;  // the launch did not complete
   s->t_engine = head.t;
   const double t_now = now_s();
-  static const uint32_t kUnused = 0xFFFFFFFFu;
+  std::vector<int> retired;
   for (int p = 0; p < nslots; ++p) {
     Request &r = s->slots[p];
-    if (r.id == 0) continue;
+    if (r.id == 0 || !r.active) continue;  // free, or staged during this slice
     PairOut o;
     std::memcpy(&o, h + sizeof(PairOut) * p, sizeof(o));
     int st = o.status;
@@ -495,17 +684,19 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
       s->backlog.push_back(c);  // the engine restarts below: report it later
     else
       continue;  // report it at the next step
-    // an unfinished pair leaves the engine's bookkeeping for good: with its
-    // start beyond every iteration, no later fold reads its counters or
-    // writes its (released) histogram
-    if (!o.done &&
-        hipMemcpyAsync(&s->P.descs[p].t_start, &kUnused, sizeof(kUnused), hipMemcpyHostToDevice,
-                       s->s) != hipSuccess)
-      return HDM_E_HIP;
-    if (r.copied) s->event_pool.push_back(r.copied);
+    retired.push_back(p);
     s->arena.release(r.off, r.bytes);
+    if (r.mg) mask_unref(s, r.mg), mask_unref(s, r.ms);
     r = Request();
     s->freed_after[p] = s->slices;
+  }
+  // the retired pairs leave the engine's bookkeeping for good: with their
+  // start beyond every iteration, no later fold reads their counters or
+  // writes their (released) histograms, and a later staging of the slot,
+  // which waits for the next slice, finds it unused
+  if (!retired.empty()) {
+    e = hdm_launch_set_start(s->s, s->P.descs, retired.data(), (int)retired.size(), kUnused);
+    if (e != hipSuccess) return map_hip(e);
   }
   if (head.error) {
     // a barrier wait expired and every request in flight failed with it: the
@@ -516,7 +707,6 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
     if (e2 != hipSuccess) return map_hip(e2);
     s->t_engine = 0;
   }
-  (void)admitted;
   return HDM_OK;
 }
 
@@ -527,5 +717,11 @@ extern "C" int hdm_server_load(hdm_server *s, int *queued, int *in_flight, uint6
   if (queued) *queued = (int)s->pending.size();
   if (in_flight) *in_flight = busy;
   if (bytes) *bytes = s->arena.used;
+  return HDM_OK;
+}
+
+extern "C" int hdm_server_stats_get(hdm_server *s, hdm_server_stats *out) {
+  if (!s || !out) return HDM_E_INVALID;
+  *out = s->st;
   return HDM_OK;
 }

@@ -290,6 +290,34 @@ int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx,
 int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int *n_done);
 /* Requests queued, in flight, and device bytes held. */
 int hdm_server_load(hdm_server *s, int *queued, int *in_flight, uint64_t *bytes_used);
+
+/* Resident masks.  No counterpart in the reference, which packs both volumes
+ * again on every call (hello.cu:537-748, freed at 2001-2012).
+ * hdm_server_register packs one volume (the voxels equal to label) with the
+ * server's tile and keeps the packed bits, face words and tile flags in the
+ * server's arena (charged to its HBM budget).  It returns once the mask is
+ * packed; the volume may be freed then.  A request naming two registered
+ * masks of equal dims (hdm_server_submit_masks) skips the pack at admission:
+ * its admission reads the masks' 2 bits per voxel instead of the volumes.
+ * HDM_E_BUDGET if the arena has no room, HDM_E_CAPACITY beyond max dims. */
+int hdm_server_register(hdm_server *s, const void *vol, int nx, int ny, int nz, int dtype,
+                        int label, int ptr_kind, uint64_t *handle_out);
+/* HDM_E_INVALID for unknown handles or unequal dims. */
+int hdm_server_submit_masks(hdm_server *s, uint64_t gold, uint64_t segm, uint64_t *id_out);
+/* Releases a registered mask; its memory returns to the arena once no queued
+ * or in-flight request names it.  HDM_E_INVALID for an unknown handle. */
+int hdm_server_unregister(hdm_server *s, uint64_t handle);
+
+typedef struct hdm_server_stats {
+  uint64_t slices;            /* slices of the several-pair engine launched     */
+  uint64_t pair_slices;       /* sum over those slices of the pairs in flight   */
+  int32_t max_in_flight;      /* most pairs in flight in one slice              */
+  int32_t reserved;
+  uint64_t lone;              /* requests run as one one-pair call              */
+  uint64_t admitted;          /* requests admitted through the slots            */
+  uint64_t registered_bytes;  /* arena bytes held by registered masks           */
+} hdm_server_stats;
+int hdm_server_stats_get(hdm_server *s, hdm_server_stats *out);
 double hdm_server_now(void);
 void hdm_server_destroy(hdm_server *s);
 

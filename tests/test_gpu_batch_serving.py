@@ -271,3 +271,81 @@ def test_poisson_serving_short_run():
     assert r["completed"] == r["requests"] > 0
     assert r["identical_alone_vs_mix"] and r["bit_exact_vs_cpu"]
     assert r["p99_ms"] >= r["p50_ms"] > 0
+
+
+@pytest.mark.parametrize("robust,conn", [(1.0, 6), (0.9, 6), (1.0, 18)])
+def test_registered_masks_equal_fresh_submit(robust, conn):
+    """Volumes registered once (packed on the device, no volume read at
+    admission) give the same result as the same pair submitted as volumes and
+    as the pair computed alone: each registered gold mask is used again with
+    several segm masks, in one wave and request by request; int32 and host
+    volumes register too.  A mask unregistered while requests name it stays
+    until they complete; a later request naming it is refused."""
+    base = mixed_pairs(4, seed=53)
+    vols = []
+    for g, s in base:  # a second segm per gold, same dims
+        s2 = s.clone()
+        s2[: s2.shape[0] // 3] = 0
+        vols += [(g, s), (g, s2)]
+    torch.cuda.synchronize()
+    alone = [hdm.hausdorff(g, s, robustness=robust, connectivity=conn) for g, s in vols]
+    srv = serving.Server(hbm_budget=2 << 30, max_slots=4, slice_iters=2, robustness=robust,
+                         connectivity=conn, max_dims=(256, 128, 100))
+    hg = {}
+    hs = []
+    for k, (g, s) in enumerate(vols):
+        if k % 2 == 0:
+            src = g.cpu().numpy() if k % 4 == 0 else g.int()  # host uint8, device int32
+            torch.cuda.synchronize()
+            hg[k // 2] = srv.register(src)
+        hs.append(srv.register(s))
+    fresh = {}
+    for rep in range(2):  # every registered mask named again in the second round
+        ids = {srv.submit_masks(hg[k // 2], hs[k]): k for k in range(len(vols))}
+        for c in srv.drain(60):
+            k = ids[c["id"]]
+            assert c["status"] == 0 and same(c, alone[k]), (rep, k)
+            fresh[k] = c
+    ids = {srv.submit(g, s): k for k, (g, s) in enumerate(vols)}
+    for c in srv.drain(60):
+        k = ids[c["id"]]
+        assert same(c, fresh[k]) and c["iterations"] == fresh[k]["iterations"], k
+    for k in range(len(vols)):  # one by one: the first request of a fresh engine
+        rid = srv.submit_masks(hg[k // 2], hs[k])
+        done = srv.drain(60)
+        assert [c["id"] for c in done] == [rid] and same(done[0], alone[k]), k
+    st = srv.stats()
+    assert st["registered_bytes"] > 0 and st["admitted"] >= 3 * len(vols)
+    rid = srv.submit_masks(hg[0], hs[0])
+    srv.unregister(hs[0])
+    done = srv.drain(60)
+    assert [c["id"] for c in done] == [rid] and same(done[0], alone[0])
+    with pytest.raises(hdm.HDMError) as e:
+        srv.submit_masks(hg[0], hs[0])
+    assert e.value.code == -1
+    small = torch.ones((8, 8, 8), dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    hsmall = srv.register(small)
+    with pytest.raises(hdm.HDMError) as e:
+        srv.submit_masks(hg[0], hsmall)  # unequal dims
+    assert e.value.code == -1
+    srv.close()
+
+
+@pytest.mark.parametrize("resident", [False, True])
+def test_loaded_serving_batches_and_equals_alone(resident):
+    """Under load -- Poisson arrivals at 6000 requests/s, and a closed loop
+    that keeps 24 requests in the server -- every result equals the pair
+    computed alone, with slices of eight steps and of two (pairs then stay in
+    flight across slices and new ones join between them); in the closed loop
+    the engine runs several pairs per slice (batch_mean > 1)."""
+    for si in (8, 2):
+        r = serving.poisson_run(seconds=0.5, rate=6000, budget_bytes=8 << 30, per_class=2,
+                                max_slots=32, slice_iters=si, resident=resident)
+        assert r["completed"] == r["requests"] > 1000, si
+        assert r["identical_alone_vs_mix"], si
+        c = serving.closed_loop_run(seconds=0.5, outstanding=24, budget_bytes=8 << 30,
+                                    per_class=2, max_slots=32, slice_iters=si,
+                                    resident=resident)
+        assert c["completed"] > 500 and c["identical_alone_vs_mix"], si
+        assert c["batch_mean"] > 4 and c["max_in_flight"] > 8, (si, c)
