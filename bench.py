@@ -32,6 +32,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 CT = dict(dims=(512, 512, 256), n=8, amin=10, amax=60, dmax=2, p=0.005, seed=2024)
+# the same generator on a 512^3 volume, twice the ellipsoids for the same
+# foreground share: a short-HD 512^3 pair (the long-iteration 512^3 pair
+# is the other kind)
+CT_CUBE = dict(dims=(512, 512, 512), n=16, amin=10, amax=60, dmax=2, p=0.005, seed=2025)
 BATCH_VOLUMES = [(64, 64, 64), (128, 128, 128), (256, 256, 128), (384, 384, 256)]
 TILE_BYTES = {  # per queued (tile, mask): read source + write target + read the
     # other mask's packed original (3 V/8) plus the six halo faces.  An entry
@@ -46,11 +50,17 @@ TILE_BYTES = {  # per queued (tile, mask): read source + write target + read the
 # 8 B each, plus 8 columns on each x side, 2 B per row), read the other mask's
 # 256 core rows and write the 256 core rows of the result
 REGION_BYTES = 32 * 32 * (8 + 2) + 2 * 16 * 16 * 8
+# per (region, mask) entry of the near pass (the same window, eight steps at
+# most, after the pack in the same launch): read the window of the source and
+# the other mask's 256 core rows; nothing is written back
+NEAR_BYTES = 32 * 32 * (8 + 2) + 16 * 16 * 8
 
 
-def entry_bytes(steps, tile=1):
-    """Algorithmic bytes one queued entry moves: a region entry (eight steps
-    per barrier) or a tile entry (one or two)."""
+def entry_bytes(steps, tile=1, near=False):
+    """Algorithmic bytes one queued entry moves: a near-pass entry, a region
+    entry (eight steps per barrier) or a tile entry (one or two)."""
+    if near:
+        return NEAR_BYTES
     return REGION_BYTES if steps == 8 else TILE_BYTES[tile]
 
 
@@ -58,43 +68,43 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def ct_pair(hdm, seed=CT["seed"]):
-    nx, ny, nz = CT["dims"]
-    sp = hdm.synth_spec("ellipsoids", nx, ny, nz, CT["n"], CT["amin"], CT["amax"], CT["dmax"],
-                        CT["p"], seed)
+def ct_pair(hdm, seed=None, spec=CT):
+    nx, ny, nz = spec["dims"]
+    sp = hdm.synth_spec("ellipsoids", nx, ny, nz, spec["n"], spec["amin"], spec["amax"],
+                        spec["dmax"], spec["p"], spec["seed"] if seed is None else seed)
     return hdm.synth_device(sp)
 
 
 COPIES = 4  # 4 x 134 MB of input, more than the 256 MB Infinity Cache
 
 
-def leg_single(hdm, torch, steps, warmup):
+def leg_single(hdm, torch, steps, warmup, spec=CT, copies=COPIES):
     """Timed steps cycle over COPIES device copies of the pair, so no step
     reads its input from the Infinity Cache where the previous step left it,
     and bracket the dominant launch with HIP events (timing 3: the fused
     launch, or the dilation kernel of a separate-kernel call; the events are
     read after the timed region); the phases are timed by as many extra calls
     with timing 1."""
-    G, S = ct_pair(hdm)
-    pairs = [(G, S)] + [(G.clone(), S.clone()) for _ in range(COPIES - 1)]
+    G, S = ct_pair(hdm, spec=spec)
+    pairs = [(G, S)] + [(G.clone(), S.clone()) for _ in range(copies - 1)]
     torch.cuda.synchronize()
     nvox = G.numel()
     for i in range(warmup):
-        res = hdm.hausdorff(*pairs[i % COPIES], timing=3)
+        res = hdm.hausdorff(*pairs[i % copies], timing=3)
     torch.cuda.synchronize()
     hdm.timing_log()
     t0 = time.perf_counter()
     for i in range(steps):
-        res = hdm.hausdorff(*pairs[i % COPIES], timing=3)
+        res = hdm.hausdorff(*pairs[i % copies], timing=3)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     dom = hdm.timing_log()  # the events of the timed steps, read after them
     lt = hdm.last_timings()
-    fused, steps_per_barrier = lt["fused"], lt["steps"]
+    fused, steps_per_barrier, launches = lt["fused"], lt["steps"], lt.get("launches", 0)
     assert len(dom) == steps and min(dom) > 0, dom
     phases = {"pack": [], "meta": [], "dilate": [], "finalize": [], "total": []}
     for i in range(steps):
-        hdm.hausdorff(*pairs[i % COPIES], timing=True)
+        hdm.hausdorff(*pairs[i % copies], timing=True)
         for k, v in hdm.last_timings().items():
             if k in phases:
                 phases[k].append(v)
@@ -102,6 +112,7 @@ def leg_single(hdm, torch, steps, warmup):
     avg["dominant_timed_steps"] = sum(dom) / len(dom)
     avg["fused"] = fused
     avg["steps_per_barrier"] = steps_per_barrier
+    avg["launches"] = launches
     del pairs
     return G, S, res, dt, nvox, avg
 
@@ -141,15 +152,19 @@ def roofline_obj(kernel, algo_bytes, ms, traffic_name, **extra):
     return out
 
 
-def roofline(res, avg, nvox, tile=1):
+def roofline(res, avg, nvox, tile=1, traffic_name="ct512"):
     """Algorithmic bytes per call of each phase over its mean event time; the
     dominant kernel is the fused launch (and the region launch after it when
     the pair was handed over)."""
     steps = avg.get("steps_per_barrier", 1)
-    moved = float(res["queue_total"]) * entry_bytes(steps, tile)
+    # the near pass finished the pair in the fused launch itself
+    near = bool(avg.get("fused")) and steps == 8 and avg.get("launches") == 1
+    moved = float(res["queue_total"]) * entry_bytes(steps, tile, near)
     algo = {
         "pack": 2.25 * nvox,  # read 2 uint8 masks, write 2 bits per voxel
-        "meta": 0.25 * nvox,  # read both packed masks
+        # the seed reads both packed masks; the near pass's count reads only
+        # the region bitmap
+        "meta": 0.0 if near else 0.25 * nvox,
         "dilate": moved,      # counted as moved: one entry's bytes per entry
     }
     rows = {}
@@ -161,11 +176,13 @@ def roofline(res, avg, nvox, tile=1):
         # one launch does it all: its algorithmic bytes are the sum of the
         # phases' (the seed reads 2 flag bytes per tile and is left out).  The
         # second count charges a two-step tile entry one tile's bytes per step.
-        extra = {"per_phase": rows, "dilation_steps_per_entry": steps}
+        extra = {"per_phase": rows, "dilation_steps_per_entry": steps,
+                 "body": "near pass" if near else "two-step" if steps == 2 else
+                 "region launch" if steps == 8 else "one step"}
         if steps in (1, 2):
             extra["algo_bytes_counting_each_step"] = int(algo["pack"] + moved * steps)
         return roofline_obj("fused", algo["pack"] + algo["dilate"], avg["dominant_timed_steps"],
-                            "ct512", **extra)
+                            traffic_name, **extra)
     dom = max(("pack", "meta", "dilate"), key=lambda k: avg[k])
     ms = avg["dominant_timed_steps"] if dom == "dilate" else avg[dom]
     return roofline_obj(dom, algo[dom], ms, f"ct512_{dom}", per_phase=rows)
@@ -235,6 +252,22 @@ def leg_small(hdm, torch, steps):
     return G, S, res, {"workload": "sphere_64x64x64_r20_shift_2_1_0_flip1pct", "hd": res["hd"],
                        "avg_hd": res["avg_hd"], "ms_per_pair": round(dt * 1e3, 4),
                        "pairs_per_s": round(1 / dt, 1), "mask_voxels_per_s": round(2 * G.numel() / dt, 1)}
+
+
+def leg_ct_cube(hdm, torch, steps, warmup):
+    """The config-2 generator on 512^3 (16 ellipsoids): a short-HD 512^3 pair,
+    timed like the headline (input copies cycled past the Infinity Cache,
+    HIP events around the fused launch, phases from extra calls)."""
+    G, S, res, dt, nvox, avg = leg_single(hdm, torch, steps, warmup, spec=CT_CUBE, copies=2)
+    out = {"workload": "ct_pair_512x512x512_16_ellipsoids", "dims": list(CT_CUBE["dims"]),
+           "seed": CT_CUBE["seed"], "hd": res["hd"], "avg_hd": res["avg_hd"],
+           "iterations": res["iterations"], "queue_total": res["queue_total"],
+           "ms_per_pair": round(dt / steps * 1e3, 4), "pairs_per_s": round(steps / dt, 2),
+           "mask_voxels_per_s": round(2 * nvox * steps / dt, 1),
+           "phase_ms": {k: round(v, 4) for k, v in avg.items()
+                        if k not in ("fused", "steps_per_barrier", "launches")},
+           "roofline": roofline(res, avg, nvox, traffic_name="ct512cube")}
+    return G, S, res, out
 
 
 def leg_long(hdm, torch, reps):
@@ -445,7 +478,8 @@ def main():
                     help="size of the device-to-device copy timed for the attainable "
                          "bandwidth (default 1024); a run's copies all have this size, the "
                          "known-size kernel that scales HBM byte counters")
-    ap.add_argument("--only", choices=("small", "long", "batched", "serving_loaded"),
+    ap.add_argument("--no-cube", action="store_true", help="skip the short-HD 512^3 line")
+    ap.add_argument("--only", choices=("small", "long", "cube", "batched", "serving_loaded"),
                     help="run this one leg (and the copy probe) and print its object alone: "
                          "a profile of one workload")
     args = ap.parse_args()
@@ -462,6 +496,8 @@ def main():
             one["small_64"] = leg_small(hdm, torch, 200)[3]
         elif args.only == "long":
             one["long_512"] = leg_long(hdm, torch, args.long_reps)[3]
+        elif args.only == "cube":
+            one["ct_512cube"] = leg_ct_cube(hdm, torch, args.steps, args.warmup)[3]
         elif args.only == "serving_loaded":
             one["serving_loaded"] = leg_serving_loaded(hdm, torch, args.serve_loaded_seconds, 64.0)
         else:
@@ -485,7 +521,8 @@ def main():
         "result": {k: res[k] for k in ("hd", "avg_hd", "count", "sum", "iterations",
                                        "queue_total", "queue_max")},
         "pairs_per_s": round(args.steps / dt, 2),
-        "phase_ms": {k: round(v, 4) for k, v in avg.items() if k not in ("fused", "steps_per_barrier")},
+        "phase_ms": {k: round(v, 4) for k, v in avg.items()
+                     if k not in ("fused", "steps_per_barrier", "launches")},
         "fused_launch": bool(avg["fused"]),
         "dilation_steps_per_barrier": int(avg["steps_per_barrier"]),
     }
@@ -500,6 +537,11 @@ def main():
         log("small 64^3 ...")
         g1, s1, r1, line["small_64"] = leg_small(hdm, torch, 200)
         extra.append(("small_64", g1, s1, r1))
+    if not args.no_extra and not args.no_cube:
+        log("short-HD 512^3 ...")
+        g4, s4, r4, line["ct_512cube"] = leg_ct_cube(hdm, torch, args.steps, args.warmup)
+        line["ct_512cube"]["roofline"]["attainable_copy_gbs"] = copy_gbs
+        extra.append(("ct_512cube", g4, s4, r4))
     if not args.no_extra:
         log("long-iteration 512^3 ...")
         g3, s3, r3, line["long_512"] = leg_long(hdm, torch, args.long_reps)

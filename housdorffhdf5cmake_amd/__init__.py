@@ -283,13 +283,18 @@ def make_opts(robustness=1.0, tile=TILE_AUTO, grid_ctas=0, chunk=0, crop=True, s
 
 def last_timings() -> dict:
     """Device ms of the phases of the last call made with timing=True; "fused"
-    is True when the call ran as one launch (phases then come from the
-    kernel's own clock, and timing=3 fills only "total")."""
+    is True when the call ran the fused single-pair path (phases then come
+    from the kernel's own clock, and timing=3 fills only "total"), "launches"
+    counts its launches (1, or 2 when the pair was handed to the region
+    launch) and "steps" the dilation steps per grid barrier of the body that
+    finished (8 for the near pass and the region launch)."""
     ms = (C.c_float * 7)()
     n = lib().hdm_last_timings(ms, 7)
     names = ("pack", "meta", "dilate", "finalize", "total")
     out = {names[i]: float(ms[i]) for i in range(min(n, 5))}
-    out["fused"] = n > 5 and ms[5] == 1.0
+    out["fused"] = n > 5 and ms[5] >= 1.0
+    # launches of the fused path: 2 when the first handed the pair to the region launch
+    out["launches"] = int(ms[5]) if n > 5 else 0
     out["steps"] = int(ms[6]) if n > 6 else 1  # dilation steps per grid barrier
     return out
 

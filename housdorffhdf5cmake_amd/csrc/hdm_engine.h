@@ -199,6 +199,14 @@ struct Engine {
   // behind it: word 0 tag(epoch base, 2), words 1 and 2 its clock stamps of
   // the start and the end of the pack
   unsigned long long handoff[3];
+  // near pass of the single-pair launch: hits per (pass, step), index m + 2k,
+  // spread over kHitSlots by workgroup; arrivals per group of workgroups
+  // (blockIdx % kHitSlots) and of groups ([kHitSlots]), so the last workgroup
+  // to arrive folds the counts; workgroup 0's clock stamps of its start, the
+  // end of its pack and the pack barrier, for the engine head
+  unsigned long long near_hits[2 * kTbSteps][kHitSlots];
+  uint32_t near_done[kHitSlots + 1];
+  unsigned long long near_phase[3];
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

@@ -31,8 +31,14 @@ struct Ctx {
   int occ_fused[kLaunchIds] = {};  // resident fused single-pair workgroups per CU
   int occ_fused_ds[kLaunchIds] = {};  // the same, two dilation steps per barrier
   int occ_fused_tb[kLaunchIds] = {};  // the same, regions of eight steps per barrier
+  int occ_fused_near[kLaunchIds] = {};  // the same, pack and near pass
   double clock_khz = 0;               // wall_clock64() rate
   DevMem ws, in, map;
+  // region mark bitmaps of the near launch, two sets used by turns
+  // (marks_set: the one the next near launch fills; it zeroes the other);
+  // cleared whenever the buffer is allocated
+  DevMem marks;
+  int marks_set = 0;
   void *host_out = nullptr;  // pinned, 4 KiB
   hipEvent_t ev[5] = {};     // phase boundaries of a timed call
   // pack, seed, dilation, 0, all; 1 when the call was one fused launch;
@@ -113,6 +119,14 @@ bool ds_enabled();
 // the two-step fused single-pair launch (64x8x8 tiles, 6 element) is
 // followed by the region launch, to which it hands long pairs, unless HDM_TB=0
 bool tb_enabled();
+// a single pair on 64x8x8 tiles with the 6 element and no first-hit map runs
+// the near launch first (pack, then eight steps around the regions holding
+// voxels to be covered), which hands pairs with HD > 8 to the region launch,
+// unless HDM_NEAR=0 (then the two-step launch runs as before)
+bool near_enabled();
+// regions beyond which a pair skips the near launch: every workgroup of it
+// reads the mark bits of every region (16 words per thread at most)
+constexpr uint32_t kNearMaxRegions = 1u << 16;
 // DilateArgs::tb_mode of the two-step launch: 0, 1 (by the HD bound) or 2
 // (every pair), from HDM_TB
 int tb_mode();

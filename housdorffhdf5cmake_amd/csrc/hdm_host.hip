@@ -121,6 +121,12 @@ int get_ctx(Ctx **out) {
          nb < 1))
       rc = HDM_E_HIP;
     c->occ_fused_tb[k] = tb_fn ? nb : 0;
+    const void *near_fn = hdm_tile_launch(k)->fused_near_fn;
+    if (rc == HDM_OK && near_fn &&
+        (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, near_fn, kWgThreads, 0) != hipSuccess ||
+         nb < 1))
+      rc = HDM_E_HIP;
+    c->occ_fused_near[k] = near_fn ? nb : 0;
   }
   // the dilation kernel stores the result and the engine head here directly:
   // coherent (fine-grained) host memory, so the stores are seen by the host
@@ -214,6 +220,13 @@ static int tb_setting() {
 }
 bool tb_enabled() { return tb_setting() != 0; }
 int tb_mode() { return tb_setting(); }
+bool near_enabled() {
+  static const bool on = [] {
+    const char *e = std::getenv("HDM_NEAR");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
 
 // Default grid of the fused launch: one workgroup per two tiles, at least
 // 64, at most every resident one.  A small volume's iterations queue few
@@ -468,12 +481,24 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const CommonLayout L = common_layout(1, qcap, g.hist_len);
   const uint64_t nvox = (uint64_t)nx * ny * nz;
   const size_t esz = dtype == HDM_DTYPE_U8 ? 1 : 4;
-  const bool grow = L.total + g.total > c->ws.cap ||
+  // the near launch (section of the fused path below) and its region marks
+  const bool near_path = fused_enabled() && conn == 6 && tl->fused_near && ds_enabled() &&
+                         tb_enabled() && near_enabled() && !map_out &&
+                         g.nregions <= kNearMaxRegions;
+  // two bitmap sets of 2 * regions bits each
+  const size_t marks_bytes = near_path ? (size_t)2 * 4 * ((2 * (size_t)g.nregions + 31) / 32) : 0;
+  const bool grow = L.total + g.total > c->ws.cap || marks_bytes > c->marks.cap ||
                     (opts.ptr_kind == HDM_PTR_HOST && 2 * align256(nvox * esz) > c->in.cap) ||
                     (map_out && opts.ptr_kind == HDM_PTR_HOST && nvox * 2 > c->map.cap) ||
                     (opts.timing == 2 && (size_t)g.hist_len * 64 > c->trace.cap);
   if (grow && (rc = c->settle()) != HDM_OK) return rc;
   if ((rc = c->ws.ensure(L.total + g.total)) != HDM_OK) return rc;
+  if (marks_bytes > c->marks.cap) {
+    if ((rc = c->marks.ensure(marks_bytes)) != HDM_OK) return rc;
+    // fresh memory: both sets start empty
+    if (hipMemsetAsync(c->marks.p, 0, c->marks.cap, s) != hipSuccess) return HDM_E_HIP;
+    c->marks_set = 0;
+  }
   char *base = (char *)c->ws.p;
   GroupPtrs P = carve_common(base, L);
   PairDesc d{};
@@ -546,12 +571,15 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   const uint32_t ebase = c->ebase;
   bool ds = false;  // two dilation steps per barrier
   bool tb = false;  // then the region launch (eight steps per barrier)
+  bool handed = false;  // the first launch handed the pair to the region launch
   if (fused) {
     // one launch: reset, pack, seed and dilation.  Its barrier tags use
     // epochs above every earlier launch's on this workspace; a new workspace
     // or a restarting base gets its engine cleared first.
     if (c->ebase_ws != c->ws.p || ebase > (1u << 23)) {
       if (hipMemsetAsync(P.engine, 0, sizeof(Engine), s) != hipSuccess) return HDM_E_HIP;
+      if (c->marks.p && hipMemsetAsync(c->marks.p, 0, c->marks.cap, s) != hipSuccess)
+        return HDM_E_HIP;
       c->ebase = 0;
       c->ebase_ws = c->ws.p;
     }
@@ -583,17 +611,30 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     ds = tl->fused_ds && conn == 6 && ds_enabled();
     tb = ds && tl->fused_tb && tb_enabled();
     a.tb_mode = tb ? tb_mode() : 0;
+    if (near_path) {  // fills one set, zeroes the other for the next near launch
+      const uint32_t set_words = (uint32_t)(c->marks.cap / 4 / 2);
+      a.marks = (uint32_t *)c->marks.p + (size_t)c->marks_set * set_words;
+      a.marks_zero = (uint32_t *)c->marks.p + (size_t)(c->marks_set ^ 1) * set_words;
+      a.marks_zero_words = set_words;
+    }
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {
-      e = (ds ? tl->fused_ds : tl->fused)(dim3(fused_grid(c, tile, opts.grid_ctas, ds, g.ntiles)),
-                                          s, a);
+      if (near_path) {
+        e = tl->fused_near(dim3(fused_grid_of(c->occ_fused_near[tile], c->cus, opts.grid_ctas,
+                                              g.ntiles)),
+                           s, a);
+        if (e == hipSuccess) c->marks_set ^= 1;
+      } else
+        e = (ds ? tl->fused_ds : tl->fused)(
+            dim3(fused_grid(c, tile, opts.grid_ctas, ds, g.ntiles)), s, a);
       if (e == hipSuccess) c->acc_set ^= 1;  // the launch zeroes the other set first
     }
     if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
     if (e == hipSuccess && tb) {
-      // the two-step launch either computes the pair or, after its pack,
-      // hands it over (status 2): then the region launch follows on the
-      // stream, and the end event is recorded again behind it
+      // the near (or two-step) launch either computes the pair or, after its
+      // pack (or its near pass), hands it over (status 2): then the region
+      // launch follows on the stream, and the end event is recorded again
+      // behind it
       if (by_marker) {
         if ((rc = wait_ready(s, ready, c->seq)) != HDM_OK) return rc;
       } else if ((e = hipStreamSynchronize(s)) != hipSuccess) {
@@ -601,6 +642,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
       }
       if (__atomic_load_n(&hout->status, __ATOMIC_ACQUIRE) == 2) {
         hout->status = 1;
+        handed = true;
         if (by_marker) {
           if (++c->seq == 0) ++c->seq;
           a.ready_seq = c->seq;
@@ -644,7 +686,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
   c->last_ms[6] = (float)std::max(1, o.steps);
   if (fused) {
     c->ebase = c->ebase - 8 - (uint32_t)g.hist_len + 4 + es.t;  // the epochs it used
-    c->last_ms[5] = 1.f;
+    c->last_ms[5] = handed ? 2.f : 1.f;  // launches of the fused path
     if (opts.timing && opts.timing != 3) {
       float all = -1.f;
       if (hipEventElapsedTime(&all, c->ev[0], c->ev[3]) != hipSuccess) all = -1.f;

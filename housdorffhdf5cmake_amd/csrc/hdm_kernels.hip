@@ -399,15 +399,29 @@ __device__ __forceinline__ void put(T *p, T v) {
     *p = v;
 }
 
+// Where the pack of the near launch marks the regions of 1 x 2 x 2 tiles that
+// hold voxels to be covered: bit m * nreg + rg of a bitmap that starts out
+// zero (pass m = 0: FP = segm and not gold, the voxels gold's dilation
+// covers; 1: FN), set by an atomic OR.
+struct NearMarks {
+  uint32_t *w;    // ceil(2 * nreg / 32) words
+  uint32_t nreg;  // regions of the pair
+  int rgx, rgy;   // region grid in x and y
+};
+
 // The pack pass over all tiles of the launch's pairs, spread over the grid's
 // waves.  COH: the same launch reads the results in later phases (the fused
 // single-pair launch), so every store is written through; the first-hit map
-// is then cleared by the host instead.
-template <int TXB, int TY, int TZ, bool COH>
+// is then cleared by the host instead.  MARK (64x8x8 tiles, one pair): a tile
+// with FP or FN voxels marks its region for the near pass (NearMarks).
+template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
-                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr) {
+                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr,
+                                           const NearMarks *nm = nullptr) {
+  static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8 && COH),
+                "near marks: 64x8x8 tiles, one pair");
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -421,8 +435,13 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
   constexpr uint64_t kStep = (uint64_t)TPW * U;
-  const uint64_t per = ((total_tiles + nw - 1) / nw + kStep - 1) / kStep * kStep;
-  const uint64_t w0 = gw * per, w1 = min(total_tiles, w0 + per);
+  // the wave steps split evenly over every wave of the grid (ranges differ by
+  // at most one step), so each CU packs the same share whatever waves it
+  // holds; ranges start on a step, so the 128-byte path's tile pairs stay
+  // x-adjacent
+  const uint64_t nsteps = (total_tiles + kStep - 1) / kStep;
+  const uint64_t w0 = min(total_tiles, gw * nsteps / nw * kStep);
+  const uint64_t w1 = min(total_tiles, (gw + 1) * nsteps / nw * kStep);
   const int slot = (int)(gw % kSumSlots);
   PackAcc acc;
   acc_clear(acc);
@@ -606,6 +625,16 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       // counts of hello.cu:624-625 plus the non-empty tile box
       const uint32_t c0 = popc_row(a), c1 = popc_row(b);
       const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
+      if constexpr (MARK) {  // one tile per wave: the ballots cover the tile
+        const bool fp = __ballot(c2 != 0) != 0, fn = __ballot(c3 != 0) != 0;
+        if (in && r == 0 && (fp || fn)) {
+          const uint32_t rg = (uint32_t)txU[u] +
+                              (uint32_t)nm->rgx * ((uint32_t)(tyU[u] >> 1) +
+                                                   (uint32_t)nm->rgy * (uint32_t)(tzU[u] >> 1));
+          if (fp) or_g(nm->w + (rg >> 5), 1u << (rg & 31));
+          if (fn) or_g(nm->w + ((nm->nreg + rg) >> 5), 1u << ((nm->nreg + rg) & 31));
+        }
+      }
       const bool ne = (a | b) != 0;
       if (p == ap) {
         acc.c[0] += c0, acc.c[1] += c1, acc.c[2] += c2, acc.c[3] += c3;
@@ -1525,11 +1554,15 @@ __device__ __forceinline__ void hit_flush(HitTable &h, PairCtr *ctrs) {
 // read, not rescanned.
 // steps: dilation iterations per engine iteration (2 when each barrier
 // covers two steps)
-__device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t_engine,
-                                 bool error, int steps = 1) {
+// The same from explicit values: totals, and per pass the covered count, the
+// sum of first hits, the last hit and the stop iteration.
+__device__ PairOut finalize_from(const PairDesc &d, unsigned long long n_gold,
+                                 unsigned long long n_segm, unsigned long long n_fp,
+                                 unsigned long long n_fn, const unsigned long long *cov,
+                                 const unsigned long long *sum_d, const uint32_t *hd_last,
+                                 const int *done_from, int status, uint32_t t_engine, bool error,
+                                 int steps) {
   PairOut o;
-  const unsigned long long n_fp = ld_sc(&c.n_fp), n_fn = ld_sc(&c.n_fn);
-  const unsigned long long n_gold = ld_sc(&c.n_gold), n_segm = ld_sc(&c.n_segm);
   const int tl = (int)(t_engine - d.t_start) * steps;  // pair-local iterations so far
   bool done = true, reached = true;
   unsigned long long sum[2], cnt[2];
@@ -1537,10 +1570,10 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
 #pragma unroll
   for (int m = 0; m < 2; ++m) {
     const unsigned long long cut = robust_cut(d.robust, m == 0 ? n_fp : n_fn);
-    sum[m] = cut ? ld_sc(&c.sum_d[m]) : 0ull;
-    cnt[m] = cut ? ld_sc(&c.cov[m]) : 0ull;
-    hd[m] = cut ? (int)ld_sc(&c.hd_last[m]) : 0;
-    const int df = ld_sc(&c.done_from[m]);
+    sum[m] = cut ? sum_d[m] : 0ull;
+    cnt[m] = cut ? cov[m] : 0ull;
+    hd[m] = cut ? (int)hd_last[m] : 0;
+    const int df = done_from[m];
     done = done && (cut == 0 || df <= tl);
     if (cut) last = max(last, df);
     reached = reached && cnt[m] >= cut;
@@ -1553,7 +1586,7 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
   o.sum_fp = sum[0], o.cnt_fp = cnt[0], o.sum_fn = sum[1], o.cnt_fn = cnt[1];
   o.n_gold = n_gold, o.n_segm = n_segm, o.n_fp = n_fp, o.n_fn = n_fn;
   o.done = done ? 1 : 0;
-  int st = ld_sc(&c.status);
+  int st = status;
   if ((n_fp && !n_gold) || (n_fn && !n_segm))
     st = -2;  // HDM_E_EMPTY_MASK
   else if (error)
@@ -1562,6 +1595,21 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
     st = -8;  // HDM_E_STUCK
   o.status = st;
   return o;
+}
+__device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t_engine,
+                                 bool error, int steps = 1) {
+  unsigned long long cov[2], sum[2];
+  uint32_t hd[2];
+  int df[2];
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    cov[m] = ld_sc(&c.cov[m]);
+    sum[m] = ld_sc(&c.sum_d[m]);
+    hd[m] = ld_sc(&c.hd_last[m]);
+    df[m] = ld_sc(&c.done_from[m]);
+  }
+  return finalize_from(d, ld_sc(&c.n_gold), ld_sc(&c.n_segm), ld_sc(&c.n_fp), ld_sc(&c.n_fn), cov,
+                       sum, hd, df, ld_sc(&c.status), t_engine, error, steps);
 }
 
 // A lower bound of the HD in dilation steps (6 element, 64x8x8 tiles) from
@@ -3462,6 +3510,444 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- near pass
+// The single-pair launch for pairs whose voxels lie near the other mask
+// (dilate_near_kernel; 64x8x8 tiles, 6 element).  It packs the pair as the
+// two-step launch does, and the pack marks, in a bitmap, every region of
+// 1 x 2 x 2 tiles that holds voxels to be covered (FP for gold's dilation, FN
+// for segm's).  After the one grid barrier that ends the pack, each workgroup
+// reads the bitmap and takes an equal share of the marked (region, mask)
+// entries -- every workgroup counts the marks itself, so no queue, counter or
+// second barrier is needed -- and for each one dilates the *other* mask's
+// original bits eight steps in the region's 32 x 32-row window (the region
+// plus an eight-voxel halo, rows of 80 bits; the same window and step as the
+// region launch).  After step k the region's own rows are exact, so the new
+// hits res & ~src & ref of step k (hello.cu:1590-1626) are exactly the
+// region's voxels at first-hit distance k: the per-step counts of all entries
+// are the histogram bins 1 .. 8.  An entry stops stepping once every voxel it
+// has to cover is covered.  Nothing is written back and no neighbour is
+// claimed, because no later step reads the dilated bits.  Each workgroup then
+// adds its counts per (pass, step) into spread slots and counts itself in;
+// the one that arrives last folds the bins in order with the robust cut
+// (hello.cu:1748-1759) and writes the results.  A pass whose cut is not
+// reached within eight steps (HD > 8) hands the packed pair to the region
+// launch (status 2), which computes it from the start; so does a pair whose
+// HD bound from the tile boxes already exceeds eight steps, right after the
+// pack.
+constexpr int kNearSteps = kTbSteps;
+// bitmap words of the region marks a thread of the share count holds at most
+// (256 threads: up to 2^16 regions of two passes)
+constexpr uint32_t kNearWordsPerThread = 16;
+
+__global__ __launch_bounds__(kWgThreads)
+__attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
+  constexpr int KS = kNearSteps;
+  constexpr int R = 64;  // rows per tile
+  static_assert(KS == 8, "the window holds an eight-voxel halo");
+  static_assert(2 * KS * kHitSlots == kWgThreads, "one hit slot per thread in the fold");
+  PairCtr *ctrs = a.ctrs;
+  Engine *es = a.es;
+  const uint64_t spin_limit = a.spin;
+  const PairDesc &d = a.desc;
+  PairCtr &c0 = ctrs[0];
+  const uint32_t qcap = a.qcap;
+
+  __shared__ u64x2 s_win[2][8][2][kTbWin];
+  __shared__ int s_ok;
+  __shared__ unsigned long long s_tot[4];
+  __shared__ int s_hand;                  // 1: the pair goes to the region launch
+  __shared__ uint32_t s_hits[2 * KS];     // the workgroup's hits, index m + 2k
+  __shared__ uint32_t s_hs[KS];           // the running entry's hits per step
+  __shared__ uint32_t s_need;             // the running entry's voxels to cover
+  __shared__ uint32_t s_list[kWgThreads]; // mark bit indices of the entries taken
+  __shared__ uint32_t s_wsum[kWavesPerWg];
+  __shared__ int s_last;
+  __shared__ unsigned long long s_h[2 * KS];
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // window mapping of the region launch: thread t holds rows (yy, zz0 .. zz0 + 3)
+  const int yy = threadIdx.x & 31;
+  const int seg = threadIdx.x >> 5;
+  const int zz0 = 4 * seg;
+  const int tyi = yy / 8 - 1, tzi = seg / 2 - 1;  // tile offsets in the region, -1 .. 2
+  const bool core = yy >= 8 && yy < 24 && seg >= 2 && seg < 6;
+  const int rgx = d.gx, rgy = (d.gy + 1) / 2, rgz = (d.gz + 1) / 2;
+  const uint32_t nreg = (uint32_t)rgx * (uint32_t)rgy * (uint32_t)rgz;
+
+  if (threadIdx.x < 2 * KS) s_hits[threadIdx.x] = 0, s_h[threadIdx.x] = 0;
+  if (threadIdx.x < KS) s_hs[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_need = 0;
+  uint64_t ph0 = 0, ph1 = 0;  // workgroup 0: clock at its start and at the end of its pack
+  if (blockIdx.x == 0) ph0 = wall_clock64();
+  // trace (timing 2): one record of 8 words per workgroup -- [0] start, [1]
+  // pack done, [2] past the pack barrier, [3] share counted, [4] entries
+  // done, [5] entries taken -- and one more for the fold of the workgroup
+  // that arrives last ([0] fold start, [1] results written), where the trace
+  // holds them
+  uint64_t *trw = (a.trace && blockIdx.x + 1 < (uint32_t)a.qstats_len)
+                      ? a.trace + 8 * (uint64_t)blockIdx.x : nullptr;
+  if (trw && threadIdx.x == 0) trw[0] = wall_clock64();
+  PackSums *acc = &es->acc[a.acc_set & 1];
+  // R: workgroup 0 zeroes the pair's counters and histograms, the engine's
+  // state, the near pass's hit slots and arrival counters, and the pack
+  // partials the next launch packs into (as the two-step launch does); every
+  // workgroup zeroes its share of the region states a hand-off to the region
+  // launch starts from and of the mark bitmap the next near launch fills.
+  // Nothing here is read before the barrier that ends the pack, except by
+  // the workgroup that wrote it.
+  if (blockIdx.x == 0) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+      st_sc(&c0.n_gold, 0ull), st_sc(&c0.n_segm, 0ull);
+      st_sc(&c0.n_fp, 0ull), st_sc(&c0.n_fn, 0ull);
+      for (int m = 0; m < 2; ++m) {
+        st_sc(&c0.cov[m], 0ull), st_sc(&c0.sum_d[m], 0ull);
+        st_sc(&c0.done_from[m], 0x7FFFFFFF), st_sc(&c0.hd_last[m], 0u);
+      }
+      st_sc(&c0.status, 0);
+      st_sc(&es->t, 0u), st_sc(&es->error, 0u), st_sc(&es->qmax, 0u);
+      st_sc(&es->qtotal, 0ull), st_sc(&es->qcap, qcap), st_sc(&es->qnext, 0u);
+    }
+    PackSums *next = &es->acc[(a.acc_set & 1) ^ 1];
+    for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&next->sums[0][0] + i, 0ull);
+    for (int i = tid; i < 18 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
+    for (int i = tid; i < d.hist_len; i += blockDim.x)
+      st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
+    for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
+    st_sc(&es->near_hits[0][0] + tid, 0ull);
+    if (tid <= kHitSlots) st_sc(&es->near_done[tid], 0u);
+  }
+  {
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, gn = gridDim.x * blockDim.x;
+    for (uint32_t i = gt; i < nreg; i += gn) st_sc(d.rstate[0] + i, 0u), st_sc(d.rstate[1] + i, 0u);
+    for (uint32_t i = gt; i < a.marks_zero_words; i += gn) st_sc(a.marks_zero + i, 0u);
+  }
+  // P: pack, every store written through, regions marked
+  {
+    __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
+    const NearMarks nm{a.marks, nreg, rgx, rgy};
+    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc, &nm);
+  }
+  if (blockIdx.x == 0) ph1 = wall_clock64();
+  if (trw && threadIdx.x == 0) trw[1] = wall_clock64();
+  bool ok = grid_sync_tagged(es, a.ebase + 2, spin_limit, &s_ok);
+  if (trw && threadIdx.x == 0) trw[2] = wall_clock64();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st_sc(&es->near_phase[0], (unsigned long long)ph0);
+    st_sc(&es->near_phase[1], (unsigned long long)ph1);
+    st_sc(&es->near_phase[2], (unsigned long long)wall_clock64());
+  }
+  // The mark bitmap (agent-coherent: bits of several workgroups share a
+  // word), thread t words [c t, c t + c), loaded together with the pack
+  // partials: one round trip for both.
+  const uint32_t W = 2 * nreg;                  // mark bits
+  const uint32_t nw = (W + 31) / 32;            // mark words
+  const uint32_t cw = (nw + kWgThreads - 1) / kWgThreads;  // words per thread (<= 16)
+  uint32_t mw[kNearWordsPerThread];
+#pragma unroll
+  for (uint32_t q = 0; q < kNearWordsPerThread; ++q) {
+    const uint32_t wi = threadIdx.x * cw + q;
+    mw[q] = (ok && q < cw && wi < nw) ? ld_sc(a.marks + wi) : 0u;
+  }
+  // totals, box and the HD bound from the pack partials
+  if (threadIdx.x < 64) {
+    unsigned long long v[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) v[f] = wave_sum64(ld_sc(&acc->sums[f][lane]));
+    int bx[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+      const int x = wave_max(ld_sc(&acc->box[k][lane]), lane);
+      bx[k] = k % 6 < 3 ? box_dec_lo(x) : box_dec_hi(x);
+    }
+    const bool hand = a.tb_mode == 2 || hd_lower_bound(bx + 6, bx + 12) > KS;
+    if (lane == 0) {
+#pragma unroll
+      for (int f = 0; f < 4; ++f) s_tot[f] = v[f];
+      s_hand = hand ? 1 : 0;
+      if (blockIdx.x == 0 && ok) {
+        st_sc(&c0.n_gold, v[0]), st_sc(&c0.n_segm, v[1]);
+        st_sc(&c0.n_fp, v[2]), st_sc(&c0.n_fn, v[3]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) st_sc(&c0.lo[k], bx[k]), st_sc(&c0.hi[k], bx[3 + k]);
+      }
+    }
+  }
+  __syncthreads();
+  // The results (the workgroup that folds) or, for a hand-off, the tagged
+  // hand-off word with the clock stamps of the start and the pack and, in
+  // place of the results, status 2; then the ready marker, after both.  The
+  // host queues the region launch on status 2.
+  auto publish = [&](bool hand, bool err, uint32_t nent, uint64_t *trm, const PairOut *res) {
+    if (hand) {
+      st_sc(&es->handoff[1], ld_sc(&es->near_phase[0]));
+      st_sc(&es->handoff[2], ld_sc(&es->near_phase[1]));
+      st_sc(&es->handoff[0], tag(a.ebase, 2ull));
+      __hip_atomic_store(&a.outs[0].status, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    } else {
+      if (a.qstats && a.qstats_len > 0) a.qstats[0] = nent;
+      a.outs[0] = res ? *res : finalize_pair(d, c0, 1, err, KS);
+      st_sc(&es->t, 0u);
+      st_sc(&es->qnext, 0u);
+      st_sc(&es->qtotal, (unsigned long long)nent), st_sc(&es->qmax, nent);
+      if (trm) trm[1] = wall_clock64();
+      if (a.head) {
+        EngineHead hh;
+        // with a trace, t counts its records (one per workgroup and the
+        // fold's), which the host copies out; the next launch's epochs start
+        // past it
+        hh.t = trm ? gridDim.x + 1 : 0;
+        hh.error = err ? 1u : 0u;
+        hh.qmax = nent;
+        hh.nwg = gridDim.x;
+        hh.qtotal = nent;
+        hh.qcap = qcap;
+        hh.qnext = 0;
+        hh.phase[0] = ld_sc(&es->near_phase[0]);
+        hh.phase[1] = ld_sc(&es->near_phase[1]);
+        hh.phase[2] = ld_sc(&es->near_phase[2]);
+        hh.phase[3] = wall_clock64();
+        *a.head = hh;
+      }
+    }
+    if (a.ready) {
+      __threadfence_system();
+      __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  };
+  if (s_hand) {  // every workgroup takes the same decision from the same partials
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (ok)
+        publish(true, false, 0, nullptr, nullptr);
+      else
+        publish(false, true, 0, nullptr, nullptr);  // the pack barrier timed out
+    }
+    return;
+  }
+
+  // One entry: mark bit wi = region rg of pass m.
+  auto near_entry = [&](uint32_t wi) {
+    const int m = wi >= nreg ? 1 : 0;
+    const uint32_t rg = wi - (m ? nreg : 0u);
+    const int rx = (int)(rg % (uint32_t)rgx);
+    const int ry = (int)((rg / (uint32_t)rgx) % (uint32_t)rgy);
+    const int rz = (int)(rg / ((uint32_t)rgx * (uint32_t)rgy));
+    // pass m dilates mask m and covers the other mask's voxels
+    const uint64_t *orig_m = (const uint64_t *)(m ? d.orig[1] : d.orig[0]);
+    const uint64_t *orig_o = (const uint64_t *)(m ? d.orig[0] : d.orig[1]);
+    const int ty = 2 * ry + tyi, tz = 2 * rz + tzi;
+    const int yl = yy & 7;
+    const bool tyz = ty >= 0 && ty < d.gy && tz >= 0 && tz < d.gz;
+    const bool yok = tyz && ty * 8 + yl < d.ny;
+    bool tin[3];
+    uint64_t vb[3];
+    uint32_t tix[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int ux = rx + c - 1;
+      tin[c] = tyz && ux >= 0 && ux < d.gx;
+      vb[c] = tin[c] ? valid_bits<uint64_t>(ux, d.nx) : 0ull;
+      tix[c] = tin[c] ? (uint32_t)ux + (uint32_t)d.gx * ((uint32_t)ty + (uint32_t)d.gy * (uint32_t)tz) : 0u;
+    }
+    // every load at once, at a valid address (tile 0 stands in for a tile
+    // outside the grid), the stand-ins dropped afterwards; the packed masks
+    // were written through by the pack and no workgroup read them before
+    // the barrier, so plain loads see them
+    bool rok[4];
+    uint64_t w[4][3], oth[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int zl = (zz0 + i) & 7;
+      rok[i] = yok && tz * 8 + zl < d.nz;
+      const uint32_t r = (uint32_t)(yl + 8 * zl);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) w[i][c] = ldg(orig_m + (uint64_t)tix[c] * R + r);
+      oth[i] = ldg(orig_o + (uint64_t)tix[1] * R + r);
+    }
+    TbRow X[4], V[4];
+    uint32_t need = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        if (!(rok[i] && tin[c])) w[i][c] = 0;
+      V[i] = rok[i] ? tb_make(vb[0], vb[1], vb[2]) : TbRow{0, 0};
+      X[i] = tb_make(w[i][0], w[i][1], w[i][2]);
+      X[i].o &= V[i].o, X[i].h &= V[i].h;
+      // the voxels to cover: the other mask's, not in the source (its own
+      // rows only; the window's ring belongs to other entries)
+      oth[i] = (core && rok[i] && tin[1]) ? oth[i] & ~X[i].o : 0ull;
+      need += (uint32_t)__popcll(oth[i]);
+    }
+    need = (uint32_t)wave_reduce32((int)need, 0, [](int x, int y) {
+      return (int)((uint32_t)x + (uint32_t)y);
+    });
+    if (lane == 0 && need) atomicAdd(&s_need, need);
+    uint32_t covered = 0, want = 0;
+#pragma unroll 1
+    for (int k = 0; k < KS; ++k) {
+      // rows z-1 / z+1 beyond the thread's four come through LDS; at the
+      // window's edges the neighbour read is some other row, and a wrong
+      // voxel there moves inward one row per step, so after eight steps it
+      // is still one row short of the region's own rows
+      s_win[k & 1][seg][0][yy] = u64x2{X[0].o, X[0].h};
+      s_win[k & 1][seg][1][yy] = u64x2{X[3].o, X[3].h};
+      __syncthreads();
+      // every count read here was added before this barrier, so the whole
+      // workgroup takes the same decision
+      if (k == 0)
+        want = s_need;
+      else
+        covered += s_hs[k - 1];
+      if (covered >= want) break;  // every voxel of the entry is covered
+      const u64x2 ezm = s_win[k & 1][seg > 0 ? seg - 1 : 0][1][yy];
+      const u64x2 ezp = s_win[k & 1][seg < 7 ? seg + 1 : 7][0][yy];
+      TbRow N[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const TbRow ym = tb_lane_nb<true>(X[i]), yp = tb_lane_nb<false>(X[i]);
+        const TbRow zm = i > 0 ? X[i - 1] : TbRow{ezm.x, (uint32_t)ezm.y};
+        const TbRow zp = i < 3 ? X[i + 1] : TbRow{ezp.x, (uint32_t)ezp.y};
+        const TbRow xd = tb_xdil(X[i]);
+        N[i] = TbRow{(xd.o | ym.o | yp.o | zm.o | zp.o) & V[i].o,
+                     (xd.h | ym.h | yp.h | zm.h | zp.h) & V[i].h};
+      }
+      // new hits of step k + 1: res & ~src & ref (hello.cu:1590-1626)
+      uint32_t c = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) c += (uint32_t)__popcll(N[i].o & ~X[i].o & oth[i]);
+      c = (uint32_t)wave_reduce32((int)c, 0, [](int x, int y) {
+        return (int)((uint32_t)x + (uint32_t)y);
+      });
+      if (lane == 0 && c) atomicAdd(&s_hs[k], c);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) X[i] = N[i];
+    }
+    __syncthreads();  // every count of the entry is in s_hs
+    if (threadIdx.x < KS) {
+      s_hits[m + 2 * threadIdx.x] += s_hs[threadIdx.x];
+      s_hs[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) s_need = 0;
+    __syncthreads();  // cleared before the next entry adds to them
+  };
+
+  // The workgroup's share of the entries: every workgroup counts the marks
+  // itself and takes entries [T b / G, T (b + 1) / G) of the T marked, in bit
+  // order.
+  uint32_t mine = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kNearWordsPerThread; ++q) {
+    const uint32_t b0 = (threadIdx.x * cw + q) * 32;  // bits past W never get set, but drop them
+    if (b0 >= W)
+      mw[q] = 0;
+    else if (W - b0 < 32)
+      mw[q] &= (1u << (W - b0)) - 1u;
+    mine += (uint32_t)__popc(mw[q]);
+  }
+  uint32_t before = 0, total = 0;
+  {
+    const uint32_t incl = wave_scan32(mine);
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    before = incl - mine;
+#pragma unroll
+    for (int k = 0; k < kWavesPerWg; ++k) {
+      if (k < wave) before += s_wsum[k];
+      total += s_wsum[k];
+    }
+  }
+  const uint32_t sb = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
+  const uint32_t se = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
+  if (trw && threadIdx.x == 0) trw[3] = wall_clock64(), trw[5] = se - sb;
+  for (uint32_t cs = sb; cs < se; cs += kWgThreads) {
+    const uint32_t ce = min(se, cs + kWgThreads);
+    // the entries [cs, ce) into s_list
+    if (before < ce && before + mine > cs) {
+      uint32_t g = before;
+#pragma unroll
+      for (uint32_t q = 0; q < kNearWordsPerThread; ++q) {
+        uint32_t bits = mw[q];
+        while (bits) {
+          const int j = __builtin_ctz(bits);
+          bits &= bits - 1;
+          if (g >= cs && g < ce) s_list[g - cs] = (threadIdx.x * cw + q) * 32 + (uint32_t)j;
+          ++g;
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < ce - cs; ++j) near_entry(s_list[j]);
+  }
+  if (trw && threadIdx.x == 0) trw[4] = wall_clock64();
+  // Arrival: the workgroup adds its hits into its slots and counts itself in
+  // its group; the last of a group counts the group in, and the workgroup
+  // that completes the last group folds.  Every workgroup's adds are done
+  // (s_waitcnt 0) before its count, so the one that folds reads them all.
+  const uint32_t grp = blockIdx.x % kHitSlots;
+  if (threadIdx.x < 2 * KS && s_hits[threadIdx.x])
+    atomicAdd(&es->near_hits[threadIdx.x][grp], (unsigned long long)s_hits[threadIdx.x]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t members = (gridDim.x - grp + kHitSlots - 1) / kHitSlots;
+    const uint32_t groups = min(gridDim.x, (uint32_t)kHitSlots);
+    int last = 0;
+    if (atomicAdd(&es->near_done[grp], 1u) == members - 1)
+      last = atomicAdd(&es->near_done[kHitSlots], 1u) == groups - 1;
+    s_last = last;
+  }
+  __syncthreads();
+  if (!s_last) return;
+
+  // the fold, by the workgroup that arrived last
+  uint64_t *trm = (a.trace && gridDim.x + 1 < (uint32_t)a.qstats_len)
+                      ? a.trace + 8 * (uint64_t)gridDim.x : nullptr;
+  if (trm && threadIdx.x == 0) trm[0] = wall_clock64();
+  {
+    const unsigned long long v = ld_sc(&es->near_hits[0][0] + threadIdx.x);
+    if (v) atomicAdd(&s_h[threadIdx.x / kHitSlots], v);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const bool err = !ok || ld_sc(&es->error) != 0;
+  const unsigned long long nf[2] = {s_tot[2], s_tot[3]};
+  unsigned long long cut[2], cov[2] = {0, 0}, sum[2] = {0, 0};
+  uint32_t hd[2] = {0, 0}, taken[2] = {0, 0};
+  int dfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
+  bool done = true;
+  for (int m = 0; m < 2; ++m) {
+    cut[m] = robust_cut(d.robust, nf[m]);
+    for (int k = 0; k < KS; ++k) {
+      const int it = k + 1;
+      const unsigned long long hm = s_h[m + 2 * k];
+      if (cut[m] == 0 || dfrom[m] < it || hm == 0) continue;
+      taken[m] |= 1u << k;
+      cov[m] += hm;
+      sum[m] += hm * (unsigned long long)it;
+      hd[m] = (uint32_t)it;
+      if (cov[m] >= cut[m]) dfrom[m] = it;
+    }
+    done = done && (cut[m] == 0 || cov[m] >= cut[m]);
+  }
+  const bool empty = (s_tot[2] && !s_tot[0]) || (s_tot[3] && !s_tot[1]);
+  if (!done && !empty && !err) {  // HD beyond eight steps: the region launch computes it
+    publish(true, false, 0, nullptr, nullptr);
+    return;
+  }
+  // the histogram bins the fold took (hello.cu:1618 numbering), then the
+  // results from the fold's own registers
+  for (int m = 0; m < 2; ++m) {
+    uint32_t *hp = m ? d.hist[1] : d.hist[0];
+    for (int k = 0; k < KS; ++k)
+      if (((taken[m] >> k) & 1u) && k + 1 < d.hist_len) st_sc(hp + k + 1, (uint32_t)s_h[m + 2 * k]);
+  }
+  const PairOut o = finalize_from(d, s_tot[0], s_tot[1], s_tot[2], s_tot[3], cov, sum, hd, dfrom,
+                                  0, 1, err, KS);
+  publish(false, err, total, trm, &o);
+}
+
 // ---------------------------------------------------------------- launchers
 template <int TXB, int TY, int TZ>
 hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
@@ -3521,6 +4007,10 @@ hipError_t launch_fused_ds(dim3 grid, hipStream_t s, const DilateArgs &a) {
 hipError_t launch_fused_tb(dim3 grid, hipStream_t s, const DilateArgs &a) {
   return launch_persistent(dilate_tb_kernel, grid, s, a);
 }
+// pack and near pass (64x8x8, 6 element)
+hipError_t launch_fused_near(dim3 grid, hipStream_t s, const DilateArgs &a) {
+  return launch_persistent(dilate_near_kernel, grid, s, a);
+}
 
 // DS: every launch of the entry runs two dilation steps per engine iteration
 // (its fused launch is dilate_ds_kernel)
@@ -3534,12 +4024,14 @@ TileLaunch make_launch() {
                     DS ? &launch_fused_ds : &launch_fused<TXB, TY, TZ, CONN>,
                     kDS ? &launch_fused_ds : nullptr,
                     kDS ? &launch_fused_tb : nullptr,
+                    kDS ? &launch_fused_near : nullptr,
                     (const void *)&dilate_kernel<TXB, TY, TZ, false, false, CONN, DS>,
                     (const void *)&dilate_kernel<TXB, TY, TZ, true, false, CONN, DS>,
                     DS ? (const void *)&dilate_ds_kernel
                        : (const void *)&dilate_kernel<TXB, TY, TZ, true, true, CONN>,
                     kDS ? (const void *)&dilate_ds_kernel : nullptr,
                     kDS ? (const void *)&dilate_tb_kernel : nullptr,
+                    kDS ? (const void *)&dilate_near_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     &launch_combine<TXB, TY, TZ>,
                     TXB,

@@ -1,8 +1,8 @@
 // hdm_hausdorff_u8_noalloc: the single-pair entry point for callers that
 // cannot allocate (code under HIP graph capture, tight serving loops,
-// harnesses that forbid allocation).  It runs the same fused launch as
+// harnesses that forbid allocation).  It runs the same launches as
 // hdm_hausdorff_u8, but its whole device workspace -- descriptor, counters,
-// engine state, queue, packed masks, dilation buffers, results -- lives in
+// engine state, queue, packed masks, dilation buffers, region marks, results -- lives in
 // `scratch`, a file-scope device array of kScratchBytes, and the results come
 // back with a stream-ordered copy into the caller's hdm_result.  No call here
 // allocates: no hipMalloc / hipFree, no pinned host memory, no events, no
@@ -35,6 +35,11 @@ namespace {
 
 constexpr uint64_t kScratchBytes = 64ull << 20;  // HDM_NOALLOC_MAX_BYTES
 constexpr int kMaxDevices = 64;
+// the near launch's two region-mark bitmaps, at a fixed place (the end of the
+// array), so the set one call zeroes is the set the next call fills whatever
+// the pair sizes: 2 x kNearMaxRegions bits per set
+constexpr uint64_t kMarkSetWords = 2ull * kNearMaxRegions / 32;
+constexpr uint64_t kMarkBytes = 2 * kMarkSetWords * 4;
 
 }  // namespace
 
@@ -50,6 +55,8 @@ struct NoAllocDev {
   int occ_fused[kLaunchIds] = {};
   int occ_fused_ds[kLaunchIds] = {};
   int occ_fused_tb[kLaunchIds] = {};
+  int occ_fused_near[kLaunchIds] = {};
+  int marks_set = 0;   // the mark bitmap the next near launch fills
   double clock_khz = 0;
   uint32_t ebase = 0;  // epoch base of the next launch's barrier tags
   int acc_set = 0;     // the pack-partial set the next launch packs into
@@ -94,6 +101,13 @@ int setup(NoAllocDev &d, int dev) {
         return HDM_E_HIP;
       d.occ_fused_tb[k] = nb;
     }
+    if (tl->fused_near_fn) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, tl->fused_near_fn, kWgThreads, 0) !=
+              hipSuccess ||
+          nb < 1)
+        return HDM_E_HIP;
+      d.occ_fused_near[k] = nb;
+    }
   }
   d.ptr = (unsigned char *)p;
   d.cus = cus;
@@ -128,10 +142,11 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
   if ((uint64_t)g.ntiles * 2 > 0xFFFFFFFFull) return HDM_E_INVALID;
   const uint32_t qcap = 2u * g.ntiles;
   const CommonLayout L = common_layout(1, qcap, g.hist_len);
-  // [common][pair][PairOut][EngineHead]
+  // [common][pair][PairOut][EngineHead] ... [mark bitmaps at the end]
   const uint64_t off_out = align256(L.total + g.total);
   const uint64_t off_head = off_out + align256(sizeof(PairOut));
-  if (off_head + align256(sizeof(EngineHead)) > kScratchBytes) return HDM_E_CAPACITY;
+  const uint64_t off_marks = kScratchBytes - kMarkBytes;
+  if (off_head + align256(sizeof(EngineHead)) > off_marks) return HDM_E_CAPACITY;
   int dev = -1;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
     (void)hipGetLastError();
@@ -154,10 +169,13 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
   // the engine (at a fixed offset: the common layout of one pair starts with
   // fixed-size regions) is cleared on the first call and when the epoch base
   // restarts; tagged words of earlier calls never match later epochs
+  uint32_t *marks = (uint32_t *)(base + off_marks);
   if (!D.cleared || D.ebase > (1u << 23)) {
     if (hipMemsetAsync(P.engine, 0, sizeof(Engine), s) != hipSuccess) return HDM_E_HIP;
+    if (hipMemsetAsync(marks, 0, kMarkBytes, s) != hipSuccess) return HDM_E_HIP;
     D.ebase = 0;
     D.acc_set = 0;
+    D.marks_set = 0;
     D.cleared = true;
   }
   const uint64_t nvox = (uint64_t)nx * ny * nz;
@@ -189,22 +207,35 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
   a.acc_set = D.acc_set;
   a.coop = coop_enabled() ? 1 : 0;
   a.tb_active = D.cus;
-  // the launches hdm_hausdorff_u8 makes: two steps per barrier (else one),
-  // then the region launch, to which the first hands long pairs
+  // the launches hdm_hausdorff_u8 makes: the near launch (pack and eight
+  // steps around the regions with voxels to cover) or two steps per barrier
+  // (else one), then the region launch, to which the first hands long pairs
   const bool ds = tl->fused_ds && conn == 6 && ds_enabled();
   const bool tb = ds && tl->fused_tb && tb_enabled();
+  const bool near_path = tb && tl->fused_near && near_enabled() && !d.map &&
+                         g.nregions <= kNearMaxRegions;
   a.tb_mode = tb ? tb_mode() : 0;
-  const int grid = fused_grid_of(ds ? D.occ_fused_ds[tile] : D.occ_fused[tile], D.cus,
-                                 opts.grid_ctas, g.ntiles);
-  hipError_t e = (ds ? tl->fused_ds : tl->fused)(dim3(grid), s, a);
-  if (e == hipSuccess) D.acc_set ^= 1;  // the launch zeroes the other set for the next one
+  if (near_path) {
+    a.marks = marks + (size_t)D.marks_set * kMarkSetWords;
+    a.marks_zero = marks + (size_t)(D.marks_set ^ 1) * kMarkSetWords;
+    a.marks_zero_words = (uint32_t)kMarkSetWords;
+  }
+  const int grid = fused_grid_of(near_path ? D.occ_fused_near[tile]
+                                 : ds      ? D.occ_fused_ds[tile]
+                                           : D.occ_fused[tile],
+                                 D.cus, opts.grid_ctas, g.ntiles);
+  hipError_t e = (near_path ? tl->fused_near : ds ? tl->fused_ds : tl->fused)(dim3(grid), s, a);
+  if (e == hipSuccess) {
+    D.acc_set ^= 1;  // the launch zeroes the other set for the next one
+    if (near_path) D.marks_set ^= 1;
+  }
   // until the head is read, assume the launches used every epoch they could
   D.ebase = a.ebase + 8 + (uint32_t)g.hist_len;
   if (e != hipSuccess) return map_hip(e);
   PairOut o;
   EngineHead h;
   if (tb) {
-    // the two-step launch computes the pair or hands it over after its pack
+    // the near (or two-step) launch computes the pair or hands it over
     // (status 2): then the region launch follows
     if (hipMemcpyAsync(&o, dout, sizeof(o), hipMemcpyDeviceToHost, s) != hipSuccess ||
         (e = hipStreamSynchronize(s)) != hipSuccess)

@@ -58,6 +58,13 @@ struct DilateArgs {
   // whose HD lower bound reaches kTbMinSteps over after its pack (result
   // status 2; the host then queues the region launch); 2 hands every pair over
   int32_t tb_mode;
+  // the near launch: a bitmap of 2 * regions bits (pass 0 then pass 1), bit
+  // set for a region holding voxels to be covered, which the launch fills,
+  // and the other set of the two used by turns, which it zeroes for the next
+  // near launch (marks_zero_words words)
+  uint32_t *marks;
+  uint32_t *marks_zero;
+  uint32_t marks_zero_words;
 };
 
 struct TileLaunch {
@@ -72,11 +79,15 @@ struct TileLaunch {
   // the fused launch over regions of 1 x 2 x 2 tiles staged in LDS, eight
   // dilation steps per barrier (null: not built)
   hipError_t (*fused_tb)(dim3, hipStream_t, const DilateArgs &);
+  // the fused launch that packs and runs the near pass (eight steps around the
+  // regions holding voxels to be covered), handing longer pairs to fused_tb
+  hipError_t (*fused_near)(dim3, hipStream_t, const DilateArgs &);
   const void *dilate_fn;         // several pairs
   const void *dilate_single_fn;  // one pair
   const void *fused_fn;          // one pair, reset + pack + seed + dilation
   const void *fused_ds_fn;
   const void *fused_tb_fn;
+  const void *fused_near_fn;
   const void *pack_fn;
   // the pack of a pair of masks packed before (combine_kernel): counts, box
   // and the pair's tile flags from the masks' own flags

@@ -127,16 +127,18 @@ uint64_t hdm_workspace_bytes(int nx, int ny, int nz, int tile);
 /* Device milliseconds of the phases of the last single-pair call on the
  * current device that ran with opts->timing = 1: ms[0] pack (+ reset), ms[1]
  * tile states and first queue, ms[2] persistent dilation (which also
- * finalizes), ms[3] 0, ms[4] all kernels; ms[5] is 1 when the call ran as one
- * fused launch (the default for a single pair: reset, pack, seed and dilation
- * in one persistent kernel, phases then timed by the kernel's own wall clock
- * between its grid barriers) and 0 when it ran as separate kernels (phases
- * timed by HIP events on the stream).  With opts->timing = 3 only the
- * dominant launch is bracketed by a pair of HIP events, the others are -1:
- * ms[2] the dilation kernel of a separate-kernel call, ms[4] the whole fused
- * launch; such a call does not wait for its events, this function does.
- * ms[6] is the number of dilation steps the call ran per grid barrier (2 for
- * the fused 64x8x8 launch with the 6 element, else 1).
+ * finalizes), ms[3] 0, ms[4] all kernels; ms[5] is the number of launches of
+ * the fused path (the default for a single pair: reset, pack and dilation in
+ * one persistent kernel, phases then timed by the kernel's own wall clock
+ * between its grid barriers): 1, or 2 when that launch handed the packed pair
+ * to the region launch; 0 when it ran as separate kernels (phases timed by
+ * HIP events on the stream).  With opts->timing = 3 only the dominant launch
+ * is bracketed by a pair of HIP events, the others are -1: ms[2] the dilation
+ * kernel of a separate-kernel call, ms[4] the whole fused path; such a call
+ * does not wait for its events, this function does.  ms[6] is the number of
+ * dilation steps the body that finished ran per grid barrier (8 for the near
+ * pass and the region launch of 64x8x8 tiles with the 6 element, 2 for the
+ * two-step launch, else 1).
  * Returns how many were written. */
 int hdm_last_timings(float *ms, int n);
 /* Device milliseconds of the dominant launch of each single-pair call made
