@@ -398,13 +398,16 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
         ms.append(b.last_ms())
     t_batch = (time.perf_counter() - t0) / reps
     out = b.sync()
+    near_entries = b.near_entries()
     b.close()
     keys = ("hd", "sum", "count", "sum_fp", "sum_fn", "n_fp", "n_fn")
     same = all(all(o[k] == p[k] for k in keys) for o, p in zip(out, plain))
-    # the batch's launches (reset, pack, totals, seed, dilation) per replay:
-    # pack bytes of every pair plus the moved bytes of every queued entry
-    # (queue_total is the batch's, reported with each pair)
-    moved = float(out[0]["queue_total"]) * TILE_BYTES[best["tile_id"]]
+    # the batch's launches (reset, pack, totals, near pass, seed, dilation) per
+    # replay: pack bytes of every pair plus the moved bytes of every near-pass
+    # entry and every queued entry (queue_total is the batch's, reported with
+    # each pair)
+    moved_tiles = float(out[0]["queue_total"]) * TILE_BYTES[best["tile_id"]]
+    moved = moved_tiles + float(near_entries) * NEAR_BYTES
     dev_ms = statistics.median(ms)
     return {"pairs": n_pairs, "mask_voxels": 2 * nvox,
             "plain_pairs_per_s": round(n_pairs / t_plain, 1),
@@ -413,14 +416,16 @@ def leg_batched(hdm, torch, n_pairs, reps, tune):
             "batched_mask_voxels_per_s": round(2 * nvox / t_batch, 1),
             "batched_device_ms_median": round(statistics.median(ms), 3),
             "queue_total": int(out[0]["queue_total"]), "queue_max": int(out[0]["queue_max"]),
+            "near_entries": near_entries,
             "tile": best["tile"], "chunk": best["chunk"], "graph": True,
             "identical_to_plain": bool(same),
             "roofline": roofline_obj("batch graph replay", 2.25 * nvox + moved, dev_ms,
                                      "batch256", dilation_algo_bytes=int(moved),
                                      pack_algo_bytes=int(2.25 * nvox),
                                      # 64x8x8 tiles run two steps per entry
-                                     algo_bytes_counting_each_step=int(2.25 * nvox + moved * (
-                                         2 if best["tile_id"] == hdm.TILE_64x8x8 else 1))),
+                                     algo_bytes_counting_each_step=int(
+                                         2.25 * nvox + moved - moved_tiles + moved_tiles * (
+                                             2 if best["tile_id"] == hdm.TILE_64x8x8 else 1))),
             "autotune": [{k: e[k] for k in ("tile", "chunk", "ms", "status")}
                          for e in (table or [])][:12]}
 

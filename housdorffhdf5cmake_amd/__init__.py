@@ -155,6 +155,7 @@ def lib() -> C.CDLL:
         "hdm_batch_create": ([C.POINTER(Pair), i32, C.POINTER(Opts), i32, C.POINTER(vp)], i32),
         "hdm_batch_run": ([vp], i32), "hdm_batch_sync": ([vp, C.POINTER(Result)], i32),
         "hdm_batch_last_ms": ([vp], C.c_float), "hdm_batch_destroy": ([vp], None),
+        "hdm_batch_near_entries": ([vp], C.c_uint32),
         "hdm_hausdorff_batch": ([C.POINTER(Pair), i32, C.POINTER(Opts), C.POINTER(Result)], i32),
         "hdm_autotune": ([C.POINTER(Pair), i32, vp, i32, vp, i32, vp, i32, i32,
                           C.POINTER(TuneEntry), i32, C.POINTER(C.c_int)], i32),
@@ -503,6 +504,10 @@ class Batch:
 
     def last_ms(self) -> float:
         return float(lib().hdm_batch_last_ms(self.h))
+
+    def near_entries(self) -> int:
+        """(region, mask) entries the near pass of the last synced run stepped."""
+        return int(lib().hdm_batch_near_entries(self.h))
 
     def close(self):
         if self.h:

@@ -92,7 +92,17 @@ struct PairDesc {
   uint32_t *tflags;
   uint32_t *hist[2];
   uint16_t *map;
+  // near pass (64x8x8 tiles, 6 element): the pair's bitmap of 2 * regions bits
+  // (regions of 1 x 2 x 2 tiles; bit m * regions + rg: region rg holds voxels
+  // pass m covers), zero before the pack sets it; null: no marks
+  uint32_t *marks;
 };
+// regions of 1 x 2 x 2 tiles of a pair's 64x8x8 tile grid
+__host__ __device__ inline uint32_t pair_regions(const PairDesc &d) {
+  return (uint32_t)d.gx * (uint32_t)((d.gy + 1) / 2) * (uint32_t)((d.gz + 1) / 2);
+}
+// words of a pair's near-pass bitmap
+__host__ __device__ constexpr uint32_t near_mark_words(uint32_t nreg) { return (2 * nreg + 31) / 32; }
 
 struct PairCtr {
   unsigned long long n_gold, n_segm, n_fp, n_fn;  // totals (written before the dilation)
@@ -111,12 +121,18 @@ struct PairCtr {
   // same value during an iteration.
   int32_t done_from[2];
   uint32_t hd_last[2];
-  int32_t pad;
+  // 1 when the batched near pass finished the pair (both passes' cuts reached
+  // within eight steps): its counters hold the results, and the seed, the
+  // dilation and its fold leave it alone
+  int32_t near_done;
   // cumulative hits, spread over slots: [m] pass m, or with two dilation
   // steps per engine iteration [2k + m] pass m in step k of each iteration
   unsigned long long hits[4][kHitSlots];
   unsigned long long sums[4][kSumSlots];        // pack partials: gold, segm, fp, fn
   int32_t box[6][kSumSlots];  // pack partials of lo[0..2] and hi[0..2] (box_enc_*)
+  // batched near pass: new hits per (pass m, step k) at index m + 2k, spread
+  // over four slots
+  unsigned long long near_hits[2 * kTbSteps][4];
 };
 
 struct PairOut {
@@ -207,6 +223,8 @@ struct Engine {
   unsigned long long near_hits[2 * kTbSteps][kHitSlots];
   uint32_t near_done[kHitSlots + 1];
   unsigned long long near_phase[3];
+  // batched near pass: entries written to the list (zeroed by the reset)
+  uint32_t near_count;
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

@@ -87,6 +87,12 @@ struct LaunchCfg {
   EngineHead *head = nullptr;      // where the engine head goes (null: nowhere)
   uint32_t *ready = nullptr;       // the dilation's ready marker (host) or null
   uint32_t ready_seq = 0;
+  // batches: the near pass between the totals and the seed (pairs' bitmaps of
+  // near_words words in all, an entry list of near_cap entries), or off
+  uint64_t *near_list = nullptr;
+  uint64_t near_words = 0;
+  uint32_t near_cap = 0;
+  int near_grid = 0;
 };
 
 struct GroupPtrs {

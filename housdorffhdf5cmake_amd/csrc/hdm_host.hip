@@ -313,6 +313,10 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_tiles);
   if (e != hipSuccess) return e;
   if (npairs > 1 && (e = hdm_launch_totals(s, P.ctrs, 0, npairs)) != hipSuccess) return e;
+  if (cfg.near_list && npairs > 1 &&
+      (e = hdm_launch_near_batch(s, P.descs, P.idx, P.ctrs, npairs, cfg.near_words, P.engine,
+                                 cfg.near_list, cfg.near_cap, cfg.near_grid)) != hipSuccess)
+    return e;
   if ((e = mark(1)) != hipSuccess) return e;
   e = tl->seed(dim3(cfg.seed_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,
                total_tiles, 0);
@@ -616,6 +620,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
       a.marks = (uint32_t *)c->marks.p + (size_t)c->marks_set * set_words;
       a.marks_zero = (uint32_t *)c->marks.p + (size_t)(c->marks_set ^ 1) * set_words;
       a.marks_zero_words = set_words;
+      a.desc.marks = a.marks;  // the pack marks through the descriptor
     }
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {
@@ -777,6 +782,7 @@ struct hdm_batch {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   void *host = nullptr;
   float last_ms = 0.f;
+  uint32_t near_entries = 0;  // of the last synced run
 };
 
 static int batch_enqueue(hdm_batch *b) {
@@ -830,17 +836,31 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   std::vector<PackIndex> idx(n);
   uint64_t pair_bytes = 0, qcap = 0;
   int maxh = 0;
+  // the near pass for the 64x8x8 tile with the 6 element at two steps per
+  // iteration (HDM_NEAR=0 turns it off): each pair's region bitmap, and a list
+  // with room for every bit of every bitmap
+  const bool near_batch = b->tl->fused_near && conn == 6 && near_enabled();
+  uint64_t mark_words = 0, mark_bits = 0;
   for (int i = 0; i < n; ++i) {
     const hdm_pair &p = pairs[i];
     const PairGeom g = pair_geom(p.nx, p.ny, p.nz, b->ts);
     b->geoms.push_back(g);
-    idx[i] = PackIndex{b->total_rw, b->total_tiles};
+    idx[i] = PackIndex{b->total_rw, b->total_tiles, mark_words};
     b->total_rw += g.row_words;
     b->total_tiles += g.ntiles;
     qcap += 2ull * g.ntiles;
     pair_bytes += g.total;
     maxh = std::max(maxh, g.hist_len);
+    if (near_batch) {
+      mark_words += near_mark_words(g.nregions);
+      mark_bits += 2ull * g.nregions;
+    }
   }
+  if (mark_bits > 0xFFFFFFFFull) {
+    delete b;
+    return HDM_E_INVALID;
+  }
+  const uint64_t near_bytes = near_batch ? align256(4 * mark_words) + align256(8 * mark_bits) : 0;
   if (qcap > 0xFFFFFFFFull || b->total_tiles > 0xFFFFFFFFull) {
     delete b;
     return HDM_E_INVALID;
@@ -850,7 +870,7 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
   b->cfg = launch_cfg(c, b->tile, b->total_rw, b->total_tiles, opts.grid_ctas, opts.chunk, n);
   b->grid = b->cfg.grid;
   const CommonLayout L = common_layout(n, qcap, maxh);
-  if (hipMalloc(&b->ws, L.total + pair_bytes) != hipSuccess) {
+  if (hipMalloc(&b->ws, L.total + pair_bytes + near_bytes) != hipSuccess) {
     (void)hipGetLastError();
     b->ws = nullptr;
     hdm_batch_destroy(b);
@@ -864,6 +884,14 @@ extern "C" int hdm_batch_create(const hdm_pair *pairs, int n, const hdm_opts *op
               opts.crop ? 1 : 0, b->tile, 0, conn);
     carve_pair(descs[i], b->geoms[i], pb);
     pb += b->geoms[i].total;
+  }
+  if (near_batch) {  // [bitmaps of every pair][entry list]
+    uint32_t *mk = (uint32_t *)pb;
+    for (int i = 0; i < n; ++i) descs[i].marks = mk + idx[i].mark_base;
+    b->cfg.near_list = (uint64_t *)(pb + align256(4 * mark_words));
+    b->cfg.near_words = mark_words;
+    b->cfg.near_cap = (uint32_t)mark_bits;
+    b->cfg.near_grid = std::min(kMaxGrid, std::max(1, c->occ_fused_near[b->tile]) * c->cus);
   }
   // a blocking stream: a run is ordered after work on the legacy default stream
   // (where callers such as torch usually produced the input volumes)
@@ -917,6 +945,11 @@ extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, b->e0, b->e1) == hipSuccess) b->last_ms = ms;
   (void)hipGetLastError();
+  b->near_entries = 0;
+  if (b->cfg.near_list &&
+      hipMemcpy(&b->near_entries, &b->P.engine->near_count, sizeof(uint32_t),
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return HDM_E_HIP;
   if (!out) return HDM_OK;
   if (hipMemcpy(b->host, b->P.outs, sizeof(PairOut) * b->n, hipMemcpyDeviceToHost) !=
           hipSuccess ||
@@ -937,6 +970,7 @@ extern "C" int hdm_batch_sync(hdm_batch *b, hdm_result *out) {
 }
 
 extern "C" float hdm_batch_last_ms(hdm_batch *b) { return b ? b->last_ms : 0.f; }
+extern "C" uint32_t hdm_batch_near_entries(hdm_batch *b) { return b ? b->near_entries : 0u; }
 
 extern "C" int hdm_hausdorff_batch(const hdm_pair *pairs, int n, const hdm_opts *opts,
                                    hdm_result *out) {

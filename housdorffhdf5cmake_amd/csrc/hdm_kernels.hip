@@ -237,6 +237,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
       es->qtotal = 0;
       es->qcap = qcap;
       es->qnext = 0;
+      es->near_count = 0;
     }
     for (int i = g; i < 4 * kLineWords; i += gs) es->qlen[0][i] = 0;
     for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
@@ -261,6 +262,15 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     }
     for (int i = tid; i < 4 * kHitSlots; i += blockDim.x) (&c.hits[0][0])[i] = 0;
     for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) (&c.sums[0][0])[i] = 0;
+    if (tid == 0) c.near_done = 0;
+    for (int i = tid; i < 2 * kTbSteps * 4; i += blockDim.x) (&c.near_hits[0][0])[i] = 0;
+    {  // the pair's near-pass bitmap, which the pack then sets
+      const PairDesc &dd = write_single ? single : descs[p];
+      if (dd.marks) {
+        const uint32_t nw = near_mark_words(pair_regions(dd));
+        for (uint32_t i = tid; i < nw; i += blockDim.x) dd.marks[i] = 0;
+      }
+    }
     for (int i = tid; i < 6 * kSumSlots; i += blockDim.x)
       (&c.box[0][0])[i] = 0;
     uint32_t *h0 = write_single ? single.hist[0] : descs[p].hist[0];
@@ -399,29 +409,20 @@ __device__ __forceinline__ void put(T *p, T v) {
     *p = v;
 }
 
-// Where the pack of the near launch marks the regions of 1 x 2 x 2 tiles that
-// hold voxels to be covered: bit m * nreg + rg of a bitmap that starts out
-// zero (pass m = 0: FP = segm and not gold, the voxels gold's dilation
-// covers; 1: FN), set by an atomic OR.
-struct NearMarks {
-  uint32_t *w;    // ceil(2 * nreg / 32) words
-  uint32_t nreg;  // regions of the pair
-  int rgx, rgy;   // region grid in x and y
-};
-
 // The pack pass over all tiles of the launch's pairs, spread over the grid's
 // waves.  COH: the same launch reads the results in later phases (the fused
 // single-pair launch), so every store is written through; the first-hit map
-// is then cleared by the host instead.  MARK (64x8x8 tiles, one pair): a tile
-// with FP or FN voxels marks its region for the near pass (NearMarks).
+// is then cleared by the host instead.  MARK (64x8x8 tiles): a tile with FP or
+// FN voxels of a pair whose descriptor carries a near-pass bitmap (marks)
+// sets, by an atomic OR, the bit m * regions + rg of its region rg of
+// 1 x 2 x 2 tiles, for pass m = 0 (FP = segm and not gold, the voxels gold's
+// dilation covers) and m = 1 (FN).
 template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
-                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr,
-                                           const NearMarks *nm = nullptr) {
-  static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8 && COH),
-                "near marks: 64x8x8 tiles, one pair");
+                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr) {
+  static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8), "near marks: 64x8x8 tiles");
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -627,12 +628,13 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
       if constexpr (MARK) {  // one tile per wave: the ballots cover the tile
         const bool fp = __ballot(c2 != 0) != 0, fn = __ballot(c3 != 0) != 0;
-        if (in && r == 0 && (fp || fn)) {
+        if (in && r == 0 && (fp || fn) && d.marks) {
+          const uint32_t rgx = (uint32_t)d.gx, rgy = (uint32_t)((d.gy + 1) / 2);
+          const uint32_t nreg = pair_regions(d);
           const uint32_t rg = (uint32_t)txU[u] +
-                              (uint32_t)nm->rgx * ((uint32_t)(tyU[u] >> 1) +
-                                                   (uint32_t)nm->rgy * (uint32_t)(tzU[u] >> 1));
-          if (fp) or_g(nm->w + (rg >> 5), 1u << (rg & 31));
-          if (fn) or_g(nm->w + ((nm->nreg + rg) >> 5), 1u << ((nm->nreg + rg) & 31));
+                              rgx * ((uint32_t)(tyU[u] >> 1) + rgy * (uint32_t)(tzU[u] >> 1));
+          if (fp) or_g(d.marks + (rg >> 5), 1u << (rg & 31));
+          if (fn) or_g(d.marks + ((nreg + rg) >> 5), 1u << ((nreg + rg) & 31));
         }
       }
       const bool ne = (a | b) != 0;
@@ -683,7 +685,8 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
                                                           uint64_t total_tiles) {
   // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
   __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-  pack_tiles<TXB, TY, TZ, false>(descs, idx, ctrs, npairs, total_tiles, s_pieces);
+  constexpr bool kMark = TXB == 64 && TY == 8 && TZ == 8;  // pairs with a bitmap get marks
+  pack_tiles<TXB, TY, TZ, false, kMark>(descs, idx, ctrs, npairs, total_tiles, s_pieces);
 }
 
 // The pack pass of a pair whose two masks were packed before, each on its own
@@ -1101,7 +1104,9 @@ __global__ __launch_bounds__(kWgThreads) void seed_kernel(const PairDesc *__rest
       const uint32_t f = (own >> (16 * m)) & 0xFFFF;
       const bool any = f & 1, full = (f >> 1) & 1;
       const bool act = ((reach >> m) & 1u) != 0;
-      const bool needed = robust_cut(d.robust, m == 0 ? nfp : nfn) != 0;
+      // a pass whose cut is zero, or whose pair the batched near pass
+      // finished, queues nothing
+      const bool needed = robust_cut(d.robust, m == 0 ? nfp : nfn) != 0 && !c.near_done;
       const uint32_t st = !in_box ? 0u : (any && full) ? kFull : (any || (act && needed)) ? 1u : 0u;
       if (in) d.state[m][tile] = st;
       want[m] = in_box && needed && (any ? !full : act);
@@ -1608,8 +1613,12 @@ __device__ PairOut finalize_pair(const PairDesc &d, const PairCtr &c, uint32_t t
     hd[m] = ld_sc(&c.hd_last[m]);
     df[m] = ld_sc(&c.done_from[m]);
   }
+  // a pair the batched near pass finished is done at its stop steps (at most
+  // eight), however many iterations the engine ran
+  const bool nd = ld_sc(&c.near_done) != 0;
   return finalize_from(d, ld_sc(&c.n_gold), ld_sc(&c.n_segm), ld_sc(&c.n_fp), ld_sc(&c.n_fn), cov,
-                       sum, hd, df, ld_sc(&c.status), t_engine, error, steps);
+                       sum, hd, df, ld_sc(&c.status), nd ? d.t_start + kTbSteps : t_engine, error,
+                       nd ? 1 : steps);
 }
 
 // A lower bound of the HD in dilation steps (6 element, 64x8x8 tiles) from
@@ -1732,7 +1741,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           st_sc(&c0.cov[m], 0ull), st_sc(&c0.sum_d[m], 0ull);
           st_sc(&c0.done_from[m], 0x7FFFFFFF), st_sc(&c0.hd_last[m], 0u);
         }
-        st_sc(&c0.status, 0);
+        st_sc(&c0.status, 0), st_sc(&c0.near_done, 0);
         st_sc(&es->t, 0u), st_sc(&es->error, 0u), st_sc(&es->qmax, 0u);
         st_sc(&es->qtotal, 0ull), st_sc(&es->qcap, qcap), st_sc(&es->qnext, 0u);
       }
@@ -2610,7 +2619,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         const bool in = i < 2 * npairs;
         unsigned long long v = 0, v2 = 0, cov = 0, c2 = 0, sum = 0, nm = 0;
         uint32_t ts = 0, hd = 0;
-        int df0 = 0;
+        int df0 = 0, nd = 0;
         if (in) {
           for (int k = s; k < kHitSlots; k += per) {
             v += ld_sc(&ctrs[pp].hits[mm][k]);
@@ -2618,6 +2627,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           }
           if (s == 0) {
             ts = descs[pp].t_start;
+            nd = ctrs[pp].near_done;  // written before this launch
             df0 = ld_sc(&ctrs[pp].done_from[mm]);
             cov = ld_sc(&ctrs[pp].cov[mm]);
             sum = ld_sc(&ctrs[pp].sum_d[mm]);
@@ -2632,7 +2642,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
           v += shfl64(v, (threadIdx.x & 63) ^ off);
           if constexpr (DS) v2 += shfl64(v2, (threadIdx.x & 63) ^ off);
         }
-        if (!in || s != 0 || tdone < ts) continue;
+        if (!in || s != 0 || tdone < ts || nd) continue;  // finished by the near pass
         const int tpp = (int)(tdone - ts);
         if (df0 <= kSteps * tpp) continue;  // the pass stopped at an earlier bin
         PairCtr &cc = ctrs[pp];
@@ -3534,6 +3544,132 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
 // launch (status 2), which computes it from the start; so does a pair whose
 // HD bound from the tile boxes already exceeds eight steps, right after the
 // pack.
+// LDS of one near-pass entry: the window rows exchanged in z per step, and
+// the entry's counts
+struct NearLds {
+  u64x2 win[2][8][2][kTbWin];
+  uint32_t hs[kTbSteps];  // new hits per step
+  uint32_t need;          // voxels to cover
+};
+// One near-pass entry by the whole workgroup: region rg of pair d (1 x 2 x 2
+// tiles of 64x8x8), pass m.  It builds the region launch's window (32 x 32
+// rows in (y, z), four rows of consecutive z per thread, each 64 bits plus
+// eight halo columns on each side) from the original packed bits of mask m,
+// and runs up to eight 6-neighbour steps there, stopping once every voxel of
+// the other mask in the region (not in mask m) is covered.  L.hs and L.need
+// must be zero on entry; on return, after a workgroup barrier, L.hs[k] holds
+// the region's new hits of step k + 1, which the caller takes and clears
+// (with L.need) before the next entry.  The packed masks must be visible to
+// plain loads: written by an earlier launch, or written through in this one
+// and read by no workgroup before.
+__device__ __forceinline__ void near_window(const PairDesc &d, uint32_t rg, int m, NearLds &L) {
+  constexpr int KS = kTbSteps;
+  constexpr int R = 64;  // rows per tile
+  const int lane = threadIdx.x & 63;
+  // window mapping of the region launch: thread t holds rows (yy, zz0 .. zz0 + 3)
+  const int yy = threadIdx.x & 31;
+  const int seg = threadIdx.x >> 5;
+  const int zz0 = 4 * seg;
+  const int tyi = yy / 8 - 1, tzi = seg / 2 - 1;  // tile offsets in the region, -1 .. 2
+  const bool core = yy >= 8 && yy < 24 && seg >= 2 && seg < 6;
+  const int rgx = d.gx, rgy = (d.gy + 1) / 2;
+  const int rx = (int)(rg % (uint32_t)rgx);
+  const int ry = (int)((rg / (uint32_t)rgx) % (uint32_t)rgy);
+  const int rz = (int)(rg / ((uint32_t)rgx * (uint32_t)rgy));
+  // pass m dilates mask m and covers the other mask's voxels
+  const uint64_t *orig_m = (const uint64_t *)(m ? d.orig[1] : d.orig[0]);
+  const uint64_t *orig_o = (const uint64_t *)(m ? d.orig[0] : d.orig[1]);
+  const int ty = 2 * ry + tyi, tz = 2 * rz + tzi;
+  const int yl = yy & 7;
+  const bool tyz = ty >= 0 && ty < d.gy && tz >= 0 && tz < d.gz;
+  const bool yok = tyz && ty * 8 + yl < d.ny;
+  bool tin[3];
+  uint64_t vb[3];
+  uint32_t tix[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int ux = rx + c - 1;
+    tin[c] = tyz && ux >= 0 && ux < d.gx;
+    vb[c] = tin[c] ? valid_bits<uint64_t>(ux, d.nx) : 0ull;
+    tix[c] = tin[c] ? (uint32_t)ux + (uint32_t)d.gx * ((uint32_t)ty + (uint32_t)d.gy * (uint32_t)tz) : 0u;
+  }
+  // every load at once, at a valid address (tile 0 stands in for a tile
+  // outside the grid), the stand-ins dropped afterwards; the packed masks
+  // were written through by the pack and no workgroup read them before
+  // the barrier, so plain loads see them
+  bool rok[4];
+  uint64_t w[4][3], oth[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int zl = (zz0 + i) & 7;
+    rok[i] = yok && tz * 8 + zl < d.nz;
+    const uint32_t r = (uint32_t)(yl + 8 * zl);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) w[i][c] = ldg(orig_m + (uint64_t)tix[c] * R + r);
+    oth[i] = ldg(orig_o + (uint64_t)tix[1] * R + r);
+  }
+  TbRow X[4], V[4];
+  uint32_t need = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (!(rok[i] && tin[c])) w[i][c] = 0;
+    V[i] = rok[i] ? tb_make(vb[0], vb[1], vb[2]) : TbRow{0, 0};
+    X[i] = tb_make(w[i][0], w[i][1], w[i][2]);
+    X[i].o &= V[i].o, X[i].h &= V[i].h;
+    // the voxels to cover: the other mask's, not in the source (its own
+    // rows only; the window's ring belongs to other entries)
+    oth[i] = (core && rok[i] && tin[1]) ? oth[i] & ~X[i].o : 0ull;
+    need += (uint32_t)__popcll(oth[i]);
+  }
+  need = (uint32_t)wave_reduce32((int)need, 0, [](int x, int y) {
+    return (int)((uint32_t)x + (uint32_t)y);
+  });
+  if (lane == 0 && need) atomicAdd(&L.need, need);
+  uint32_t covered = 0, want = 0;
+#pragma unroll 1
+  for (int k = 0; k < KS; ++k) {
+    // rows z-1 / z+1 beyond the thread's four come through LDS; at the
+    // window's edges the neighbour read is some other row, and a wrong
+    // voxel there moves inward one row per step, so after eight steps it
+    // is still one row short of the region's own rows
+    L.win[k & 1][seg][0][yy] = u64x2{X[0].o, X[0].h};
+    L.win[k & 1][seg][1][yy] = u64x2{X[3].o, X[3].h};
+    __syncthreads();
+    // every count read here was added before this barrier, so the whole
+    // workgroup takes the same decision
+    if (k == 0)
+      want = L.need;
+    else
+      covered += L.hs[k - 1];
+    if (covered >= want) break;  // every voxel of the entry is covered
+    const u64x2 ezm = L.win[k & 1][seg > 0 ? seg - 1 : 0][1][yy];
+    const u64x2 ezp = L.win[k & 1][seg < 7 ? seg + 1 : 7][0][yy];
+    TbRow N[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const TbRow ym = tb_lane_nb<true>(X[i]), yp = tb_lane_nb<false>(X[i]);
+      const TbRow zm = i > 0 ? X[i - 1] : TbRow{ezm.x, (uint32_t)ezm.y};
+      const TbRow zp = i < 3 ? X[i + 1] : TbRow{ezp.x, (uint32_t)ezp.y};
+      const TbRow xd = tb_xdil(X[i]);
+      N[i] = TbRow{(xd.o | ym.o | yp.o | zm.o | zp.o) & V[i].o,
+                   (xd.h | ym.h | yp.h | zm.h | zp.h) & V[i].h};
+    }
+    // new hits of step k + 1: res & ~src & ref (hello.cu:1590-1626)
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c += (uint32_t)__popcll(N[i].o & ~X[i].o & oth[i]);
+    c = (uint32_t)wave_reduce32((int)c, 0, [](int x, int y) {
+      return (int)((uint32_t)x + (uint32_t)y);
+    });
+    if (lane == 0 && c) atomicAdd(&L.hs[k], c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) X[i] = N[i];
+  }
+  __syncthreads();  // every count of the entry is in L.hs
+}
+
 constexpr int kNearSteps = kTbSteps;
 // bitmap words of the region marks a thread of the share count holds at most
 // (256 threads: up to 2^16 regions of two passes)
@@ -3542,7 +3678,6 @@ constexpr uint32_t kNearWordsPerThread = 16;
 __global__ __launch_bounds__(kWgThreads)
 __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   constexpr int KS = kNearSteps;
-  constexpr int R = 64;  // rows per tile
   static_assert(KS == 8, "the window holds an eight-voxel halo");
   static_assert(2 * KS * kHitSlots == kWgThreads, "one hit slot per thread in the fold");
   PairCtr *ctrs = a.ctrs;
@@ -3552,13 +3687,11 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   PairCtr &c0 = ctrs[0];
   const uint32_t qcap = a.qcap;
 
-  __shared__ u64x2 s_win[2][8][2][kTbWin];
+  __shared__ NearLds s_near;              // the running entry's window and counts
   __shared__ int s_ok;
   __shared__ unsigned long long s_tot[4];
   __shared__ int s_hand;                  // 1: the pair goes to the region launch
   __shared__ uint32_t s_hits[2 * KS];     // the workgroup's hits, index m + 2k
-  __shared__ uint32_t s_hs[KS];           // the running entry's hits per step
-  __shared__ uint32_t s_need;             // the running entry's voxels to cover
   __shared__ uint32_t s_list[kWgThreads]; // mark bit indices of the entries taken
   __shared__ uint32_t s_wsum[kWavesPerWg];
   __shared__ int s_last;
@@ -3566,18 +3699,11 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  // window mapping of the region launch: thread t holds rows (yy, zz0 .. zz0 + 3)
-  const int yy = threadIdx.x & 31;
-  const int seg = threadIdx.x >> 5;
-  const int zz0 = 4 * seg;
-  const int tyi = yy / 8 - 1, tzi = seg / 2 - 1;  // tile offsets in the region, -1 .. 2
-  const bool core = yy >= 8 && yy < 24 && seg >= 2 && seg < 6;
-  const int rgx = d.gx, rgy = (d.gy + 1) / 2, rgz = (d.gz + 1) / 2;
-  const uint32_t nreg = (uint32_t)rgx * (uint32_t)rgy * (uint32_t)rgz;
+  const uint32_t nreg = pair_regions(d);
 
   if (threadIdx.x < 2 * KS) s_hits[threadIdx.x] = 0, s_h[threadIdx.x] = 0;
-  if (threadIdx.x < KS) s_hs[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_need = 0;
+  if (threadIdx.x < KS) s_near.hs[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_near.need = 0;
   uint64_t ph0 = 0, ph1 = 0;  // workgroup 0: clock at its start and at the end of its pack
   if (blockIdx.x == 0) ph0 = wall_clock64();
   // trace (timing 2): one record of 8 words per workgroup -- [0] start, [1]
@@ -3605,7 +3731,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
         st_sc(&c0.cov[m], 0ull), st_sc(&c0.sum_d[m], 0ull);
         st_sc(&c0.done_from[m], 0x7FFFFFFF), st_sc(&c0.hd_last[m], 0u);
       }
-      st_sc(&c0.status, 0);
+      st_sc(&c0.status, 0), st_sc(&c0.near_done, 0);
       st_sc(&es->t, 0u), st_sc(&es->error, 0u), st_sc(&es->qmax, 0u);
       st_sc(&es->qtotal, 0ull), st_sc(&es->qcap, qcap), st_sc(&es->qnext, 0u);
     }
@@ -3626,8 +3752,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   // P: pack, every store written through, regions marked
   {
     __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-    const NearMarks nm{a.marks, nreg, rgx, rgy};
-    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc, &nm);
+    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc);
   }
   if (blockIdx.x == 0) ph1 = wall_clock64();
   if (trw && threadIdx.x == 0) trw[1] = wall_clock64();
@@ -3730,106 +3855,12 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   auto near_entry = [&](uint32_t wi) {
     const int m = wi >= nreg ? 1 : 0;
     const uint32_t rg = wi - (m ? nreg : 0u);
-    const int rx = (int)(rg % (uint32_t)rgx);
-    const int ry = (int)((rg / (uint32_t)rgx) % (uint32_t)rgy);
-    const int rz = (int)(rg / ((uint32_t)rgx * (uint32_t)rgy));
-    // pass m dilates mask m and covers the other mask's voxels
-    const uint64_t *orig_m = (const uint64_t *)(m ? d.orig[1] : d.orig[0]);
-    const uint64_t *orig_o = (const uint64_t *)(m ? d.orig[0] : d.orig[1]);
-    const int ty = 2 * ry + tyi, tz = 2 * rz + tzi;
-    const int yl = yy & 7;
-    const bool tyz = ty >= 0 && ty < d.gy && tz >= 0 && tz < d.gz;
-    const bool yok = tyz && ty * 8 + yl < d.ny;
-    bool tin[3];
-    uint64_t vb[3];
-    uint32_t tix[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int ux = rx + c - 1;
-      tin[c] = tyz && ux >= 0 && ux < d.gx;
-      vb[c] = tin[c] ? valid_bits<uint64_t>(ux, d.nx) : 0ull;
-      tix[c] = tin[c] ? (uint32_t)ux + (uint32_t)d.gx * ((uint32_t)ty + (uint32_t)d.gy * (uint32_t)tz) : 0u;
-    }
-    // every load at once, at a valid address (tile 0 stands in for a tile
-    // outside the grid), the stand-ins dropped afterwards; the packed masks
-    // were written through by the pack and no workgroup read them before
-    // the barrier, so plain loads see them
-    bool rok[4];
-    uint64_t w[4][3], oth[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int zl = (zz0 + i) & 7;
-      rok[i] = yok && tz * 8 + zl < d.nz;
-      const uint32_t r = (uint32_t)(yl + 8 * zl);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) w[i][c] = ldg(orig_m + (uint64_t)tix[c] * R + r);
-      oth[i] = ldg(orig_o + (uint64_t)tix[1] * R + r);
-    }
-    TbRow X[4], V[4];
-    uint32_t need = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        if (!(rok[i] && tin[c])) w[i][c] = 0;
-      V[i] = rok[i] ? tb_make(vb[0], vb[1], vb[2]) : TbRow{0, 0};
-      X[i] = tb_make(w[i][0], w[i][1], w[i][2]);
-      X[i].o &= V[i].o, X[i].h &= V[i].h;
-      // the voxels to cover: the other mask's, not in the source (its own
-      // rows only; the window's ring belongs to other entries)
-      oth[i] = (core && rok[i] && tin[1]) ? oth[i] & ~X[i].o : 0ull;
-      need += (uint32_t)__popcll(oth[i]);
-    }
-    need = (uint32_t)wave_reduce32((int)need, 0, [](int x, int y) {
-      return (int)((uint32_t)x + (uint32_t)y);
-    });
-    if (lane == 0 && need) atomicAdd(&s_need, need);
-    uint32_t covered = 0, want = 0;
-#pragma unroll 1
-    for (int k = 0; k < KS; ++k) {
-      // rows z-1 / z+1 beyond the thread's four come through LDS; at the
-      // window's edges the neighbour read is some other row, and a wrong
-      // voxel there moves inward one row per step, so after eight steps it
-      // is still one row short of the region's own rows
-      s_win[k & 1][seg][0][yy] = u64x2{X[0].o, X[0].h};
-      s_win[k & 1][seg][1][yy] = u64x2{X[3].o, X[3].h};
-      __syncthreads();
-      // every count read here was added before this barrier, so the whole
-      // workgroup takes the same decision
-      if (k == 0)
-        want = s_need;
-      else
-        covered += s_hs[k - 1];
-      if (covered >= want) break;  // every voxel of the entry is covered
-      const u64x2 ezm = s_win[k & 1][seg > 0 ? seg - 1 : 0][1][yy];
-      const u64x2 ezp = s_win[k & 1][seg < 7 ? seg + 1 : 7][0][yy];
-      TbRow N[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const TbRow ym = tb_lane_nb<true>(X[i]), yp = tb_lane_nb<false>(X[i]);
-        const TbRow zm = i > 0 ? X[i - 1] : TbRow{ezm.x, (uint32_t)ezm.y};
-        const TbRow zp = i < 3 ? X[i + 1] : TbRow{ezp.x, (uint32_t)ezp.y};
-        const TbRow xd = tb_xdil(X[i]);
-        N[i] = TbRow{(xd.o | ym.o | yp.o | zm.o | zp.o) & V[i].o,
-                     (xd.h | ym.h | yp.h | zm.h | zp.h) & V[i].h};
-      }
-      // new hits of step k + 1: res & ~src & ref (hello.cu:1590-1626)
-      uint32_t c = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) c += (uint32_t)__popcll(N[i].o & ~X[i].o & oth[i]);
-      c = (uint32_t)wave_reduce32((int)c, 0, [](int x, int y) {
-        return (int)((uint32_t)x + (uint32_t)y);
-      });
-      if (lane == 0 && c) atomicAdd(&s_hs[k], c);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) X[i] = N[i];
-    }
-    __syncthreads();  // every count of the entry is in s_hs
+    near_window(d, rg, m, s_near);
     if (threadIdx.x < KS) {
-      s_hits[m + 2 * threadIdx.x] += s_hs[threadIdx.x];
-      s_hs[threadIdx.x] = 0;
+      s_hits[m + 2 * threadIdx.x] += s_near.hs[threadIdx.x];
+      s_near.hs[threadIdx.x] = 0;
     }
-    if (threadIdx.x == 0) s_need = 0;
+    if (threadIdx.x == 0) s_near.need = 0;
     __syncthreads();  // cleared before the next entry adds to them
   };
 
@@ -3946,6 +3977,151 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   const PairOut o = finalize_from(d, s_tot[0], s_tot[1], s_tot[2], s_tot[3], cov, sum, hd, dfrom,
                                   0, 1, err, KS);
   publish(false, err, total, trm, &o);
+}
+
+// ---------------------------------------------------------------- near pass, batches
+// The same near pass over many pairs (hdm_batch_*, 64x8x8 tiles, 6 element),
+// as three kernels between the pack (which sets each pair's bitmap) and the
+// seed: near_list_kernel compacts the set bits of every pair's bitmap into one
+// list of (pair, bit) entries; near_batch_kernel steps each entry in its
+// window (near_window) and adds the per-step hits into the pair's spread
+// slots; near_fold_kernel folds each pair's bins 1 .. 8 with the robust cut
+// (hello.cu:1748-1759).  A pair whose two cuts are reached within eight steps
+// is finished there (near_done, histogram bins, counters); the seed, the
+// dilation and its fold leave it alone and compute the others from the start.
+
+// One thread per bitmap word of all pairs (pair p's words start at
+// idx[p].mark_base, its descriptor's marks pointing at them); each workgroup
+// reserves room for its entries with one atomic add.  An entry is
+// (pair << 32) | bit, bit = m * regions + rg.
+__global__ __launch_bounds__(kWgThreads) void near_list_kernel(const PairDesc *__restrict__ descs,
+                                                               const PackIndex *__restrict__ idx,
+                                                               int npairs, uint64_t total_words,
+                                                               uint64_t *list, uint32_t *count,
+                                                               uint32_t cap) {
+  __shared__ uint32_t s_wsum[kWavesPerWg];
+  __shared__ uint32_t s_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint64_t w0 = (uint64_t)blockIdx.x * kWgThreads; w0 < total_words;
+       w0 += (uint64_t)gridDim.x * kWgThreads) {
+    const uint64_t g = w0 + threadIdx.x;
+    const bool in = g < total_words;
+    int p = 0;
+    uint32_t bits = 0, lw = 0;
+    if (in) {
+      p = find_pair(npairs, g, [&](int i) { return idx[i].mark_base; });
+      lw = (uint32_t)(g - idx[p].mark_base);
+      bits = descs[p].marks[lw];
+      const uint32_t nbits = 2 * pair_regions(descs[p]);  // bits past the pair's end stay clear
+      if (lw * 32 + 32 > nbits) bits &= (1u << (nbits - lw * 32)) - 1u;
+    }
+    const uint32_t n = (uint32_t)__popc(bits);
+    const uint32_t incl = wave_scan32(n);
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    uint32_t before = incl - n, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kWavesPerWg; ++k) {
+      if (k < wave) before += s_wsum[k];
+      tot += s_wsum[k];
+    }
+    if (threadIdx.x == 0) s_base = tot ? atomicAdd(count, tot) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + before;
+    while (bits) {
+      const uint32_t b = (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1;
+      if (pos < cap) list[pos] = ((uint64_t)(uint32_t)p << 32) | (lw * 32 + b);
+      ++pos;
+    }
+    __syncthreads();  // s_wsum and s_base are free for the next words
+  }
+}
+
+// Entries taken round robin by the workgroups, one whole workgroup per entry.
+// An entry past the list's capacity was never written (the host sizes the list
+// for every bit of every bitmap, so none is).
+__global__ __launch_bounds__(kWgThreads)
+__attribute__((amdgpu_waves_per_eu(2))) void near_batch_kernel(const PairDesc *__restrict__ descs,
+                                                              PairCtr *ctrs, const uint64_t *list,
+                                                              const uint32_t *count, uint32_t cap) {
+  __shared__ NearLds s_near;
+  __shared__ uint64_t s_ent;
+  if (threadIdx.x < kTbSteps) s_near.hs[threadIdx.x] = 0;
+  if (threadIdx.x == 0) s_near.need = 0;
+  const uint32_t n = min(*count, cap);
+  for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    if (threadIdx.x == 0) s_ent = list[e];
+    __syncthreads();
+    const uint64_t ent = s_ent;
+    const int p = (int)(ent >> 32);
+    const uint32_t bit = (uint32_t)ent;
+    const PairDesc &d = descs[p];
+    const uint32_t nreg = pair_regions(d);
+    const int m = bit >= nreg ? 1 : 0;
+    near_window(d, bit - (m ? nreg : 0u), m, s_near);
+    if (threadIdx.x < kTbSteps) {
+      const uint32_t h = s_near.hs[threadIdx.x];
+      if (h) atomicAdd(&ctrs[p].near_hits[m + 2 * threadIdx.x][blockIdx.x & 3], (unsigned long long)h);
+      s_near.hs[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) s_near.need = 0;
+    __syncthreads();  // counts cleared and s_ent read before the next entry
+  }
+}
+
+// One wave per pair: the bins 1 .. 8 of both passes, folded in order up to
+// the cut; a pair whose two cuts are reached is finished (its histogram bins,
+// covered counts, sums of first hits, last hits and stop steps, and
+// near_done); the others are left to the seed and the dilation.  A pair with
+// an empty mask facing a non-empty one is left to them too (they report it).
+__global__ __launch_bounds__(kWgThreads) void near_fold_kernel(const PairDesc *__restrict__ descs,
+                                                              PairCtr *ctrs, int npairs) {
+  const int lane = threadIdx.x & 63;
+  const int p = blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+  if (p >= npairs) return;
+  PairCtr &c = ctrs[p];
+  const PairDesc &d = descs[p];
+  unsigned long long h = 0;
+  if (lane < 2 * kTbSteps * 4) h = (&c.near_hits[0][0])[lane];
+  // lanes 4i .. 4i + 3 hold the slots of index i (m + 2k): sum them
+  h += dpp64<0x111, 0xF>(h);  // row_shr:1
+  h += dpp64<0x112, 0xF>(h);  // row_shr:2
+  unsigned long long bin[2 * kTbSteps];
+#pragma unroll
+  for (int i = 0; i < 2 * kTbSteps; ++i)
+    bin[i] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)h, 4 * i + 3) |
+             ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(h >> 32), 4 * i + 3) << 32);
+  if (lane != 0) return;
+  const unsigned long long nf[2] = {c.n_fp, c.n_fn};
+  if ((c.n_fp && !c.n_gold) || (c.n_fn && !c.n_segm)) return;
+  unsigned long long cut[2], cov[2] = {0, 0}, sum[2] = {0, 0};
+  uint32_t hd[2] = {0, 0}, taken[2] = {0, 0};
+  int dfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
+  for (int m = 0; m < 2; ++m) {
+    cut[m] = robust_cut(d.robust, nf[m]);
+    for (int k = 0; k < kTbSteps; ++k) {
+      const int it = k + 1;
+      const unsigned long long hm = bin[m + 2 * k];
+      if (cut[m] == 0 || dfrom[m] < it || hm == 0) continue;
+      taken[m] |= 1u << k;
+      cov[m] += hm;
+      sum[m] += hm * (unsigned long long)it;
+      hd[m] = (uint32_t)it;
+      if (cov[m] >= cut[m]) dfrom[m] = it;
+    }
+    if (!(cut[m] == 0 || cov[m] >= cut[m])) return;  // beyond eight steps
+  }
+  for (int m = 0; m < 2; ++m) {
+    for (int k = 0; k < kTbSteps; ++k)
+      if (((taken[m] >> k) & 1u) && k + 1 < d.hist_len) d.hist[m][k + 1] = (uint32_t)bin[m + 2 * k];
+    c.cov[m] = cov[m];
+    c.cov2[m] = 0;
+    c.sum_d[m] = sum[m];
+    c.hd_last[m] = hd[m];
+    c.done_from[m] = dfrom[m];
+  }
+  c.near_done = 1;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -4069,6 +4245,28 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
   const int blocks = std::max(16, std::min(npairs, 1024));
   hipLaunchKernelGGL(hdm::reset_kernel, dim3(blocks), dim3(hdm::kWgThreads), 0, s, descs, one,
                      one_idx, idx, single ? 1 : 0, ctrs, p0, npairs, es, reset_engine, qcap);
+  return hipGetLastError();
+}
+
+// The batched near pass, between the totals and the seed: the entry list
+// (Engine::near_count entries, at most cap, into list), the entries on `grid`
+// workgroups, and the fold of every pair.
+extern "C" hipError_t hdm_launch_near_batch(hipStream_t s, const hdm::PairDesc *descs,
+                                            const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
+                                            int npairs, uint64_t total_words, hdm::Engine *es,
+                                            uint64_t *list, uint32_t cap, int grid) {
+  const uint64_t lb = std::min<uint64_t>(4096, (total_words + hdm::kWgThreads - 1) / hdm::kWgThreads);
+  hipLaunchKernelGGL(hdm::near_list_kernel, dim3((unsigned)std::max<uint64_t>(1, lb)),
+                     dim3(hdm::kWgThreads), 0, s, descs, idx, npairs, total_words, list,
+                     &es->near_count, cap);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(hdm::near_batch_kernel, dim3(std::max(1, grid)), dim3(hdm::kWgThreads), 0, s,
+                     descs, ctrs, list, &es->near_count, cap);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(hdm::near_fold_kernel,
+                     dim3((npairs + hdm::kWavesPerWg - 1) / hdm::kWavesPerWg),
+                     dim3(hdm::kWgThreads), 0, s, descs, ctrs, npairs);
   return hipGetLastError();
 }
 

@@ -219,6 +219,7 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
     a.marks = marks + (size_t)D.marks_set * kMarkSetWords;
     a.marks_zero = marks + (size_t)(D.marks_set ^ 1) * kMarkSetWords;
     a.marks_zero_words = (uint32_t)kMarkSetWords;
+    a.desc.marks = a.marks;  // the pack marks through the descriptor
   }
   const int grid = fused_grid_of(near_path ? D.occ_fused_near[tile]
                                  : ds      ? D.occ_fused_ds[tile]

@@ -13,6 +13,7 @@ namespace hdm {
 struct PackIndex {
   uint64_t rw_base;
   uint64_t tile_base;
+  uint64_t mark_base;  // first word of the pair's near-pass bitmap in the batch's
 };
 
 // Arguments of one persistent dilation launch.  When the launch leaves, its
@@ -239,6 +240,10 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
                                        hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
                                        int reset_engine, uint32_t qcap);
 extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs);
+extern "C" hipError_t hdm_launch_near_batch(hipStream_t s, const hdm::PairDesc *descs,
+                                            const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
+                                            int npairs, uint64_t total_words, hdm::Engine *es,
+                                            uint64_t *list, uint32_t cap, int grid);
 // descs[slots[i]].t_start = t_start for i < n
 extern "C" hipError_t hdm_launch_set_start(hipStream_t s, hdm::PairDesc *descs, const int *slots,
                                            int n, uint32_t t_start);

@@ -227,6 +227,10 @@ int hdm_batch_run(hdm_batch *b);                    /* enqueue one run (async) *
 int hdm_batch_sync(hdm_batch *b, hdm_result *out);  /* wait; out[n] results    */
 /* Device time of the last run, from HIP events around it (ms). */
 float hdm_batch_last_ms(hdm_batch *b);
+/* (region, mask) entries the near pass of the last synced run stepped (the
+ * 64x8x8 tile with the 6 element; pairs whose HD is at most 8 finish there),
+ * 0 when it did not run. */
+uint32_t hdm_batch_near_entries(hdm_batch *b);
 void hdm_batch_destroy(hdm_batch *b);
 
 /* One-shot convenience: create (no graph), run, sync, destroy. */

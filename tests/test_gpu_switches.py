@@ -85,11 +85,13 @@ import torch
 import housdorffhdf5cmake_amd as hdm
 keys = json.loads(sys.argv[2])
 g, s = hdm.synth_device(hdm.synth_spec("ellipsoids", 160, 128, 96, 8, 6, 24, 2, 0.005, 41))
+# a pair beyond the near pass's eight steps, so the batch runs the barriers too
+lg, ls = hdm.synth_device(hdm.synth_spec("long", 128, 96, 128, 30, 64, 48, 20, 15, 12, 60, 4))
 torch.cuda.synchronize()
 def pick(r):
     return {k: r[k] for k in keys}
 def batch_run():
-    b = hdm.Batch([(g, s), (s, g)], graph=False)
+    b = hdm.Batch([(g, s), (s, g), (lg, ls)], graph=False)
     b.run()
     try:
         return [pick(r) for r in b.sync()]
