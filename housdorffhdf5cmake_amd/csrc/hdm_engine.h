@@ -160,6 +160,17 @@ struct PackSums {
   // rows 0..5 the box of the tiles with voxels of either mask (lo x, y, z,
   // hi x, y, z), 6..11 of gold, 12..17 of segm
   int32_t box[18][kSumSlots];
+  // pack steps handed out one at a time past the waves' static shares: one
+  // counter per pool ([32 p], each in a cache line of its own, away from the
+  // partials' atomics)
+  alignas(128) uint32_t grab[8 * 32];
+  // near pass of the single-pair launch: hits per (pass, step), index m + 2k,
+  // spread over kHitSlots by workgroup, and the workgroups' arrivals, so the
+  // last workgroup to arrive folds the counts.  Zeroed with the set, by the
+  // launch before, so a workgroup that arrives early (after a barrier that
+  // timed out) still counts from zero.
+  unsigned long long near_hits[2 * kTbSteps][kHitSlots];
+  uint32_t near_done;
 };
 
 struct EngineHead {
@@ -215,13 +226,8 @@ struct Engine {
   // behind it: word 0 tag(epoch base, 2), words 1 and 2 its clock stamps of
   // the start and the end of the pack
   unsigned long long handoff[3];
-  // near pass of the single-pair launch: hits per (pass, step), index m + 2k,
-  // spread over kHitSlots by workgroup; arrivals per group of workgroups
-  // (blockIdx % kHitSlots) and of groups ([kHitSlots]), so the last workgroup
-  // to arrive folds the counts; workgroup 0's clock stamps of its start, the
-  // end of its pack and the pack barrier, for the engine head
-  unsigned long long near_hits[2 * kTbSteps][kHitSlots];
-  uint32_t near_done[kHitSlots + 1];
+  // near pass of the single-pair launch: workgroup 0's clock stamps of its
+  // start, the end of its pack and the pack barrier, for the engine head
   unsigned long long near_phase[3];
   // batched near pass: entries written to the list (zeroed by the reset)
   uint32_t near_count;

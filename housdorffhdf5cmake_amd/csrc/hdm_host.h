@@ -130,6 +130,9 @@ bool tb_enabled();
 // voxels to be covered), which hands pairs with HD > 8 to the region launch,
 // unless HDM_NEAR=0 (then the two-step launch runs as before)
 bool near_enabled();
+// DilateArgs::pack_pools of the fused launches: 8 (a counter per XCD for the
+// pack's last quarter), or 0 with HDM_PACK_POOLS=0 (even static shares)
+int pack_pools();
 // regions beyond which a pair skips the near launch: every workgroup of it
 // reads the mark bits of every region (16 words per thread at most)
 constexpr uint32_t kNearMaxRegions = 1u << 16;

@@ -220,6 +220,14 @@ static int tb_setting() {
 }
 bool tb_enabled() { return tb_setting() != 0; }
 int tb_mode() { return tb_setting(); }
+int pack_pools() {
+  static const int v = [] {
+    const char *e = std::getenv("HDM_PACK_POOLS");
+    return e && std::atoi(e) == 0 ? 0 : 8;
+  }();
+  return v;
+}
+
 bool near_enabled() {
   static const bool on = [] {
     const char *e = std::getenv("HDM_NEAR");
@@ -615,6 +623,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     ds = tl->fused_ds && conn == 6 && ds_enabled();
     tb = ds && tl->fused_tb && tb_enabled();
     a.tb_mode = tb ? tb_mode() : 0;
+    a.pack_pools = pack_pools();
     if (near_path) {  // fills one set, zeroes the other for the next near launch
       const uint32_t set_words = (uint32_t)(c->marks.cap / 4 / 2);
       a.marks = (uint32_t *)c->marks.p + (size_t)c->marks_set * set_words;

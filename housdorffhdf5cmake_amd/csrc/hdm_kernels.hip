@@ -416,12 +416,18 @@ __device__ __forceinline__ void put(T *p, T v) {
 // FN voxels of a pair whose descriptor carries a near-pass bitmap (marks)
 // sets, by an atomic OR, the bit m * regions + rg of its region rg of
 // 1 x 2 x 2 tiles, for pass m = 0 (FP = segm and not gold, the voxels gold's
-// dilation covers) and m = 1 (FN).
+// dilation covers) and m = 1 (FN).  grab (zeroed counters, 32 words apart,
+// or null) and pools (1 or 8): the waves split three quarters of the steps
+// evenly and take the rest one step at a time from a counter (with 8 pools,
+// the counter of their XCD, over its eighth of those steps), so CUs that
+// stream faster pack more and the pack ends at about the same time on every
+// CU (one pair: npairs 1).
 template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
-                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr) {
+                                           uint16_t (*s_pieces)[1024], PackSums *alt = nullptr,
+                                           uint32_t *grab = nullptr, int pools = 1) {
   static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8), "near marks: 64x8x8 tiles");
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
@@ -441,8 +447,35 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   // holds; ranges start on a step, so the 128-byte path's tile pairs stay
   // x-adjacent
   const uint64_t nsteps = (total_tiles + kStep - 1) / kStep;
-  const uint64_t w0 = min(total_tiles, gw * nsteps / nw * kStep);
-  const uint64_t w1 = min(total_tiles, (gw + 1) * nsteps / nw * kStep);
+  // with grab: a static share of sfix steps per wave, then steps dyn0 + the
+  // counter's values; the next one is asked for while the wave packs its last
+  // static step or its current dynamic one, so the atomic's round trip hides
+  // behind that step's loads
+  const uint64_t sfix = grab ? nsteps * 3 / 4 / nw : 0;
+  uint64_t dyn0 = sfix * nw, dyn1 = nsteps;  // the wave's pool of steps
+  if (grab && pools > 1) {
+    const uint32_t pool = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) % (uint32_t)pools;
+    const uint64_t nd = nsteps - dyn0, b = dyn0;
+    dyn0 = b + nd * pool / (uint32_t)pools;
+    dyn1 = b + nd * (pool + 1) / (uint32_t)pools;
+    grab += 32 * pool;
+  }
+  const uint64_t w0 = grab ? min(total_tiles, gw * sfix * kStep)
+                           : min(total_tiles, gw * nsteps / nw * kStep);
+  uint64_t w1 = grab ? min(total_tiles, (gw + 1) * sfix * kStep)
+                     : min(total_tiles, (gw + 1) * nsteps / nw * kStep);
+  uint32_t gv = 0;     // lane 0: the counter's value for the wave's next step
+  bool asked = false;  // an atomic for it is in flight
+  // the counter's address as a per-lane value (an opaque zero offset): the
+  // compiler then keeps one plain returning atomic for lane 0 instead of
+  // combining the wave's lanes, which reads the result back at once; this
+  // way the wave waits for it only where the value is used
+  uint32_t gz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(gz));
+  auto ask = [&]() {
+    if (lane == 0) gv = atomicAdd(grab + gz, 1u);
+    asked = true;
+  };
   const int slot = (int)(gw % kSumSlots);
   PackAcc acc;
   acc_clear(acc);
@@ -456,7 +489,19 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
     cp = find_pair(npairs, w0, [&](int i) { return idx[i].tile_base; });
     ce = cp + 1 < npairs ? idx[cp + 1].tile_base : total_tiles;
   }
-  for (uint64_t base = w0; base < w1; base += kStep) {
+  for (uint64_t base = w0;;) {
+    // one call site for the atomic: at the last static step, at an empty
+    // static share, and at every dynamic step (which is its range's last)
+    if (grab && !asked && base + kStep >= w1) ask();
+    if (base >= w1) {
+      if (!grab) break;
+      const uint64_t st = dyn0 + (uint32_t)__builtin_amdgcn_readfirstlane((int)gv);
+      asked = false;
+      if (st >= dyn1) break;
+      base = st * kStep;
+      w1 = min(total_tiles, base + kStep);
+      continue;  // asks for the next one, then packs this one
+    }
     bool inU[U], okU[U];
     int pU[U], txU[U], tyU[U], tzU[U], cntU[U];
     uint32_t tileU[U];
@@ -519,6 +564,10 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
           vg[k] = *(const u32x4 *)(gp + nxy * k);
           vs[k] = *(const u32x4 *)(sp + nxy * k);
         }
+        // all sixteen loads in flight before the first compare: one round
+        // trip per step (the scheduler would otherwise start on the first
+        // loads and issue the rest after their wait)
+        __builtin_amdgcn_sched_barrier(0);
         const uint32_t lab4 = (uint32_t)(uint8_t)d.label * 0x01010101u;
         uint16_t *pc = s_pieces[threadIdx.x >> 6];
 #pragma unroll
@@ -669,6 +718,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         }
       }
     }
+    base += kStep;
   }
   if (ap >= 0) {
     if (alt)
@@ -1748,6 +1798,10 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
       PackSums *next = &es->acc[(a.acc_set & 1) ^ 1];
       for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&next->sums[0][0] + i, 0ull);
       for (int i = tid; i < 18 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
+      if (tid < 8) st_sc(&next->grab[32 * tid], 0u);
+      for (int i = tid; i < 2 * kTbSteps * kHitSlots; i += blockDim.x)
+        st_sc(&next->near_hits[0][0] + i, 0ull);
+      if (tid == 0) st_sc(&next->near_done, 0u);
       for (int i = tid; i < d.hist_len; i += blockDim.x)
         st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
       for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
@@ -1755,7 +1809,8 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     // P: pack, every store written through
     {
       __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-      pack_tiles<TXB, TY, TZ, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc);
+      pack_tiles<TXB, TY, TZ, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc,
+                                    a.pack_pools ? &acc->grab[0] : nullptr, a.pack_pools);
       if (blockIdx.x == 0 && threadIdx.x == 0) s_phase[1] = wall_clock64();
       run = grid_sync_tagged(es, a.ebase + 2, spin_limit, &s_ok);
     }
@@ -3708,16 +3763,22 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   if (blockIdx.x == 0) ph0 = wall_clock64();
   // trace (timing 2): one record of 8 words per workgroup -- [0] start, [1]
   // pack done, [2] past the pack barrier, [3] share counted, [4] entries
-  // done, [5] entries taken -- and one more for the fold of the workgroup
-  // that arrives last ([0] fold start, [1] results written), where the trace
-  // holds them
+  // done, [5] entries taken, [6] partials read, [7] HW_ID | XCC_ID << 32 --
+  // and one more for the fold of the workgroup that arrives last ([0] fold
+  // start, [1] results written, [2] hit slots read, [3] fold computed, [4]
+  // ready marker stored), where the trace holds them
   uint64_t *trw = (a.trace && blockIdx.x + 1 < (uint32_t)a.qstats_len)
                       ? a.trace + 8 * (uint64_t)blockIdx.x : nullptr;
-  if (trw && threadIdx.x == 0) trw[0] = wall_clock64();
+  if (trw && threadIdx.x == 0) {
+    trw[0] = wall_clock64();
+    // where the workgroup runs: HW_ID (CU, shader array, engine) and XCC_ID
+    trw[7] = (uint64_t)__builtin_amdgcn_s_getreg((31 << 11) | 4) |
+             (uint64_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) << 32;
+  }
   PackSums *acc = &es->acc[a.acc_set & 1];
   // R: workgroup 0 zeroes the pair's counters and histograms, the engine's
-  // state, the near pass's hit slots and arrival counters, and the pack
-  // partials the next launch packs into (as the two-step launch does); every
+  // state, and the pack partials (with the near pass's hit slots and arrival
+  // counters) the next launch packs into (as the two-step launch does); every
   // workgroup zeroes its share of the region states a hand-off to the region
   // launch starts from and of the mark bitmap the next near launch fills.
   // Nothing here is read before the barrier that ends the pack, except by
@@ -3738,11 +3799,13 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
     PackSums *next = &es->acc[(a.acc_set & 1) ^ 1];
     for (int i = tid; i < 4 * kSumSlots; i += blockDim.x) st_sc(&next->sums[0][0] + i, 0ull);
     for (int i = tid; i < 18 * kSumSlots; i += blockDim.x) st_sc(&next->box[0][0] + i, 0);
+    if (tid < 8) st_sc(&next->grab[32 * tid], 0u);
+    for (int i = tid; i < 2 * kTbSteps * kHitSlots; i += blockDim.x)
+      st_sc(&next->near_hits[0][0] + i, 0ull);
+    if (tid == 0) st_sc(&next->near_done, 0u);
     for (int i = tid; i < d.hist_len; i += blockDim.x)
       st_sc(d.hist[0] + i, 0u), st_sc(d.hist[1] + i, 0u);
     for (int i = tid; i < 4 * kLineWords; i += blockDim.x) st_sc(&es->qlen[0][0] + i, 0u);
-    st_sc(&es->near_hits[0][0] + tid, 0ull);
-    if (tid <= kHitSlots) st_sc(&es->near_done[tid], 0u);
   }
   {
     const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, gn = gridDim.x * blockDim.x;
@@ -3752,7 +3815,8 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   // P: pack, every store written through, regions marked
   {
     __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc);
+    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc,
+                                     a.pack_pools ? &acc->grab[0] : nullptr, a.pack_pools);
   }
   if (blockIdx.x == 0) ph1 = wall_clock64();
   if (trw && threadIdx.x == 0) trw[1] = wall_clock64();
@@ -3797,6 +3861,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) st_sc(&c0.lo[k], bx[k]), st_sc(&c0.hi[k], bx[3 + k]);
       }
+      if (trw) trw[6] = wall_clock64();
     }
   }
   __syncthreads();
@@ -3840,16 +3905,13 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
       __threadfence_system();
       __hip_atomic_store(a.ready, a.ready_seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
+    if (trm) trm[4] = wall_clock64();
   };
-  if (s_hand) {  // every workgroup takes the same decision from the same partials
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      if (ok)
-        publish(true, false, 0, nullptr, nullptr);
-      else
-        publish(false, true, 0, nullptr, nullptr);  // the pack barrier timed out
-    }
-    return;
-  }
+  // Entries only past a barrier that held and for a pair that stays: every
+  // workgroup whose barrier held takes the same decision from the same
+  // partials.  Every workgroup arrives below whatever it decided, so the one
+  // that arrives last publishes (the hand-off, an error, or the results).
+  const bool work = ok && !s_hand;
 
   // One entry: mark bit wi = region rg of pass m.
   auto near_entry = [&](uint32_t wi) {
@@ -3871,7 +3933,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
 #pragma unroll
   for (uint32_t q = 0; q < kNearWordsPerThread; ++q) {
     const uint32_t b0 = (threadIdx.x * cw + q) * 32;  // bits past W never get set, but drop them
-    if (b0 >= W)
+    if (b0 >= W || !work)
       mw[q] = 0;
     else if (W - b0 < 32)
       mw[q] &= (1u << (W - b0)) - 1u;
@@ -3912,44 +3974,47 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
     for (uint32_t j = 0; j < ce - cs; ++j) near_entry(s_list[j]);
   }
   if (trw && threadIdx.x == 0) trw[4] = wall_clock64();
-  // Arrival: the workgroup adds its hits into its slots and counts itself in
-  // its group; the last of a group counts the group in, and the workgroup
-  // that completes the last group folds.  Every workgroup's adds are done
-  // (s_waitcnt 0) before its count, so the one that folds reads them all.
+  // Arrival: the workgroup adds its hits into its slots and counts itself;
+  // the last to arrive folds.  Every workgroup's adds are done (s_waitcnt 0)
+  // before its count, so the one that folds reads them all.
   const uint32_t grp = blockIdx.x % kHitSlots;
   if (threadIdx.x < 2 * KS && s_hits[threadIdx.x])
-    atomicAdd(&es->near_hits[threadIdx.x][grp], (unsigned long long)s_hits[threadIdx.x]);
+    atomicAdd(&acc->near_hits[threadIdx.x][grp], (unsigned long long)s_hits[threadIdx.x]);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t members = (gridDim.x - grp + kHitSlots - 1) / kHitSlots;
-    const uint32_t groups = min(gridDim.x, (uint32_t)kHitSlots);
-    int last = 0;
-    if (atomicAdd(&es->near_done[grp], 1u) == members - 1)
-      last = atomicAdd(&es->near_done[kHitSlots], 1u) == groups - 1;
-    s_last = last;
-  }
+  if (threadIdx.x == 0) s_last = atomicAdd(&acc->near_done, 1u) == gridDim.x - 1;
   __syncthreads();
   if (!s_last) return;
 
-  // the fold, by the workgroup that arrived last
+  // the fold, by the workgroup that arrived last: the hit slots and the
+  // error flag in one round trip, the slots summed per bin in LDS, then
+  // thread 0 folds from registers
   uint64_t *trm = (a.trace && gridDim.x + 1 < (uint32_t)a.qstats_len)
                       ? a.trace + 8 * (uint64_t)gridDim.x : nullptr;
   if (trm && threadIdx.x == 0) trm[0] = wall_clock64();
+  uint32_t eflag = 0;
   {
-    const unsigned long long v = ld_sc(&es->near_hits[0][0] + threadIdx.x);
+    const unsigned long long v = ld_sc(&acc->near_hits[0][0] + threadIdx.x);
+    if (threadIdx.x == 0) eflag = ld_sc(&es->error);
     if (v) atomicAdd(&s_h[threadIdx.x / kHitSlots], v);
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  const bool err = !ok || ld_sc(&es->error) != 0;
+  if (trm) trm[2] = wall_clock64();
+  const bool err = !ok || eflag != 0;
+  if (err || s_hand) {  // a barrier timed out, or the pair goes to the region launch
+    publish(!err, err, 0, nullptr, nullptr);
+    return;
+  }
   const unsigned long long nf[2] = {s_tot[2], s_tot[3]};
   unsigned long long cut[2], cov[2] = {0, 0}, sum[2] = {0, 0};
   uint32_t hd[2] = {0, 0}, taken[2] = {0, 0};
   int dfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
   bool done = true;
+#pragma unroll
   for (int m = 0; m < 2; ++m) {
     cut[m] = robust_cut(d.robust, nf[m]);
+#pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int it = k + 1;
       const unsigned long long hm = s_h[m + 2 * k];
@@ -3969,13 +4034,16 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   }
   // the histogram bins the fold took (hello.cu:1618 numbering), then the
   // results from the fold's own registers
+#pragma unroll
   for (int m = 0; m < 2; ++m) {
     uint32_t *hp = m ? d.hist[1] : d.hist[0];
+#pragma unroll
     for (int k = 0; k < KS; ++k)
       if (((taken[m] >> k) & 1u) && k + 1 < d.hist_len) st_sc(hp + k + 1, (uint32_t)s_h[m + 2 * k]);
   }
   const PairOut o = finalize_from(d, s_tot[0], s_tot[1], s_tot[2], s_tot[3], cov, sum, hd, dfrom,
                                   0, 1, err, KS);
+  if (trm) trm[3] = wall_clock64();
   publish(false, err, total, trm, &o);
 }
 

@@ -215,6 +215,7 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
   const bool near_path = tb && tl->fused_near && near_enabled() && !d.map &&
                          g.nregions <= kNearMaxRegions;
   a.tb_mode = tb ? tb_mode() : 0;
+  a.pack_pools = pack_pools();
   if (near_path) {
     a.marks = marks + (size_t)D.marks_set * kMarkSetWords;
     a.marks_zero = marks + (size_t)(D.marks_set ^ 1) * kMarkSetWords;

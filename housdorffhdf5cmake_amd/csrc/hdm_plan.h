@@ -65,6 +65,11 @@ struct DilateArgs {
   // near launch (marks_zero_words words)
   uint32_t *marks;
   uint32_t *marks_zero;
+  // fused launch: the pack's last quarter of steps taken one at a time from
+  // counters (0: every wave packs an even static share; 1: one counter; 8:
+  // one counter per XCD, each over its own eighth of those steps; one counter
+  // for the whole chip serializes its atomics and runs the pack at half speed)
+  int32_t pack_pools;
   uint32_t marks_zero_words;
 };
 
