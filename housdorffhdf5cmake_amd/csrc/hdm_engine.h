@@ -207,13 +207,15 @@ struct Engine {
   uint32_t rel[kReleaseCopies][kLineWords];
   // Single-pair barrier: every word carries its epoch in the top 24 bits
   // (tag()), so a reader that sees the right tag has the value that goes with
-  // it and no ordering between words is needed.  arr[wg]: this iteration's
-  // hits of pass 0 and 1 in its first dilation step, the entries the
-  // workgroup pushed and kept, then the hits of pass 0 and 1 in each further
-  // step of the iteration (two or eight steps per barrier); rel2[copy]:
-  // word 0 flags (bit m: pass m done, bit 2: error), words 1..kSubQ+1 prefix
-  // offsets of the next queue.
-  unsigned long long arr[kMaxGrid][kArrWords];
+  // it and no ordering between words is needed.  arr[j][wg], word j of
+  // workgroup wg's record: this iteration's hits of pass 0 and 1 in its first
+  // dilation step, the entries the workgroup pushed and kept, then the hits
+  // of pass 0 and 1 in each further step of the iteration (two or eight steps
+  // per barrier).  Word-major, so the master's poll of word j over 64
+  // consecutive workgroups is one load of four cache lines, not 64 requests.
+  // rel2[copy]: word 0 flags (bit m: pass m done, bit 2: error), words
+  // 1..kSubQ+1 prefix offsets of the next queue.
+  unsigned long long arr[kArrWords][kMaxGrid];
   unsigned long long rel2[kReleaseCopies][kLineWords];
   // tag(epoch, first global entry) of each workgroup's free wave slots: the
   // master's prefix over workgroups of (slots - entries the workgroup kept)

@@ -661,9 +661,13 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
           if (++c->seq == 0) ++c->seq;
           a.ready_seq = c->seq;
         }
-        e = tl->fused_tb(
-            dim3(tb_grid(c->occ_fused_tb[tile], c->cus, opts.grid_ctas, g.ntiles, g.nregions)),
-            s, a);
+        // a traced call: the region launch's per-period records start clear
+        // (the first launch wrote per-workgroup records into the same buffer)
+        if (cfg.trace) e = hipMemsetAsync(cfg.trace, 0, trace_bytes, s);
+        if (e == hipSuccess)
+            e = tl->fused_tb(
+              dim3(tb_grid(c->occ_fused_tb[tile], c->cus, opts.grid_ctas, g.ntiles, g.nregions)),
+              s, a);
         if (e == hipSuccess && cfg.ev) e = hipEventRecord(cfg.ev[3], s);
       }
     }

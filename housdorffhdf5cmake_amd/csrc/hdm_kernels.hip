@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
     for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel[0][i] = 0;
     constexpr int kArrWords = (int)(sizeof(es->arr) / sizeof(es->arr[0][0]));
-    for (int i = g; i < kArrWords; i += gs) es->arr[0][i] = 0;
+    for (int i = g; i < kArrWords; i += gs) (&es->arr[0][0])[i] = 0;
     for (int i = g; i < kMaxGrid; i += gs) es->fslot[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel2[0][i] = 0;
   }
@@ -1325,7 +1325,7 @@ __device__ __forceinline__ bool grid_barrier_single(
   if (threadIdx.x == 0) {
     *s_ok = 1;
 #pragma unroll
-    for (int j = 0; j < kArr; ++j) st_sc(&es->arr[blockIdx.x][j], tag(epoch, s_cnt[j]));
+    for (int j = 0; j < kArr; ++j) st_sc(&es->arr[j][blockIdx.x], tag(epoch, s_cnt[j]));
 #pragma unroll
     for (int j = 0; j < kArr; ++j) s_cnt[j] = 0;  // the next iteration's counts start here
     *s_zero = 0;
@@ -1353,7 +1353,7 @@ __device__ __forceinline__ bool grid_barrier_single(
         for (int k = 0; k < kPoll; ++k) {
           if ((pending >> k) & 1) {
 #pragma unroll
-            for (int j = 0; j < kArr; ++j) w[k][j] = ld_sc(&es->arr[i0 + k * blockDim.x][j]);
+            for (int j = 0; j < kArr; ++j) w[k][j] = ld_sc(&es->arr[j][i0 + k * blockDim.x]);
           }
         }
 #pragma unroll
@@ -1515,7 +1515,7 @@ __device__ __forceinline__ bool grid_sync_tagged(Engine *es, uint32_t epoch, uin
   __syncthreads();
   if (threadIdx.x == 0) {
     *s_ok = 1;
-    st_sc(&es->arr[blockIdx.x][0], tag(epoch, 0));
+    st_sc(&es->arr[0][blockIdx.x], tag(epoch, 0));
   }
   if (blockIdx.x == 0) {
     int ok = 1;
@@ -1530,7 +1530,7 @@ __device__ __forceinline__ bool grid_sync_tagged(Engine *es, uint32_t epoch, uin
         unsigned long long w[kPoll];
 #pragma unroll
         for (int k = 0; k < kPoll; ++k)
-          w[k] = ((pending >> k) & 1) ? ld_sc(&es->arr[i0 + k * blockDim.x][0]) : 0ull;
+          w[k] = ((pending >> k) & 1) ? ld_sc(&es->arr[0][i0 + k * blockDim.x]) : 0ull;
 #pragma unroll
         for (int k = 0; k < kPoll; ++k)
           if (((pending >> k) & 1) && tagged(w[k], epoch)) pending &= ~(1u << k);
