@@ -1277,6 +1277,55 @@ struct PassState {
   uint32_t qmax;
 };
 
+// The fold of one barrier period's hit counts, one bin per lane of a wave
+// (all 64 lanes call it, the state arrays wave-uniform; lane l < 2 KS takes
+// bin l: pass m = l % 2, step it0 + l / 2).  A pass whose cut is reached at
+// step s takes the bins up to s and stops (hello.cu:1748-1759), and each
+// taken bin goes to the histogram (bins at hist_len and beyond are dropped):
+// the loop of that rule as prefix sums and ballots, with no branch per bin.
+template <int KS, typename Bin>
+__device__ __forceinline__ void fold_bins(const unsigned long long *cut, unsigned long long *cov,
+                                          unsigned long long *sum, uint32_t *hd, int *done_from,
+                                          int it0, Bin bin, uint32_t *hist0, uint32_t *hist1,
+                                          int hist_len) {
+  static_assert(2 * KS <= 64, "one bin per lane");
+  const int lane = threadIdx.x & 63;
+  const int m = lane & 1, k = lane >> 1;
+  const bool mine = lane < 2 * KS;
+  const int it = it0 + k;
+  // a pass folds unless its cut is zero or an earlier period reached it
+  const bool act0 = cut[0] != 0 && done_from[0] >= it0;
+  const bool act1 = cut[1] != 0 && done_from[1] >= it0;
+  const bool active = mine && (m ? act1 : act0);
+  const unsigned long long h = mine ? bin(lane) : 0ull;
+  unsigned long long pre = m ? cov[1] : cov[0];  // coverage through this bin
+#pragma unroll
+  for (int j = 0; j < KS; ++j)
+    if (j <= k) pre += bin(m + 2 * j);
+  const uint64_t R = __ballot(active && pre >= (m ? cut[1] : cut[0]));
+  constexpr uint64_t kPass0 = ((1ull << (2 * KS)) - 1) & 0x5555555555555555ull;
+  const uint64_t Rm = R & (m ? kPass0 << 1 : kPass0);
+  const int kstar = Rm ? (int)(__builtin_ctzll(Rm) >> 1) : KS;  // the bin reaching the cut
+  const bool taken = active && k <= kstar && h != 0;
+  const uint64_t T = __ballot(taken);
+  if (taken && it < hist_len) st_sc((m ? hist1 : hist0) + it, (uint32_t)h);
+  const unsigned long long term = taken ? h * (unsigned long long)it : 0ull;
+  const unsigned long long s0 = wave_sum64(m == 0 ? term : 0ull);
+  const unsigned long long s1 = wave_sum64(m == 1 ? term : 0ull);
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if (!(p ? act1 : act0)) continue;
+    const uint64_t Rp = R & (kPass0 << p), Tp = T & (kPass0 << p);
+    const int ks = Rp ? (int)(__builtin_ctzll(Rp) >> 1) : KS - 1;
+    const int src = p + 2 * ks;
+    cov[p] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pre, src) |
+             ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pre >> 32), src) << 32);
+    sum[p] += p ? s1 : s0;
+    if (Tp) hd[p] = (uint32_t)(it0 + (int)((63 - __builtin_clzll(Tp)) >> 1));
+    if (Rp) done_from[p] = it0 + ks;
+  }
+}
+
 // Grid barrier of a single-pair launch.  It has the same structure as
 // grid_barrier, but every word that crosses workgroups carries its epoch
 // (tag()), so neither side needs a second round trip:
@@ -1419,6 +1468,13 @@ __device__ __forceinline__ bool grid_barrier_single(
       }
       if (threadIdx.x == 0) s_fs[0] = 0, s_fs[1] = total;  // workgroup 0's range starts at 0
     }
+    // the fold by wave 0 (whose lanes all hold the pass state): this engine
+    // iteration covered dilation steps KS tp + 1 .. KS tp + KS
+    if (good && wave == 0)
+      fold_bins<KS>(ps.cut, ps.cov, ps.sum, ps.hd, ps.done_from,
+                    KS * (int)(tdone - d.t_start) + 1,
+                    [&](int i) { return s_hw[i][0] + s_hw[i][1] + s_hw[i][2] + s_hw[i][3]; },
+                    d.hist0, d.hist1, d.hist_len);
     if (threadIdx.x == 0) {
       uint32_t flags = 0;
       uint32_t acc = 0;
@@ -1430,22 +1486,6 @@ __device__ __forceinline__ bool grid_barrier_single(
       if (good) {
         const uint32_t kept = (uint32_t)(s_red[8] + s_red[9] + s_red[10] + s_red[11]);
         const int tp = (int)(tdone - d.t_start);
-        // this engine iteration covered dilation steps KS*tp+1 .. KS*tp+KS
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-          for (int k = 0; k < KS; ++k) {
-            const int it = KS * tp + 1 + k;  // the dilation step of this bin
-            const int i = m + 2 * k;
-            const unsigned long long hm = s_hw[i][0] + s_hw[i][1] + s_hw[i][2] + s_hw[i][3];
-            if (ps.cut[m] == 0 || ps.done_from[m] < it || hm == 0) continue;
-            uint32_t *hp = m ? d.hist1 : d.hist0;
-            if (it < d.hist_len) hp[it] = (uint32_t)hm;
-            ps.cov[m] += hm;
-            ps.sum[m] += hm * (unsigned long long)it;
-            ps.hd[m] = (uint32_t)it;
-            if (ps.cov[m] >= ps.cut[m]) ps.done_from[m] = it;
-          }
-        }
 #pragma unroll
         for (int m = 0; m < 2; ++m)
           if (ps.cut[m] == 0 || ps.done_from[m] <= KS * tp + KS) flags |= 1u << m;
@@ -1917,7 +1957,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
         return;
       }
       v0 = make_view(d, s_box, s_box + 3, s_tot[2], s_tot[3]);
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (blockIdx.x == 0 && threadIdx.x < 64) {  // wave 0 holds the pass state
         for (int m = 0; m < 2; ++m) {
           ps.cut[m] = v0.cut[m];
           ps.cov[m] = ps.sum[m] = 0;
@@ -1974,7 +2014,7 @@ __device__ __forceinline__ void dilate_body(const DilateArgs &a) {
     }
   } else {
     if constexpr (SINGLE) v0 = make_view(descs[0], ctrs[0]);
-    if (SINGLE && blockIdx.x == 0 && threadIdx.x == 0) {  // the master's registers
+    if (SINGLE && blockIdx.x == 0 && threadIdx.x < 64) {  // the master's registers (wave 0)
       const PairCtr &c0 = ctrs[0];
       for (int m = 0; m < 2; ++m) {
         ps.cut[m] = robust_cut(descs[0].robust, ld_sc(m == 0 ? &c0.n_fp : &c0.n_fn));
@@ -2894,10 +2934,13 @@ __device__ __forceinline__ bool tb_tile_in_box(const Box &b, int tx, int ty, int
 
 // The single-pair dilation of a pair the two-step launch handed over after
 // its pack: region seed merged with the first eight steps, then periods of
-// eight steps between grid barriers, then the results.  Held to two
-// resident workgroups per CU (240 VGPRs, no spill).
+// eight steps between grid barriers, then the results.  One workgroup per
+// CU: with two, the 256 VGPRs a wave could hold spilled to scratch memory
+// (the master's fold and barrier among them); with one, the overflow goes to
+// AGPRs, and a long pair's periods run about 5% faster -- a sparse front has
+// fewer regions than CUs anyway.
 __global__ __launch_bounds__(kWgThreads)
-__attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
+__attribute__((amdgpu_waves_per_eu(1))) void dilate_tb_kernel(DilateArgs a) {
   constexpr int KS = kTbSteps;
   constexpr int R = 64;  // rows per tile
   PairCtr *ctrs = a.ctrs;
@@ -2992,7 +3035,7 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_tb_kernel(DilateArgs a) {
   __syncthreads();
   const PairView v0 = make_view(d, s_box, s_box + 3, s_tot[2], s_tot[3]);
   const Box &bb = v0.b;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blockIdx.x == 0 && threadIdx.x < 64) {  // wave 0 holds the pass state
     for (int m = 0; m < 2; ++m) {
       ps.cut[m] = v0.cut[m];
       ps.cov[m] = ps.sum[m] = 0;
@@ -3990,59 +4033,41 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
   if (!s_last) return;
 
   // the fold, by the workgroup that arrived last: the hit slots and the
-  // error flag in one round trip, the slots summed per bin in LDS, then
-  // thread 0 folds from registers
+  // error flag in one round trip, the slots summed per bin in LDS, then wave
+  // 0 folds the bins (one per lane) and thread 0 publishes
   uint64_t *trm = (a.trace && gridDim.x + 1 < (uint32_t)a.qstats_len)
                       ? a.trace + 8 * (uint64_t)gridDim.x : nullptr;
   if (trm && threadIdx.x == 0) trm[0] = wall_clock64();
   uint32_t eflag = 0;
   {
     const unsigned long long v = ld_sc(&acc->near_hits[0][0] + threadIdx.x);
-    if (threadIdx.x == 0) eflag = ld_sc(&es->error);
+    if (threadIdx.x < 64) eflag = ld_sc(&es->error);
     if (v) atomicAdd(&s_h[threadIdx.x / kHitSlots], v);
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  if (trm) trm[2] = wall_clock64();
+  if (threadIdx.x >= 64) return;
+  if (trm && threadIdx.x == 0) trm[2] = wall_clock64();
   const bool err = !ok || eflag != 0;
   if (err || s_hand) {  // a barrier timed out, or the pair goes to the region launch
-    publish(!err, err, 0, nullptr, nullptr);
+    if (threadIdx.x == 0) publish(!err, err, 0, nullptr, nullptr);
     return;
   }
-  const unsigned long long nf[2] = {s_tot[2], s_tot[3]};
+  // bins 1 .. 8 with the cut (hello.cu:1618 numbering); the bins taken go to
+  // the histograms -- a pair handed over below has the region launch write
+  // the same values into the same bins again
   unsigned long long cut[2], cov[2] = {0, 0}, sum[2] = {0, 0};
-  uint32_t hd[2] = {0, 0}, taken[2] = {0, 0};
+  uint32_t hd[2] = {0, 0};
   int dfrom[2] = {0x7FFFFFFF, 0x7FFFFFFF};
-  bool done = true;
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    cut[m] = robust_cut(d.robust, nf[m]);
-#pragma unroll
-    for (int k = 0; k < KS; ++k) {
-      const int it = k + 1;
-      const unsigned long long hm = s_h[m + 2 * k];
-      if (cut[m] == 0 || dfrom[m] < it || hm == 0) continue;
-      taken[m] |= 1u << k;
-      cov[m] += hm;
-      sum[m] += hm * (unsigned long long)it;
-      hd[m] = (uint32_t)it;
-      if (cov[m] >= cut[m]) dfrom[m] = it;
-    }
-    done = done && (cut[m] == 0 || cov[m] >= cut[m]);
-  }
+  cut[0] = robust_cut(d.robust, s_tot[2]);
+  cut[1] = robust_cut(d.robust, s_tot[3]);
+  fold_bins<KS>(cut, cov, sum, hd, dfrom, 1, [&](int i) { return s_h[i]; }, d.hist[0], d.hist[1],
+                d.hist_len);
+  if (threadIdx.x != 0) return;
+  const bool done = (cut[0] == 0 || cov[0] >= cut[0]) && (cut[1] == 0 || cov[1] >= cut[1]);
   const bool empty = (s_tot[2] && !s_tot[0]) || (s_tot[3] && !s_tot[1]);
-  if (!done && !empty && !err) {  // HD beyond eight steps: the region launch computes it
+  if (!done && !empty) {  // HD beyond eight steps: the region launch computes it
     publish(true, false, 0, nullptr, nullptr);
     return;
-  }
-  // the histogram bins the fold took (hello.cu:1618 numbering), then the
-  // results from the fold's own registers
-#pragma unroll
-  for (int m = 0; m < 2; ++m) {
-    uint32_t *hp = m ? d.hist[1] : d.hist[0];
-#pragma unroll
-    for (int k = 0; k < KS; ++k)
-      if (((taken[m] >> k) & 1u) && k + 1 < d.hist_len) st_sc(hp + k + 1, (uint32_t)s_h[m + 2 * k]);
   }
   const PairOut o = finalize_from(d, s_tot[0], s_tot[1], s_tot[2], s_tot[3], cov, sum, hd, dfrom,
                                   0, 1, err, KS);
