@@ -233,6 +233,9 @@ struct Engine {
   unsigned long long near_phase[3];
   // batched near pass: entries written to the list (zeroed by the reset)
   uint32_t near_count;
+  // the multi-pair pack's step counters, one per XCD in a cache line of its
+  // own (zeroed by the reset; see PackSums::grab)
+  alignas(128) uint32_t pack_grab[8 * 32];
 };
 constexpr int kTagShift = 40;
 constexpr unsigned long long kTagValue = (1ull << kTagShift) - 1;

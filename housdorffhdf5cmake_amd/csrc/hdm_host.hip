@@ -318,7 +318,10 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
                                   P.engine, 1, qcap);
   if (e != hipSuccess || (e = mark(0)) != hipSuccess) return e;
   (void)total_rw;
-  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_tiles);
+  // several pairs: the last quarter of the steps from the engine's per-XCD
+  // counters, which the reset above zeroed
+  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_tiles,
+               npairs > 1 && pack_pools() ? &P.engine->pack_grab[0] : nullptr);
   if (e != hipSuccess) return e;
   if (npairs > 1 && (e = hdm_launch_totals(s, P.ctrs, 0, npairs)) != hipSuccess) return e;
   if (cfg.near_list && npairs > 1 &&

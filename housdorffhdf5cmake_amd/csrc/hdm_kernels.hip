@@ -239,6 +239,7 @@ __global__ __launch_bounds__(kWgThreads) void reset_kernel(
       es->qnext = 0;
       es->near_count = 0;
     }
+    if (g < 8) es->pack_grab[32 * g] = 0;
     for (int i = g; i < 4 * kLineWords; i += gs) es->qlen[0][i] = 0;
     for (int i = g; i < kMaxGrid; i += gs) es->arrive[i] = 0;
     for (int i = g; i < kReleaseCopies * kLineWords; i += gs) es->rel[0][i] = 0;
@@ -421,7 +422,7 @@ __device__ __forceinline__ void put(T *p, T v) {
 // evenly and take the rest one step at a time from a counter (with 8 pools,
 // the counter of their XCD, over its eighth of those steps), so CUs that
 // stream faster pack more and the pack ends at about the same time on every
-// CU (one pair: npairs 1).
+// CU.
 template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
@@ -500,6 +501,10 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       if (st >= dyn1) break;
       base = st * kStep;
       w1 = min(total_tiles, base + kStep);
+      if (npairs > 1) {  // a jump past any number of pairs: find the pair again
+        cp = find_pair(npairs, base, [&](int i) { return idx[i].tile_base; });
+        ce = cp + 1 < npairs ? idx[cp + 1].tile_base : total_tiles;
+      }
       continue;  // asks for the next one, then packs this one
     }
     bool inU[U], okU[U];
@@ -735,11 +740,12 @@ template <int TXB, int TY, int TZ>
 __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs,
-                                                          uint64_t total_tiles) {
+                                                          uint64_t total_tiles, uint32_t *grab) {
   // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
   __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
   constexpr bool kMark = TXB == 64 && TY == 8 && TZ == 8;  // pairs with a bitmap get marks
-  pack_tiles<TXB, TY, TZ, false, kMark>(descs, idx, ctrs, npairs, total_tiles, s_pieces);
+  pack_tiles<TXB, TY, TZ, false, kMark>(descs, idx, ctrs, npairs, total_tiles, s_pieces, nullptr,
+                                        grab, 8);
 }
 
 // The pack pass of a pair whose two masks were packed before, each on its own
@@ -4223,8 +4229,9 @@ __global__ __launch_bounds__(kWgThreads) void near_fold_kernel(const PairDesc *_
 // ---------------------------------------------------------------- launchers
 template <int TXB, int TY, int TZ>
 hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
-                       PairCtr *c, int np, uint64_t tw) {
-  hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw);
+                       PairCtr *c, int np, uint64_t tw, uint32_t *grab) {
+  hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw,
+                     grab);
   return hipGetLastError();
 }
 template <int TXB, int TY, int TZ>

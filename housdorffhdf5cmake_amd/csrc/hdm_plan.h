@@ -74,8 +74,9 @@ struct DilateArgs {
 };
 
 struct TileLaunch {
+  // the last argument: the pack's step counters (Engine::pack_grab, zeroed) or null
   hipError_t (*pack)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
-                     uint64_t);
+                     uint64_t, uint32_t *);
   hipError_t (*seed)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
                      Engine *, uint64_t *, uint32_t, uint64_t, int);
   hipError_t (*dilate)(dim3, hipStream_t, const DilateArgs &);

@@ -375,7 +375,7 @@ extern "C" int hdm_server_register(hdm_server *s, const void *vol, int nx, int n
     e = hdm_launch_reset(s->s_adm, ddev, &d, &pi, idev, cdev, 0, 1, nullptr, 0, 0);
   if (e == hipSuccess) {
     const LaunchCfg lc = launch_cfg(s->ctx, s->tile, g.row_words, g.ntiles, 0, 1, 1);
-    e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, ddev, idev, cdev, 1, g.ntiles);
+    e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, ddev, idev, cdev, 1, g.ntiles, nullptr);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s->s_adm);
   if (in_bytes) s->arena.release(in_off, in_bytes);
@@ -474,7 +474,7 @@ static int stage(hdm_server *s, Request &r, int p) {
                        mg->tflags, ms->tflags, r.geom.ntiles);
   else
     e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, s->P.descs + p, s->P.idx + p,
-                    s->P.ctrs + p, 1, r.geom.ntiles);
+                    s->P.ctrs + p, 1, r.geom.ntiles, nullptr);
   if (e != hipSuccess) return map_hip(e);
   r.staged = take_event(s);
   if (!r.staged || hipEventRecord(r.staged, s->s_adm) != hipSuccess) return HDM_E_HIP;

@@ -47,6 +47,28 @@ def test_batch_equals_single_and_oracle():
         assert rc == 0 and all(int(want[k]) == int(a[k]) for k in KEYS)
 
 
+@pytest.mark.parametrize("robust", [1.0, 0.9])
+def test_batch_near_pass_with_short_and_long_pairs(robust):
+    """The batch's near pass finishes the pairs whose cut is reached within
+    eight steps and leaves the others to the seed and the dilation: a mix of
+    both equals every pair alone on two graph replays, and the entries the
+    near pass stepped are reported."""
+    short = mixed_pairs(4, seed=51)
+    lg = hdm.synth_device(hdm.synth_spec("long", 128, 96, 128, 30, 64, 48, 20, 15, 12, 60, 4))
+    torch.cuda.synchronize()
+    vols = [short[0], lg, short[1], short[2], (lg[1], lg[0]), short[3]]
+    alone = [hdm.hausdorff(g, s, robustness=robust) for g, s in vols]
+    assert max(a["hd"] for a in alone) > 8 and min(a["hd"] for a in alone) <= 8
+    b = hdm.Batch(vols, robustness=robust, graph=True)
+    for _ in range(2):
+        b.run()
+        out = b.sync()
+        for o, a in zip(out, alone):
+            assert same(o, a)
+        assert b.near_entries() > 0
+    b.close()
+
+
 @pytest.mark.parametrize("robust", [0.5, 0.95])
 def test_batch_robust_cut_equals_single_and_oracle(robust):
     """A batch runs two dilation steps per grid barrier (64x8x8 tiles, 6
