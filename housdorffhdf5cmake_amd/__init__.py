@@ -85,7 +85,8 @@ class ServerOpts(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("slices", C.c_uint64), ("pair_slices", C.c_uint64), ("max_in_flight", C.c_int32),
                 ("reserved", C.c_int32), ("lone", C.c_uint64), ("admitted", C.c_uint64),
-                ("registered_bytes", C.c_uint64)]
+                ("registered_bytes", C.c_uint64), ("near_finished", C.c_uint64),
+                ("groups", C.c_uint64), ("group_pairs", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_ if k != "reserved"}

@@ -325,7 +325,8 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   if (e != hipSuccess) return e;
   if (npairs > 1 && (e = hdm_launch_totals(s, P.ctrs, 0, npairs)) != hipSuccess) return e;
   if (cfg.near_list && npairs > 1 &&
-      (e = hdm_launch_near_batch(s, P.descs, P.idx, P.ctrs, npairs, cfg.near_words, P.engine,
+      (e = hdm_launch_near_batch(s, P.descs, P.idx, P.ctrs, npairs, cfg.near_words,
+                                 &P.engine->near_count,
                                  cfg.near_list, cfg.near_cap, cfg.near_grid)) != hipSuccess)
     return e;
   if ((e = mark(1)) != hipSuccess) return e;

@@ -402,6 +402,18 @@ __device__ __forceinline__ void acc_flush(S &c, PackAcc &a, int lane, int slot) 
 // lane and are published once per (wave, pair) into spread slots.
 // store that other workgroups of the same launch read later (COH: agent-
 // coherent, written through) or that only later launches read (plain)
+// The near-pass mark of a 64x8x8 tile (tx, ty, tz) that holds FP (fp) or FN
+// (fn) voxels: bit m * regions + rg of the pair's bitmap, for its region rg
+// of 1 x 2 x 2 tiles (pass 0: FP, pass 1: FN), by an atomic OR.
+__device__ __forceinline__ void mark_tile(const PairDesc &d, int tx, int ty, int tz, bool fp,
+                                          bool fn) {
+  const uint32_t rgx = (uint32_t)d.gx, rgy = (uint32_t)((d.gy + 1) / 2);
+  const uint32_t nreg = pair_regions(d);
+  const uint32_t rg = (uint32_t)tx + rgx * ((uint32_t)(ty >> 1) + rgy * (uint32_t)(tz >> 1));
+  if (fp) or_g(d.marks + (rg >> 5), 1u << (rg & 31));
+  if (fn) or_g(d.marks + ((nreg + rg) >> 5), 1u << ((nreg + rg) & 31));
+}
+
 template <bool COH, typename T>
 __device__ __forceinline__ void put(T *p, T v) {
   if constexpr (COH)
@@ -685,14 +697,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const uint32_t c2 = popc_row((Row)(b & ~a)), c3 = popc_row((Row)(a & ~b));
       if constexpr (MARK) {  // one tile per wave: the ballots cover the tile
         const bool fp = __ballot(c2 != 0) != 0, fn = __ballot(c3 != 0) != 0;
-        if (in && r == 0 && (fp || fn) && d.marks) {
-          const uint32_t rgx = (uint32_t)d.gx, rgy = (uint32_t)((d.gy + 1) / 2);
-          const uint32_t nreg = pair_regions(d);
-          const uint32_t rg = (uint32_t)txU[u] +
-                              rgx * ((uint32_t)(tyU[u] >> 1) + rgy * (uint32_t)(tzU[u] >> 1));
-          if (fp) or_g(d.marks + (rg >> 5), 1u << (rg & 31));
-          if (fn) or_g(d.marks + ((nreg + rg) >> 5), 1u << ((nreg + rg) & 31));
-        }
+        if (in && r == 0 && (fp || fn) && d.marks) mark_tile(d, txU[u], tyU[u], tzU[u], fp, fn);
       }
       const bool ne = (a | b) != 0;
       if (p == ap) {
@@ -789,6 +794,10 @@ __global__ __launch_bounds__(kWgThreads) void combine_kernel(const PairDesc *__r
     const Row b = rowok && (sf & 1u) ? ((const Row *)d.orig[1])[o] : (Row)0;
     acc.c[0] += popc_row(a), acc.c[1] += popc_row(b);
     acc.c[2] += popc_row((Row)(b & ~a)), acc.c[3] += popc_row((Row)(a & ~b));
+    if constexpr (TXB == 64 && TY == 8 && TZ == 8) {  // one tile per wave: near-pass marks
+      const bool fp = __ballot((b & ~a) != 0) != 0, fn = __ballot((a & ~b) != 0) != 0;
+      if (in && r == 0 && (fp || fn) && d.marks) mark_tile(d, tx, ty, tz, fp, fn);
+    }
     if ((a | b) != 0) {
       acc.mn[0] = min(acc.mn[0], tx), acc.mx[0] = max(acc.mx[0], tx);
       acc.mn[1] = min(acc.mn[1], ty), acc.mx[1] = max(acc.mx[1], ty);
@@ -4320,6 +4329,61 @@ TileLaunch make_launch() {
                     DS ? 2 : 1};
 }
 
+// ---------------------------------------------------------------- server admission groups
+// The outcome of a server admission group's near pass, one thread per pair:
+// a pair the near pass finished (or one with an empty mask) gets its
+// results, the others status 2 (they join the engine); then the ready
+// marker behind a system-scope fence, so the host takes the outcomes
+// without waiting for the stream.
+__global__ __launch_bounds__(kWgThreads) void group_out_kernel(const PairDesc *__restrict__ descs,
+                                                               const PairCtr *__restrict__ ctrs,
+                                                               int n, PairOut *outs,
+                                                               uint32_t *ready, uint32_t seq) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const PairDesc &d = descs[k];
+    const PairCtr &c = ctrs[k];
+    const bool empty = (c.n_fp && !c.n_gold) || (c.n_fn && !c.n_segm);
+    PairOut o{};
+    if (ld_sc(&c.near_done) != 0 || empty)
+      o = finalize_pair(d, c, d.t_start + kTbSteps, false, 1);
+    else
+      o.status = 2;
+    outs[k] = o;
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(ready, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+struct MoveList {
+  int n;
+  uint16_t src[kSlotListMax], dst[kSlotListMax];
+  uint32_t gen[kSlotListMax];
+};
+// Pairs of an admission group into engine slots, one workgroup per pair:
+// the counters word by word, the descriptor with the slot's generation and
+// no start iteration yet (the join sets it), a one-pair pack index.
+__global__ __launch_bounds__(kWgThreads) void move_pairs_kernel(const PairDesc *__restrict__ gd,
+                                                                const PairCtr *__restrict__ gc,
+                                                                PairDesc *descs, PairCtr *ctrs,
+                                                                PackIndex *idx, MoveList l) {
+  const int i = blockIdx.x;
+  if (i >= l.n) return;
+  const int k = l.src[i], p = l.dst[i];
+  const uint32_t *src = (const uint32_t *)&gc[k];
+  uint32_t *dst = (uint32_t *)&ctrs[p];
+  static_assert(sizeof(PairCtr) % 4 == 0, "counters copied as words");
+  for (uint32_t w = threadIdx.x; w < sizeof(PairCtr) / 4; w += blockDim.x) dst[w] = src[w];
+  if (threadIdx.x == 0) {
+    PairDesc d = gd[k];
+    d.gen = l.gen[i];
+    d.t_start = 0xFFFFFFFFu;
+    descs[p] = d;
+    idx[p] = PackIndex{0, 0, 0};
+  }
+}
+
 }  // namespace hdm
 
 // table indexed by launch id: HDM_TILE_* (1..4) with the 6 element, then the
@@ -4356,16 +4420,16 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
 // workgroups, and the fold of every pair.
 extern "C" hipError_t hdm_launch_near_batch(hipStream_t s, const hdm::PairDesc *descs,
                                             const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
-                                            int npairs, uint64_t total_words, hdm::Engine *es,
+                                            int npairs, uint64_t total_words, uint32_t *count,
                                             uint64_t *list, uint32_t cap, int grid) {
   const uint64_t lb = std::min<uint64_t>(4096, (total_words + hdm::kWgThreads - 1) / hdm::kWgThreads);
   hipLaunchKernelGGL(hdm::near_list_kernel, dim3((unsigned)std::max<uint64_t>(1, lb)),
-                     dim3(hdm::kWgThreads), 0, s, descs, idx, npairs, total_words, list,
-                     &es->near_count, cap);
+                     dim3(hdm::kWgThreads), 0, s, descs, idx, npairs, total_words, list, count,
+                     cap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(hdm::near_batch_kernel, dim3(std::max(1, grid)), dim3(hdm::kWgThreads), 0, s,
-                     descs, ctrs, list, &es->near_count, cap);
+                     descs, ctrs, list, count, cap);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(hdm::near_fold_kernel,
                      dim3((npairs + hdm::kWavesPerWg - 1) / hdm::kWavesPerWg),
@@ -4392,3 +4456,28 @@ extern "C" hipError_t hdm_launch_set_start(hipStream_t s, hdm::PairDesc *descs, 
   return hipSuccess;
 }
 
+extern "C" hipError_t hdm_launch_group_out(hipStream_t s, const hdm::PairDesc *descs,
+                                           const hdm::PairCtr *ctrs, int n, hdm::PairOut *outs,
+                                           uint32_t *ready, uint32_t seq) {
+  hipLaunchKernelGGL(hdm::group_out_kernel, dim3(1), dim3(hdm::kWgThreads), 0, s, descs, ctrs, n,
+                     outs, ready, seq);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t hdm_launch_move_pairs(hipStream_t s, const hdm::PairDesc *gdescs,
+                                            const hdm::PairCtr *gctrs, hdm::PairDesc *descs,
+                                            hdm::PairCtr *ctrs, hdm::PackIndex *idx,
+                                            const int *src, const int *dst, const uint32_t *gen,
+                                            int n) {
+  for (int i = 0; i < n; i += hdm::kSlotListMax) {
+    hdm::MoveList l{};
+    l.n = std::min(n - i, hdm::kSlotListMax);
+    for (int k = 0; k < l.n; ++k)
+      l.src[k] = (uint16_t)src[i + k], l.dst[k] = (uint16_t)dst[i + k], l.gen[k] = gen[i + k];
+    hipLaunchKernelGGL(hdm::move_pairs_kernel, dim3(l.n), dim3(hdm::kWgThreads), 0, s, gdescs,
+                       gctrs, descs, ctrs, idx, l);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}

@@ -246,10 +246,25 @@ extern "C" hipError_t hdm_launch_reset(hipStream_t s, hdm::PairDesc *descs,
                                        hdm::PairCtr *ctrs, int p0, int npairs, hdm::Engine *es,
                                        int reset_engine, uint32_t qcap);
 extern "C" hipError_t hdm_launch_totals(hipStream_t s, hdm::PairCtr *ctrs, int p0, int npairs);
+// the near pass over npairs pairs (bitmaps of total_words words in all): list
+// (count, zero on entry, receives the entries; at most cap), steps, fold
 extern "C" hipError_t hdm_launch_near_batch(hipStream_t s, const hdm::PairDesc *descs,
                                             const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
-                                            int npairs, uint64_t total_words, hdm::Engine *es,
+                                            int npairs, uint64_t total_words, uint32_t *count,
                                             uint64_t *list, uint32_t cap, int grid);
+// server admission group: outs[k] = the results of pair k if its near pass
+// finished it (or a mask is empty), else status 2; then seq into *ready
+// (pinned host memory) behind a system-scope fence
+extern "C" hipError_t hdm_launch_group_out(hipStream_t s, const hdm::PairDesc *descs,
+                                           const hdm::PairCtr *ctrs, int n, hdm::PairOut *outs,
+                                           uint32_t *ready, uint32_t seq);
+// server admission group: pair src[i] of the group into engine slot dst[i]
+// (counters, and the descriptor with generation gen[i] and no start yet)
+extern "C" hipError_t hdm_launch_move_pairs(hipStream_t s, const hdm::PairDesc *gdescs,
+                                            const hdm::PairCtr *gctrs, hdm::PairDesc *descs,
+                                            hdm::PairCtr *ctrs, hdm::PackIndex *idx,
+                                            const int *src, const int *dst, const uint32_t *gen,
+                                            int n);
 // descs[slots[i]].t_start = t_start for i < n
 extern "C" hipError_t hdm_launch_set_start(hipStream_t s, hdm::PairDesc *descs, const int *slots,
                                            int n, uint32_t t_start);

@@ -24,6 +24,15 @@
 // Registered masks: a volume packed once (hdm_server_register) stays in the
 // arena as packed bits, face words and tile flags; requests naming two of
 // them read those instead of re-packing their volumes.
+//
+// Admission groups (64x8x8 tiles, 6 element, near pass on): the waiting
+// requests are staged together as one group on the admission stream -- one
+// reset, one pack over the group's volume pairs (a combine per registered
+// pair), the totals, the near pass and an outcome kernel, whose ready marker
+// the step waits for.  A pair whose cuts the near pass reached within eight
+// steps (most evaluation pairs) completes there and never takes a slot; the
+// others move into free slots and join the engine as staged pairs do.  The
+// group runs between slices, so no kernel of it runs beside a slice's grid.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -158,6 +167,24 @@ struct hdm_server {
   std::map<uint64_t, MaskReg> masks;
   uint64_t next_mask = 1;
   hdm_server_stats st{};
+  // admission groups (near pass on): device descriptors, pack index,
+  // counters, entry count and entry list of up to max_slots pairs; pinned
+  // staging of the descriptors and index, the outcomes and their marker
+  bool near = false;
+  int near_grid = 1;
+  void *gmem = nullptr;
+  PairDesc *gdescs = nullptr;
+  PackIndex *gidx = nullptr;
+  PairCtr *gctrs = nullptr;
+  uint32_t *gcount = nullptr;
+  uint64_t *glist = nullptr;
+  uint32_t glist_cap = 0;
+  void *ghost = nullptr;
+  PairDesc *hdescs = nullptr;
+  PackIndex *hidx = nullptr;
+  PairOut *gouts = nullptr;
+  volatile uint32_t *gready = nullptr;
+  uint32_t gseq = 0;
 };
 
 static hipEvent_t take_event(hdm_server *s) {
@@ -201,6 +228,8 @@ extern "C" void hdm_server_destroy(hdm_server *s) {
   if (s->common) (void)hipFree(s->common);
   if (s->arena_mem) (void)hipFree(s->arena_mem);
   if (s->host) (void)hipHostFree(s->host);
+  if (s->gmem) (void)hipFree(s->gmem);
+  if (s->ghost) (void)hipHostFree(s->ghost);
   if (s->s) (void)hipStreamDestroy(s->s);
   if (s->s_adm) (void)hipStreamDestroy(s->s_adm);
   (void)hipGetLastError();
@@ -265,6 +294,34 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
   s->freed_after.assign(o.max_slots, 0);
   s->slices = 1;
   s->cfg = launch_cfg(c, s->tile, 1, 1, o.grid_ctas, 1, o.max_slots);  // several pairs
+  s->near = s->tl->fused_near && conn == 6 && near_enabled();
+  if (s->near) {
+    const uint64_t G = (uint64_t)o.max_slots;
+    const uint64_t lcap = G * 2 * (uint64_t)gmax.nregions;
+    s->glist_cap = (uint32_t)std::min<uint64_t>(lcap, 0xFFFFFFFFull);
+    const uint64_t bd = align256(G * sizeof(PairDesc)), bi = align256(G * sizeof(PackIndex)),
+                   bc = align256(G * sizeof(PairCtr));
+    const uint64_t hb = bd + bi + align256(G * sizeof(PairOut)) + 256;
+    if (hipMalloc(&s->gmem, bd + bi + bc + 256 + 8 * (uint64_t)s->glist_cap) != hipSuccess ||
+        hipHostMalloc(&s->ghost, hb, hipHostMallocCoherent) != hipSuccess) {
+      (void)hipGetLastError();
+      hdm_server_destroy(s);
+      return HDM_E_NOMEM;
+    }
+    char *g = (char *)s->gmem;
+    s->gdescs = (PairDesc *)g;
+    s->gidx = (PackIndex *)(g + bd);
+    s->gctrs = (PairCtr *)(g + bd + bi);
+    s->gcount = (uint32_t *)(g + bd + bi + bc);
+    s->glist = (uint64_t *)(g + bd + bi + bc + 256);
+    std::memset(s->ghost, 0, hb);  // no marker left by an earlier allocation matches
+    char *h = (char *)s->ghost;
+    s->hdescs = (PairDesc *)h;
+    s->hidx = (PackIndex *)(h + bd);
+    s->gouts = (PairOut *)(h + bd + bi);
+    s->gready = (volatile uint32_t *)(h + bd + bi + align256(G * sizeof(PairOut)));
+    s->near_grid = std::max(1, std::min(kMaxGrid, c->occ_fused_near[s->tile] * c->cus));
+  }
   // every slot starts unused: a start beyond any engine iteration keeps it
   // out of the per-iteration bookkeeping
   std::vector<PairDesc> d(o.max_slots);
@@ -294,11 +351,12 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
 }
 
 static uint64_t request_bytes(const hdm_server *s, const Request &r, uint64_t *in_bytes) {
-  (void)s;
   const uint64_t nvox = (uint64_t)r.nx * r.ny * r.nz;
   const uint64_t esz = r.dtype == HDM_DTYPE_U8 ? 1 : 4;
   *in_bytes = r.ptr_kind == HDM_PTR_HOST && !r.mg ? 2 * align256(nvox * esz) : 0;
-  return *in_bytes + r.geom.total;
+  // with admission groups, the pair's near-pass bitmap after its workspace
+  const uint64_t marks = s->near ? align256(4ull * near_mark_words(r.geom.nregions)) : 0;
+  return *in_bytes + r.geom.total + marks;
 }
 
 extern "C" int hdm_server_submit(hdm_server *s, const void *gold, const void *segm, int nx,
@@ -538,6 +596,144 @@ static int stage_pending(hdm_server *s, bool *quarantined) {
   return HDM_OK;
 }
 
+// One admission group (servers with the near pass): the waiting requests, in
+// arrival order, while a free slot for each (any of them may need one), arena
+// room and room among the caller's completions remain.  Volume pairs come
+// first (one pack over them), registered pairs after (a combine each).  The
+// pairs the near pass finishes complete here; the others move into slots as
+// staged pairs, which the join then starts.  Returns an error only for a
+// failure of the step itself (the group's requests then complete with it).
+static int run_group(hdm_server *s, hdm_completion *done, int cap, int *n_done,
+                     bool *quarantined) {
+  std::vector<int> free_slots;
+  for (int i = 0; i < (int)s->slots.size(); ++i)
+    if (s->slots[i].id == 0) {
+      if (s->freed_after[i] < s->slices)
+        free_slots.push_back(i);
+      else
+        *quarantined = true;
+    }
+  const int room = std::min<int>((int)free_slots.size(), cap - *n_done);
+  std::vector<Request> grp;
+  while (!s->pending.empty() && (int)grp.size() < room) {
+    Request &r = s->pending.front();
+    if (!s->arena.alloc(r.bytes, &r.off)) break;
+    grp.push_back(r);
+    s->pending.pop_front();
+  }
+  if (grp.empty()) return HDM_OK;
+  ++s->st.groups;
+  s->st.group_pairs += grp.size();
+  std::stable_partition(grp.begin(), grp.end(), [](const Request &r) { return r.mg == 0; });
+  const int n = (int)grp.size();
+  int nv = 0;
+  uint64_t tiles = 0, rw = 0, words = 0;
+  hipError_t e = hipSuccess;
+  const double t_admit = now_s();
+  for (int k = 0; k < n && e == hipSuccess; ++k) {
+    Request &r = grp[k];
+    uint64_t inb = 0;
+    request_bytes(s, r, &inb);
+    char *base = (char *)s->arena_mem + r.off;
+    const void *gd = r.gold, *sd = r.segm;
+    if (r.ptr_kind == HDM_PTR_HOST && !r.mg) {
+      const uint64_t nb = (uint64_t)r.nx * r.ny * r.nz * (r.dtype == HDM_DTYPE_U8 ? 1 : 4);
+      e = hipMemcpyAsync(base, r.gold, nb, hipMemcpyHostToDevice, s->s_adm);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(base + inb / 2, r.segm, nb, hipMemcpyHostToDevice, s->s_adm);
+      gd = base, sd = base + inb / 2;
+    }
+    PairDesc d{};
+    fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, kUnused, s->conn);
+    carve_pair(d, r.geom, base + inb);
+    d.marks = (uint32_t *)(base + inb + r.geom.total);
+    if (r.mg) {
+      const MaskReg &mg = s->masks.at(r.mg), &ms = s->masks.at(r.ms);
+      d.gold = d.segm = nullptr;
+      d.orig[0] = mg.orig, d.orig[1] = ms.orig;
+      d.xf_orig[0] = mg.xf, d.xf_orig[1] = ms.xf;
+    } else {
+      ++nv;
+    }
+    s->hdescs[k] = d;
+    s->hidx[k] = PackIndex{rw, r.mg ? 0 : tiles, words};
+    if (!r.mg) tiles += r.geom.ntiles, rw += r.geom.row_words;
+    words += near_mark_words(r.geom.nregions);
+    r.t_admit = t_admit;
+  }
+  const uint32_t seq = ++s->gseq ? s->gseq : ++s->gseq;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->gdescs, s->hdescs, sizeof(PairDesc) * n, hipMemcpyHostToDevice, s->s_adm);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->gidx, s->hidx, sizeof(PackIndex) * n, hipMemcpyHostToDevice, s->s_adm);
+  if (e == hipSuccess) e = hipMemsetAsync(s->gcount, 0, sizeof(uint32_t), s->s_adm);
+  if (e == hipSuccess)
+    e = hdm_launch_reset(s->s_adm, s->gdescs, nullptr, nullptr, s->gidx, s->gctrs, 0, n, nullptr,
+                         0, 0);
+  if (e == hipSuccess && nv > 0) {
+    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, rw, tiles, 0, 1, nv);
+    e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, s->gdescs, s->gidx, s->gctrs, nv, tiles,
+                    nullptr);
+  }
+  for (int k = nv; k < n && e == hipSuccess; ++k) {
+    const Request &r = grp[k];
+    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1, 1);
+    e = s->tl->combine(dim3(lc.pack_blocks), s->s_adm, s->gdescs + k, s->gctrs + k,
+                       s->masks.at(r.mg).tflags, s->masks.at(r.ms).tflags, r.geom.ntiles);
+  }
+  if (e == hipSuccess) e = hdm_launch_totals(s->s_adm, s->gctrs, 0, n);
+  if (e == hipSuccess)
+    e = hdm_launch_near_batch(s->s_adm, s->gdescs, s->gidx, s->gctrs, n, words, s->gcount,
+                              s->glist, s->glist_cap, s->near_grid);
+  if (e == hipSuccess)
+    e = hdm_launch_group_out(s->s_adm, s->gdescs, s->gctrs, n, s->gouts, (uint32_t *)s->gready,
+                             seq);
+  int rc = e == hipSuccess ? wait_ready(s->s_adm, s->gready, seq) : map_hip(e);
+  const double t_now = now_s();
+  std::vector<int> src, dst;
+  std::vector<uint32_t> gen;
+  size_t next_slot = 0;
+  for (int k = 0; k < n; ++k) {
+    Request &r = grp[k];
+    PairOut o;
+    std::memcpy(&o, s->gouts + k, sizeof(o));
+    if (rc == HDM_OK && o.status == 2) {  // to the engine
+      const int p = free_slots[next_slot++];
+      src.push_back(k), dst.push_back(p);
+      gen.push_back((++s->slot_gen[p]) & kGenMask);
+      r.slot = p;
+      r.active = false;
+      s->slots[p] = r;
+      ++s->st.admitted;
+      continue;
+    }
+    hdm_completion &c = done[(*n_done)++];
+    std::memset(&c, 0, sizeof(c));
+    c.id = r.id;
+    c.status = rc == HDM_OK ? o.status : rc;
+    if (rc == HDM_OK) fill_result(&c.result, o, nullptr, r.geom.ntiles, std::max(o.hd_fp, o.hd_fn));
+    c.t_submit = r.t_submit;
+    c.t_admit = r.t_admit;
+    c.t_done = t_now;
+    s->arena.release(r.off, r.bytes);
+    if (r.mg) mask_unref(s, r.mg), mask_unref(s, r.ms);
+    if (rc == HDM_OK) ++s->st.near_finished;
+  }
+  if (rc != HDM_OK) return HDM_OK;  // the group's requests completed with the error
+  if (!src.empty()) {
+    e = hdm_launch_move_pairs(s->s_adm, s->gdescs, s->gctrs, s->P.descs, s->P.ctrs, s->P.idx,
+                              src.data(), dst.data(), gen.data(), (int)src.size());
+    for (int p : dst) {
+      Request &r = s->slots[p];
+      r.staged = take_event(s);
+      if (e == hipSuccess && (!r.staged || hipEventRecord(r.staged, s->s_adm) != hipSuccess))
+        e = hipErrorUnknown;
+    }
+    if (e != hipSuccess) return map_hip(e);
+  }
+  return HDM_OK;
+}
+
 // The lone request at the head of the FIFO as a one-pair call (hdm_hausdorff_*)
 // on the engine stream: the same results as the pair computed alone, which
 // every request's result equals.  Its workspace is the library context's (the
@@ -602,7 +798,7 @@ extern "C" int hdm_server_step(hdm_server *s, hdm_completion *done, int cap, int
   // stage what waits, then join it: between slices, never beside one
   bool quarantined = false;
   int active = 0, joined = 0;
-  int rc = stage_pending(s, &quarantined);
+  int rc = s->near ? run_group(s, done, cap, n_done, &quarantined) : stage_pending(s, &quarantined);
   if (rc != HDM_OK) return rc;
   if ((rc = join_staged(s, &joined)) != HDM_OK) return rc;
   for (auto &r : s->slots) active += r.id != 0 && r.active;

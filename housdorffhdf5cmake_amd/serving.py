@@ -76,11 +76,14 @@ class Server:
     def stats(self) -> dict:
         """Slices run, pairs in flight summed over them (batch_mean = the
         ratio), the most in one slice, lone one-pair runs, admissions through
-        the slots, and the arena bytes of registered masks."""
+        the slots, the arena bytes of registered masks, and (64x8x8 tiles, 6
+        element) the admission groups, their requests (group_mean = the
+        ratio) and the requests their near pass finished."""
         st = ServerStats()
         check(lib().hdm_server_stats_get(self.h, C.byref(st)), "hdm_server_stats_get")
         d = st.as_dict()
         d["batch_mean"] = round(d["pair_slices"] / d["slices"], 2) if d["slices"] else 0.0
+        d["group_mean"] = round(d["group_pairs"] / d["groups"], 2) if d["groups"] else 0.0
         return d
 
     def step(self):
@@ -214,6 +217,8 @@ def poisson_run(seconds=10.0, rate=200.0, budget_bytes=64 << 30, verify=None,
                "batch_mean": st["batch_mean"], "max_in_flight": st["max_in_flight"],
                "slices": st["slices"], "lone_runs": st["lone"],
                "admitted_through_slots": st["admitted"],
+               # requests per admission group, and those its near pass finished
+               "group_mean": st["group_mean"], "near_finished": st["near_finished"],
                "max_slots": max_slots, "hbm_budget_gb": budget_bytes / (1 << 30),
                "identical_alone_vs_mix": bool(ok), "bit_exact_vs_cpu": exact_cpu,
                "pool": f"{per_class} pairs per volume class, device resident" + (
@@ -270,6 +275,7 @@ def closed_loop_run(seconds=5.0, outstanding=64, budget_bytes=64 << 30, per_clas
             "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
             "batch_mean": st["batch_mean"], "max_in_flight": st["max_in_flight"],
             "slices": st["slices"], "lone_runs": st["lone"],
+            "group_mean": st["group_mean"], "near_finished": st["near_finished"],
             "max_slots": max_slots, "slice_iters": slice_iters,
             "identical_alone_vs_mix": bool(ok),
             "pool": f"{per_class} pairs per volume class, device resident" + (

@@ -322,6 +322,10 @@ typedef struct hdm_server_stats {
   uint64_t lone;              /* requests run as one one-pair call              */
   uint64_t admitted;          /* requests admitted through the slots            */
   uint64_t registered_bytes;  /* arena bytes held by registered masks           */
+  uint64_t near_finished;     /* requests an admission group's near pass
+                                 finished (64x8x8 tiles, 6 element): no slot    */
+  uint64_t groups;            /* admission groups run                           */
+  uint64_t group_pairs;       /* sum over those groups of their requests        */
 } hdm_server_stats;
 int hdm_server_stats_get(hdm_server *s, hdm_server_stats *out);
 double hdm_server_now(void);

@@ -337,7 +337,8 @@ def test_registered_masks_equal_fresh_submit(robust, conn):
         done = srv.drain(60)
         assert [c["id"] for c in done] == [rid] and same(done[0], alone[k]), k
     st = srv.stats()
-    assert st["registered_bytes"] > 0 and st["admitted"] >= 3 * len(vols)
+    assert st["registered_bytes"] > 0
+    assert st["admitted"] + st["near_finished"] + st["lone"] >= 3 * len(vols)
     rid = srv.submit_masks(hg[0], hs[0])
     srv.unregister(hs[0])
     done = srv.drain(60)
@@ -360,7 +361,9 @@ def test_loaded_serving_batches_and_equals_alone(resident):
     that keeps 24 requests in the server -- every result equals the pair
     computed alone, with slices of eight steps and of two (pairs then stay in
     flight across slices and new ones join between them); in the closed loop
-    the engine runs several pairs per slice (batch_mean > 1)."""
+    several requests are admitted together, and the admission group's near
+    pass finishes them (this mix has HD <= 8 in every pair).  The slices'
+    own batching under load runs with the near pass off, below."""
     for si in (8, 2):
         r = serving.poisson_run(seconds=0.5, rate=6000, budget_bytes=8 << 30, per_class=2,
                                 max_slots=32, slice_iters=si, resident=resident)
@@ -370,4 +373,36 @@ def test_loaded_serving_batches_and_equals_alone(resident):
                                     per_class=2, max_slots=32, slice_iters=si,
                                     resident=resident)
         assert c["completed"] > 500 and c["identical_alone_vs_mix"], si
-        assert c["batch_mean"] > 4 and c["max_in_flight"] > 8, (si, c)
+        assert c["group_mean"] > 4 and c["near_finished"] > 500, (si, c)
+
+
+CHILD_LOADED = r"""
+import sys, json
+sys.path.insert(0, sys.argv[1])
+from housdorffhdf5cmake_amd import serving
+out = []
+for si in (8, 2):
+    c = serving.closed_loop_run(seconds=0.5, outstanding=24, budget_bytes=8 << 30, per_class=2,
+                                max_slots=32, slice_iters=si, resident=True)
+    out.append(c)
+print(json.dumps(out))
+"""
+
+
+def test_loaded_serving_through_the_slices():
+    """With the near pass off (HDM_NEAR=0) every admitted request goes
+    through the slots and the slices: under a closed loop of 24 requests the
+    engine runs several pairs per slice, and every result equals the pair
+    computed alone."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, HDM_NEAR="0")
+    r = subprocess.run([sys.executable, "-c", CHILD_LOADED, root], capture_output=True, text=True,
+                       timeout=240, env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    for c in json.loads(r.stdout.strip().splitlines()[-1]):
+        assert c["completed"] > 500 and c["identical_alone_vs_mix"], c
+        assert c["batch_mean"] > 4 and c["max_in_flight"] > 8 and c["near_finished"] == 0, c

@@ -176,8 +176,9 @@ import torch
 import housdorffhdf5cmake_amd as hdm
 from housdorffhdf5cmake_amd import serving
 keys = json.loads(sys.argv[2])
-vols = [hdm.synth_device(hdm.synth_spec("ellipsoids", 160, 128, 96, 8, 6, 24, 2, 0.005, 41)),
-        hdm.synth_device(hdm.synth_spec("ellipsoids", 67, 45, 31, 6, 4, 20, 2, 0.01, 7))]
+# pairs with HD > 8: the admission group's near pass hands them to the slots
+lg, ls = hdm.synth_device(hdm.synth_spec("long", 128, 96, 128, 30, 64, 48, 20, 15, 12, 60, 4))
+vols = [(lg, ls), (ls, lg)]
 torch.cuda.synchronize()
 alone = [{k: hdm.hausdorff(g, s)[k] for k in keys} for g, s in vols]
 srv = serving.Server(hbm_budget=1 << 30, max_slots=4, max_dims=(160, 128, 128))
