@@ -400,8 +400,7 @@ __device__ __forceinline__ void acc_flush(S &c, PackAcc &a, int lane, int slot) 
 // non-empty, full, and which of its six faces hold voxels.  Voxel counts and
 // the box of non-empty tiles (getMinMaxes, hello.cu:396-526) accumulate per
 // lane and are published once per (wave, pair) into spread slots.
-// store that other workgroups of the same launch read later (COH: agent-
-// coherent, written through) or that only later launches read (plain)
+
 // The near-pass mark of a 64x8x8 tile (tx, ty, tz) that holds FP (fp) or FN
 // (fn) voxels: bit m * regions + rg of the pair's bitmap, for its region rg
 // of 1 x 2 x 2 tiles (pass 0: FP, pass 1: FN), by an atomic OR.
@@ -414,6 +413,8 @@ __device__ __forceinline__ void mark_tile(const PairDesc &d, int tx, int ty, int
   if (fn) or_g(d.marks + ((nreg + rg) >> 5), 1u << ((nreg + rg) & 31));
 }
 
+// store that other workgroups of the same launch read later (COH: agent-
+// coherent, written through) or that only later launches read (plain)
 template <bool COH, typename T>
 __device__ __forceinline__ void put(T *p, T v) {
   if constexpr (COH)
@@ -805,6 +806,58 @@ __global__ __launch_bounds__(kWgThreads) void combine_kernel(const PairDesc *__r
     }
   }
   acc_flush(*ctr, acc, lane, slot);
+}
+
+// The combine pass of several pairs of registered masks (a server admission
+// group; 64x8x8 tiles): pair p's tiles are [idx[p].tile_base, + its tiles),
+// its masks' tile flags gf[p] and sf[p]; each wave walks a contiguous range
+// of tiles and publishes its counts and box once per pair it crossed.
+__global__ __launch_bounds__(kWgThreads) void combine_group_kernel(
+    const PairDesc *__restrict__ descs, const PackIndex *__restrict__ idx, PairCtr *ctrs,
+    int npairs, uint64_t total_tiles, const uint32_t *const *__restrict__ gfs,
+    const uint32_t *const *__restrict__ sfs) {
+  using Row = uint64_t;
+  constexpr int R = 64;
+  const int lane = threadIdx.x & 63;
+  const int yl = lane % 8, zl = lane / 8;
+  const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerWg + (threadIdx.x >> 6);
+  const uint64_t nw = (uint64_t)gridDim.x * kWavesPerWg;
+  const uint64_t t0 = min(total_tiles, gw * total_tiles / nw);
+  const uint64_t t1 = min(total_tiles, (gw + 1) * total_tiles / nw);
+  const int slot = (int)(gw % kSumSlots);
+  PackAcc acc;
+  acc_clear(acc);
+  int ap = -1;
+  uint64_t ce = 0;  // end of pair ap's tiles
+  for (uint64_t g = t0; g < t1; ++g) {
+    if (ap < 0 || g >= ce) {  // the next pair (wave-uniform)
+      if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
+      ap = find_pair(npairs, g, [&](int i) { return idx[i].tile_base; });
+      ce = ap + 1 < npairs ? idx[ap + 1].tile_base : total_tiles;
+    }
+    const PairDesc &d = descs[ap];
+    const uint32_t tile = (uint32_t)(g - idx[ap].tile_base);
+    const uint32_t gf = gfs[ap][tile] & 0xFFFFu, sf = sfs[ap][tile] & 0xFFFFu;
+    if (lane == 0) d.tflags[tile] = gf | (sf << 16);
+    const int tx = (int)(tile % (uint32_t)d.gx);
+    const int ty = (int)((tile / (uint32_t)d.gx) % (uint32_t)d.gy);
+    const int tz = (int)(tile / ((uint32_t)d.gx * d.gy));
+    // rows beyond the volume were never written by the masks' pack
+    const bool rowok = ty * 8 + yl < d.ny && tz * 8 + zl < d.nz;
+    const uint64_t o = (uint64_t)tile * R + lane;
+    const Row a = rowok && (gf & 1u) ? ((const Row *)d.orig[0])[o] : (Row)0;
+    const Row b = rowok && (sf & 1u) ? ((const Row *)d.orig[1])[o] : (Row)0;
+    acc.c[0] += popc_row(a), acc.c[1] += popc_row(b);
+    acc.c[2] += popc_row((Row)(b & ~a)), acc.c[3] += popc_row((Row)(a & ~b));
+    const bool fp = __ballot((b & ~a) != 0) != 0, fn = __ballot((a & ~b) != 0) != 0;
+    if (lane == 0 && (fp || fn) && d.marks) mark_tile(d, tx, ty, tz, fp, fn);
+    if ((a | b) != 0) {
+      acc.mn[0] = min(acc.mn[0], tx), acc.mx[0] = max(acc.mx[0], tx);
+      acc.mn[1] = min(acc.mn[1], ty), acc.mx[1] = max(acc.mx[1], ty);
+      acc.mn[2] = min(acc.mn[2], tz), acc.mx[2] = max(acc.mx[2], tz);
+    }
+  }
+  if (ap >= 0) acc_flush(ctrs[ap], acc, lane, slot);
 }
 
 // The engine iteration at which the pairs of some server slots start (a
@@ -4480,4 +4533,14 @@ extern "C" hipError_t hdm_launch_move_pairs(hipStream_t s, const hdm::PairDesc *
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+extern "C" hipError_t hdm_launch_combine_group(hipStream_t s, const hdm::PairDesc *descs,
+                                               const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
+                                               int npairs, uint64_t total_tiles,
+                                               const uint32_t *const *gfs,
+                                               const uint32_t *const *sfs, int grid) {
+  hipLaunchKernelGGL(hdm::combine_group_kernel, dim3(std::max(1, grid)), dim3(hdm::kWgThreads), 0,
+                     s, descs, idx, ctrs, npairs, total_tiles, gfs, sfs);
+  return hipGetLastError();
 }

@@ -258,6 +258,14 @@ extern "C" hipError_t hdm_launch_near_batch(hipStream_t s, const hdm::PairDesc *
 extern "C" hipError_t hdm_launch_group_out(hipStream_t s, const hdm::PairDesc *descs,
                                            const hdm::PairCtr *ctrs, int n, hdm::PairOut *outs,
                                            uint32_t *ready, uint32_t seq);
+// server admission group, registered masks (64x8x8 tiles): the combine pass
+// of npairs pairs at once (pair p: tiles from idx[p].tile_base, its masks'
+// tile flags gfs[p] and sfs[p])
+extern "C" hipError_t hdm_launch_combine_group(hipStream_t s, const hdm::PairDesc *descs,
+                                               const hdm::PackIndex *idx, hdm::PairCtr *ctrs,
+                                               int npairs, uint64_t total_tiles,
+                                               const uint32_t *const *gfs,
+                                               const uint32_t *const *sfs, int grid);
 // server admission group: pair src[i] of the group into engine slot dst[i]
 // (counters, and the descriptor with generation gen[i] and no start yet)
 extern "C" hipError_t hdm_launch_move_pairs(hipStream_t s, const hdm::PairDesc *gdescs,

@@ -185,6 +185,9 @@ struct hdm_server {
   PairOut *gouts = nullptr;
   volatile uint32_t *gready = nullptr;
   uint32_t gseq = 0;
+  // registered pairs of a group: their masks' tile flags (device, and the
+  // pinned staging)
+  const uint32_t **gflags = nullptr, **hflags = nullptr;
 };
 
 static hipEvent_t take_event(hdm_server *s) {
@@ -301,8 +304,9 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
     s->glist_cap = (uint32_t)std::min<uint64_t>(lcap, 0xFFFFFFFFull);
     const uint64_t bd = align256(G * sizeof(PairDesc)), bi = align256(G * sizeof(PackIndex)),
                    bc = align256(G * sizeof(PairCtr));
-    const uint64_t hb = bd + bi + align256(G * sizeof(PairOut)) + 256;
-    if (hipMalloc(&s->gmem, bd + bi + bc + 256 + 8 * (uint64_t)s->glist_cap) != hipSuccess ||
+    const uint64_t bf = align256(2 * G * sizeof(void *));
+    const uint64_t hb = bd + bi + align256(G * sizeof(PairOut)) + 256 + bf;
+    if (hipMalloc(&s->gmem, bd + bi + bc + 256 + bf + 8 * (uint64_t)s->glist_cap) != hipSuccess ||
         hipHostMalloc(&s->ghost, hb, hipHostMallocCoherent) != hipSuccess) {
       (void)hipGetLastError();
       hdm_server_destroy(s);
@@ -313,13 +317,15 @@ extern "C" int hdm_server_create(const hdm_server_opts *oin, hdm_server **out) {
     s->gidx = (PackIndex *)(g + bd);
     s->gctrs = (PairCtr *)(g + bd + bi);
     s->gcount = (uint32_t *)(g + bd + bi + bc);
-    s->glist = (uint64_t *)(g + bd + bi + bc + 256);
+    s->gflags = (const uint32_t **)(g + bd + bi + bc + 256);
+    s->glist = (uint64_t *)(g + bd + bi + bc + 256 + bf);
     std::memset(s->ghost, 0, hb);  // no marker left by an earlier allocation matches
     char *h = (char *)s->ghost;
     s->hdescs = (PairDesc *)h;
     s->hidx = (PackIndex *)(h + bd);
     s->gouts = (PairOut *)(h + bd + bi);
     s->gready = (volatile uint32_t *)(h + bd + bi + align256(G * sizeof(PairOut)));
+    s->hflags = (const uint32_t **)(h + bd + bi + align256(G * sizeof(PairOut)) + 256);
     s->near_grid = std::max(1, std::min(kMaxGrid, c->occ_fused_near[s->tile] * c->cus));
   }
   // every slot starts unused: a start beyond any engine iteration keeps it
@@ -627,7 +633,7 @@ static int run_group(hdm_server *s, hdm_completion *done, int cap, int *n_done,
   std::stable_partition(grp.begin(), grp.end(), [](const Request &r) { return r.mg == 0; });
   const int n = (int)grp.size();
   int nv = 0;
-  uint64_t tiles = 0, rw = 0, words = 0;
+  uint64_t tiles = 0, rw = 0, words = 0, rtiles = 0;
   hipError_t e = hipSuccess;
   const double t_admit = now_s();
   for (int k = 0; k < n && e == hipSuccess; ++k) {
@@ -647,17 +653,22 @@ static int run_group(hdm_server *s, hdm_completion *done, int cap, int *n_done,
     fill_desc(d, r.geom, gd, sd, r.dtype, r.label, s->o.robustness, 1, s->tile, kUnused, s->conn);
     carve_pair(d, r.geom, base + inb);
     d.marks = (uint32_t *)(base + inb + r.geom.total);
+    const uint64_t G = (uint64_t)s->o.max_slots;
     if (r.mg) {
       const MaskReg &mg = s->masks.at(r.mg), &ms = s->masks.at(r.ms);
       d.gold = d.segm = nullptr;
       d.orig[0] = mg.orig, d.orig[1] = ms.orig;
       d.xf_orig[0] = mg.xf, d.xf_orig[1] = ms.xf;
+      s->hflags[k - nv] = mg.tflags, s->hflags[G + k - nv] = ms.tflags;
     } else {
       ++nv;
     }
     s->hdescs[k] = d;
-    s->hidx[k] = PackIndex{rw, r.mg ? 0 : tiles, words};
+    // tile bases: volume pairs among themselves (the pack), registered pairs
+    // among themselves (the combine)
+    s->hidx[k] = PackIndex{rw, r.mg ? rtiles : tiles, words};
     if (!r.mg) tiles += r.geom.ntiles, rw += r.geom.row_words;
+    else rtiles += r.geom.ntiles;
     words += near_mark_words(r.geom.nregions);
     r.t_admit = t_admit;
   }
@@ -666,6 +677,9 @@ static int run_group(hdm_server *s, hdm_completion *done, int cap, int *n_done,
     e = hipMemcpyAsync(s->gdescs, s->hdescs, sizeof(PairDesc) * n, hipMemcpyHostToDevice, s->s_adm);
   if (e == hipSuccess)
     e = hipMemcpyAsync(s->gidx, s->hidx, sizeof(PackIndex) * n, hipMemcpyHostToDevice, s->s_adm);
+  if (e == hipSuccess && nv < n)
+    e = hipMemcpyAsync(s->gflags, s->hflags, 2 * sizeof(void *) * s->o.max_slots,
+                       hipMemcpyHostToDevice, s->s_adm);
   if (e == hipSuccess) e = hipMemsetAsync(s->gcount, 0, sizeof(uint32_t), s->s_adm);
   if (e == hipSuccess)
     e = hdm_launch_reset(s->s_adm, s->gdescs, nullptr, nullptr, s->gidx, s->gctrs, 0, n, nullptr,
@@ -675,11 +689,11 @@ static int run_group(hdm_server *s, hdm_completion *done, int cap, int *n_done,
     e = s->tl->pack(dim3(lc.pack_blocks), s->s_adm, s->gdescs, s->gidx, s->gctrs, nv, tiles,
                     nullptr);
   }
-  for (int k = nv; k < n && e == hipSuccess; ++k) {
-    const Request &r = grp[k];
-    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, r.geom.row_words, r.geom.ntiles, 0, 1, 1);
-    e = s->tl->combine(dim3(lc.pack_blocks), s->s_adm, s->gdescs + k, s->gctrs + k,
-                       s->masks.at(r.mg).tflags, s->masks.at(r.ms).tflags, r.geom.ntiles);
+  if (e == hipSuccess && nv < n) {
+    const LaunchCfg lc = launch_cfg(s->ctx, s->tile, 0, rtiles, 0, 1, n - nv);
+    e = hdm_launch_combine_group(s->s_adm, s->gdescs + nv, s->gidx + nv, s->gctrs + nv, n - nv,
+                                 rtiles, s->gflags, s->gflags + s->o.max_slots,
+                                 (int)lc.pack_blocks);
   }
   if (e == hipSuccess) e = hdm_launch_totals(s->s_adm, s->gctrs, 0, n);
   if (e == hipSuccess)
