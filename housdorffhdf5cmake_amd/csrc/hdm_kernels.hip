@@ -433,9 +433,9 @@ __device__ __forceinline__ void put(T *p, T v) {
 // dilation covers) and m = 1 (FN).  grab (zeroed counters, 32 words apart,
 // or null) and pools (1 or 8): the waves split three quarters of the steps
 // evenly and take the rest one step at a time from a counter (with 8 pools,
-// the counter of their XCD, over its eighth of those steps), so CUs that
-// stream faster pack more and the pack ends at about the same time on every
-// CU.
+// the counter of their workgroup's XCD, over its eighth of those steps), so
+// CUs that stream faster pack more and the pack ends at about the same time
+// on every CU.
 template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
@@ -468,10 +468,16 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   const uint64_t sfix = grab ? nsteps * 3 / 4 / nw : 0;
   uint64_t dyn0 = sfix * nw, dyn1 = nsteps;  // the wave's pool of steps
   if (grab && pools > 1) {
-    const uint32_t pool = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) % (uint32_t)pools;
+    // pool = blockIdx % pools, which is the workgroup's XCD when the GPU runs
+    // as one partition (workgroups go to the XCDs round robin); every pool
+    // has workgroups as long as the grid has at least as many as pools, and
+    // a grid with fewer takes one pool (a pool without waves would leave its
+    // steps unpacked)
+    const uint32_t np = min((uint32_t)pools, gridDim.x);
+    const uint32_t pool = blockIdx.x % np;
     const uint64_t nd = nsteps - dyn0, b = dyn0;
-    dyn0 = b + nd * pool / (uint32_t)pools;
-    dyn1 = b + nd * (pool + 1) / (uint32_t)pools;
+    dyn0 = b + nd * pool / np;
+    dyn1 = b + nd * (pool + 1) / np;
     grab += 32 * pool;
   }
   const uint64_t w0 = grab ? min(total_tiles, gw * sfix * kStep)

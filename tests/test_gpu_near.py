@@ -133,3 +133,25 @@ def test_calls_of_changing_size_in_a_row():
         pairs.append((g, s))
     for i, (g, s) in enumerate(pairs * 2):
         check(g, s, what=f"call {i}")
+
+
+@pytest.mark.parametrize("grid", [1, 2, 3, 5, 8, 13])
+def test_small_grids_take_every_pack_step(grid):
+    """The fused launches' pack hands its last quarter of steps out from one
+    counter per group of workgroups; with fewer workgroups than groups every
+    step must still be packed (a group without workgroups would leave its
+    steps out): a short pair and a pair beyond eight steps, each at a grid of
+    a few workgroups, equal the oracle."""
+    short = blob((40, 48, 150), (20, 24, 60), 6)
+    s1 = short.copy()
+    s1[20, 24, 60 + 6 + 3] = 1  # HD 3
+    lg, ls = hdm.synth_device(hdm.synth_spec("long", 128, 96, 64, 16, 48, 32, 12, 10, 8, 40, 3))
+    torch.cuda.synchronize()
+    for g, s in ((short, s1), (lg.cpu().numpy(), ls.cpu().numpy())):
+        hl = sum(g.shape) + 2
+        res = hdm.hausdorff(dev(g), dev(s), hist_len=hl, grid_ctas=grid)
+        rc, want, hf, hn, _ = oracle.run(g, s)
+        assert rc == 0
+        bad = {k: (res[k], want[k]) for k in KEYS if res[k] != want[k]}
+        assert not bad, (grid, g.shape, bad)
+        assert np.array_equal(res["hist_fp"], hf) and np.array_equal(res["hist_fn"], hn)
