@@ -467,23 +467,34 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   // behind that step's loads
   const uint64_t sfix = grab ? nsteps * 3 / 4 / nw : 0;
   uint64_t dyn0 = sfix * nw, dyn1 = nsteps;  // the wave's pool of steps
+  // one pair: the static steps strided (gw, gw + nw, ...) and the pools'
+  // dynamic steps interleaved, so all waves sweep the volume together in
+  // tile order (the cube's launch 86.9 -> 85.4 us against contiguous ranges);
+  // several pairs: contiguous ranges, which keep the pair walk short
+  const bool strided = grab && npairs == 1;
+  uint32_t np = 1, pool = 0;
   if (grab && pools > 1) {
     // pool = blockIdx % pools, which is the workgroup's XCD when the GPU runs
     // as one partition (workgroups go to the XCDs round robin); every pool
     // has workgroups as long as the grid has at least as many as pools, and
     // a grid with fewer takes one pool (a pool without waves would leave its
     // steps unpacked)
-    const uint32_t np = min((uint32_t)pools, gridDim.x);
-    const uint32_t pool = blockIdx.x % np;
-    const uint64_t nd = nsteps - dyn0, b = dyn0;
-    dyn0 = b + nd * pool / np;
-    dyn1 = b + nd * (pool + 1) / np;
+    np = min((uint32_t)pools, gridDim.x);
+    pool = blockIdx.x % np;
+    if (!strided) {
+      const uint64_t nd = nsteps - dyn0, b = dyn0;
+      dyn0 = b + nd * pool / np;
+      dyn1 = b + nd * (pool + 1) / np;
+    }
     grab += 32 * pool;
   }
-  const uint64_t w0 = grab ? min(total_tiles, gw * sfix * kStep)
-                           : min(total_tiles, gw * nsteps / nw * kStep);
-  uint64_t w1 = grab ? min(total_tiles, (gw + 1) * sfix * kStep)
-                     : min(total_tiles, (gw + 1) * nsteps / nw * kStep);
+  uint64_t si = 0;  // strided: the static steps begun
+  const uint64_t w0 = strided ? (sfix ? min(total_tiles, gw * kStep) : 0)
+                      : grab  ? min(total_tiles, gw * sfix * kStep)
+                              : min(total_tiles, gw * nsteps / nw * kStep);
+  uint64_t w1 = strided ? (sfix ? min(total_tiles, gw * kStep + kStep) : 0)
+                : grab  ? min(total_tiles, (gw + 1) * sfix * kStep)
+                        : min(total_tiles, (gw + 1) * nsteps / nw * kStep);
   uint32_t gv = 0;     // lane 0: the counter's value for the wave's next step
   bool asked = false;  // an atomic for it is in flight
   // the counter's address as a per-lane value (an opaque zero offset): the
@@ -510,12 +521,18 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
     ce = cp + 1 < npairs ? idx[cp + 1].tile_base : total_tiles;
   }
   for (uint64_t base = w0;;) {
+    if (strided && base >= w1 && si + 1 < sfix) {  // the next strided static step
+      ++si;
+      base = (gw + si * nw) * kStep;
+      w1 = min(total_tiles, base + kStep);
+    }
     // one call site for the atomic: at the last static step, at an empty
     // static share, and at every dynamic step (which is its range's last)
-    if (grab && !asked && base + kStep >= w1) ask();
+    if (grab && !asked && (!strided || si + 1 >= sfix) && base + kStep >= w1) ask();
     if (base >= w1) {
       if (!grab) break;
-      const uint64_t st = dyn0 + (uint32_t)__builtin_amdgcn_readfirstlane((int)gv);
+      const uint64_t gval = (uint32_t)__builtin_amdgcn_readfirstlane((int)gv);
+      const uint64_t st = strided ? dyn0 + gval * np + pool : dyn0 + gval;
       asked = false;
       if (st >= dyn1) break;
       base = st * kStep;
