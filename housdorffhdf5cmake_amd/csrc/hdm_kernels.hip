@@ -435,14 +435,18 @@ __device__ __forceinline__ void put(T *p, T v) {
 // evenly and take the rest one step at a time from a counter (with 8 pools,
 // the counter of their workgroup's XCD, over its eighth of those steps), so
 // CUs that stream faster pack more and the pack ends at about the same time
-// on every CU.
-template <int TXB, int TY, int TZ, bool COH, bool MARK = false>
+// on every CU.  LEAN (the near launch, whose only later reader is the region
+// launch): no x-face words, and of the tile flags only bits 0, 1, 14 and 15
+// per mask -- a quarter of the ballots per tile (512^3 pack about 66 -> 57 us
+// measured alone: the waves compute between their steps' loads).
+template <int TXB, int TY, int TZ, bool COH, bool MARK = false, bool LEAN = false>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
                                            uint16_t (*s_pieces)[1024], PackSums *alt = nullptr,
                                            uint32_t *grab = nullptr, int pools = 1) {
   static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8), "near marks: 64x8x8 tiles");
+  static_assert(!LEAN || MARK, "the lean pack is the near launch's");
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -668,10 +672,19 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
         const Row w = m == 0 ? a : b;
-        const uint64_t fxl = group_bits<R>(__ballot(rowok && (w & 1)), lane);
-        const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w >> (TXB - 1)) & 1)), lane);
         const bool any = group_bits<R>(__ballot(w != 0), lane) != 0;
         const bool full = group_bits<R>(__ballot(rowok && w != vb), lane) == 0;
+        constexpr bool kDeep = TXB == 64 && TY == 8 && TZ == 8;
+        bool sxl = false, sxh = false;  // voxels in the eight columns toward -x / +x
+        if constexpr (kDeep) {
+          sxl = group_bits<R>(__ballot(rowok && (w & 0xFFu) != 0), lane) != 0;
+          sxh = group_bits<R>(__ballot(rowok && (w >> (TXB - 8)) != 0), lane) != 0;
+        }
+        flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)sxl << 14 | (uint32_t)sxh << 15)
+                 << (16 * m);
+        if constexpr (LEAN) continue;
+        const uint64_t fxl = group_bits<R>(__ballot(rowok && (w & 1)), lane);
+        const uint64_t fxh = group_bits<R>(__ballot(rowok && ((w >> (TXB - 1)) & 1)), lane);
         const bool fyl = group_bits<R>(__ballot(yl == 0 && w != 0), lane) != 0;
         const bool fyh = group_bits<R>(__ballot(yl == TY - 1 && w != 0), lane) != 0;
         const bool fzl = group_bits<R>(__ballot(zl == 0 && w != 0), lane) != 0;
@@ -679,9 +692,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
         // the two outermost layers toward each face (reach within two steps;
         // read only by the two-steps-per-barrier seed and steps, which run on
         // 64x8x8): the second column from each x face as face words, and flags
-        constexpr bool kDeep = TXB == 64 && TY == 8 && TZ == 8;
         bool gxl = false, gxh = false, gyl = false, gyh = false, gzl = false, gzh = false;
-        bool sxl = false, sxh = false;  // voxels in the eight columns toward -x / +x
         uint64_t fx2l = 0, fx2h = 0;
         if constexpr (kDeep) {
           fx2l = group_bits<R>(__ballot(rowok && ((w >> 1) & 1)), lane);
@@ -692,8 +703,6 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
           gyh = group_bits<R>(__ballot(yl >= TY - 2 && w != 0), lane) != 0;
           gzl = group_bits<R>(__ballot(zl <= 1 && w != 0), lane) != 0;
           gzh = group_bits<R>(__ballot(zl >= TZ - 2 && w != 0), lane) != 0;
-          sxl = group_bits<R>(__ballot(rowok && (w & 0xFFu) != 0), lane) != 0;
-          sxh = group_bits<R>(__ballot(rowok && (w >> (TXB - 8)) != 0), lane) != 0;
         }
         if (in && r == 0) {
           put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile, fxl);
@@ -703,12 +712,10 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
             put<COH>(d.xf_orig[m] + kXf * (uint64_t)tile + 3, fx2h);
           }
         }
-        flags |= ((uint32_t)any | (uint32_t)full << 1 | (uint32_t)(fxl != 0) << 2 |
-                  (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 | (uint32_t)fyh << 5 |
-                  (uint32_t)fzl << 6 | (uint32_t)fzh << 7 | (uint32_t)gxl << 8 |
-                  (uint32_t)gxh << 9 | (uint32_t)gyl << 10 | (uint32_t)gyh << 11 |
-                  (uint32_t)gzl << 12 | (uint32_t)gzh << 13 | (uint32_t)sxl << 14 |
-                  (uint32_t)sxh << 15)
+        flags |= ((uint32_t)(fxl != 0) << 2 | (uint32_t)(fxh != 0) << 3 | (uint32_t)fyl << 4 |
+                  (uint32_t)fyh << 5 | (uint32_t)fzl << 6 | (uint32_t)fzh << 7 |
+                  (uint32_t)gxl << 8 | (uint32_t)gxh << 9 | (uint32_t)gyl << 10 |
+                  (uint32_t)gyh << 11 | (uint32_t)gzl << 12 | (uint32_t)gzh << 13)
                  << (16 * m);
       }
       if (in && r == 0) put<COH>(d.tflags + tile, flags);
@@ -3714,10 +3721,11 @@ __attribute__((amdgpu_waves_per_eu(1))) void dilate_tb_kernel(DilateArgs a) {
 
 // ---------------------------------------------------------------- near pass
 // The single-pair launch for pairs whose voxels lie near the other mask
-// (dilate_near_kernel; 64x8x8 tiles, 6 element).  It packs the pair as the
-// two-step launch does, and the pack marks, in a bitmap, every region of
+// (dilate_near_kernel; 64x8x8 tiles, 6 element).  It packs the pair's rows,
+// counts and box as the two-step launch does, and marks, in a bitmap, every region of
 // 1 x 2 x 2 tiles that holds voxels to be covered (FP for gold's dilation, FN
-// for segm's).  After the one grid barrier that ends the pack, each workgroup
+// for segm's); it writes no x-face words and only the tile flags the region
+// launch reads (non-empty, full, voxels in the eight columns toward -x / +x).  After the one grid barrier that ends the pack, each workgroup
 // reads the bitmap and takes an equal share of the marked (region, mask)
 // entries -- every workgroup counts the marks itself, so no queue, counter or
 // second barrier is needed -- and for each one dilates the *other* mask's
@@ -3949,10 +3957,11 @@ __attribute__((amdgpu_waves_per_eu(2))) void dilate_near_kernel(DilateArgs a) {
     for (uint32_t i = gt; i < nreg; i += gn) st_sc(d.rstate[0] + i, 0u), st_sc(d.rstate[1] + i, 0u);
     for (uint32_t i = gt; i < a.marks_zero_words; i += gn) st_sc(a.marks_zero + i, 0u);
   }
-  // P: pack, every store written through, regions marked
+  // P: pack, every store written through, regions marked; lean: the rows,
+  // counts, box, marks and the tile flags the region launch reads
   {
     __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
-    pack_tiles<64, 8, 8, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc,
+    pack_tiles<64, 8, 8, true, true, true>(&d, nullptr, ctrs, 1, a.total_tiles, s_pieces, acc,
                                      a.pack_pools ? &acc->grab[0] : nullptr, a.pack_pools);
   }
   if (blockIdx.x == 0) ph1 = wall_clock64();
