@@ -663,7 +663,15 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const uint32_t tile = tileU[u];
       const Row a = aU[u], b = bU[u];
       const uint64_t o = (uint64_t)tile * R + r;
-      if (rowok) {
+      if constexpr (LEAN) {
+        // a mask with no voxel in the tile writes no rows there: the readers
+        // of the near launch's pack (near_window, the region launch) take
+        // zero rows wherever the tile's flag says the mask is empty
+        // (512^3 pack measured alone, a fifth of the tiles non-empty: 58 ->
+        // 53 us)
+        if (rowok && __ballot(a != 0) != 0) put<COH>((Row *)d.orig[0] + o, a);
+        if (rowok && __ballot(b != 0) != 0) put<COH>((Row *)d.orig[1] + o, b);
+      } else if (rowok) {
         put<COH>((Row *)d.orig[0] + o, a);
         put<COH>((Row *)d.orig[1] + o, b);
       }
@@ -3203,6 +3211,11 @@ __attribute__((amdgpu_waves_per_eu(1))) void dilate_tb_kernel(DilateArgs a) {
     }
     bool rok[4];
     uint64_t wo[4][3], wb[4][3], oth[4];
+    // the three tiles' flags with the rows: a mask empty in a tile has zero
+    // original rows there, which the near launch's pack does not write
+    uint32_t tfl[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) tfl[c] = ldg(d.tflags + tix[c]);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int zl = (zz0 + i) & 7;
@@ -3219,12 +3232,15 @@ __attribute__((amdgpu_waves_per_eu(1))) void dilate_tb_kernel(DilateArgs a) {
       }
       oth[i] = ldg(orig_o + (uint64_t)tix[1] * R + r);
     }
+    const bool oth_any = (tfl[1] >> (m ? 0 : 16)) & 1u;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (!(core && check && rok[i] && tin[1])) oth[i] = 0;
+      if (!(core && check && rok[i] && tin[1] && oth_any)) oth[i] = 0;
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 3; ++c) {
         if (!(rok[i] && tin[c])) wo[i][c] = wb[i][c] = 0;
+        if (!((tfl[c] >> (m ? 16 : 0)) & 1u)) wo[i][c] = 0;
+      }
     }
     __syncthreads();
     const int oyr = tyi < 0 ? -1 : tyi > 1 ? 1 : 0, ozr = tzi < 0 ? -1 : tzi > 1 ? 1 : 0;
@@ -3799,6 +3815,11 @@ __device__ __forceinline__ void near_window(const PairDesc &d, uint32_t rg, int 
   // the barrier, so plain loads see them
   bool rok[4];
   uint64_t w[4][3], oth[4];
+  // the three tiles' flags with the rows: a mask empty in a tile has zero
+  // rows there, which the near launch's pack does not write
+  uint32_t tfl[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) tfl[c] = ldg(d.tflags + tix[c]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int zl = (zz0 + i) & 7;
@@ -3810,11 +3831,13 @@ __device__ __forceinline__ void near_window(const PairDesc &d, uint32_t rg, int 
   }
   TbRow X[4], V[4];
   uint32_t need = 0;
+  const bool oth_any = (tfl[1] >> (m ? 0 : 16)) & 1u;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
+    if (!oth_any) oth[i] = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      if (!(rok[i] && tin[c])) w[i][c] = 0;
+      if (!(rok[i] && tin[c] && ((tfl[c] >> (m ? 16 : 0)) & 1u))) w[i][c] = 0;
     V[i] = rok[i] ? tb_make(vb[0], vb[1], vb[2]) : TbRow{0, 0};
     X[i] = tb_make(w[i][0], w[i][1], w[i][2]);
     X[i].o &= V[i].o, X[i].h &= V[i].h;
