@@ -142,6 +142,7 @@ int tb_mode();
 int fused_grid(const Ctx *c, int tile, int requested, bool ds, uint64_t ntiles);
 // the same from the kernel's resident workgroups per CU and the CU count
 int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles);
+int near_grid_of(int occ, int cus, int requested, uint64_t ntiles);
 // the grid of the region launch (regions of 1 x 2 x 2 tiles)
 int tb_grid(int occ, int cus, int requested, uint64_t ntiles, uint32_t nregions);
 GroupPtrs carve_common(char *base, const CommonLayout &L);

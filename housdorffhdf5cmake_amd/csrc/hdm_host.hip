@@ -246,6 +246,15 @@ int fused_grid_of(int occ, int cus, int requested, uint64_t ntiles) {
   if (requested > 0) return std::min(requested, maxres);
   return (int)std::min<uint64_t>((uint64_t)maxres, std::max<uint64_t>(64, ntiles / 2));
 }
+// Default grid of the near launch: the fused rule over at most two workgroups
+// per CU.  Its pack streams as fast on two as on three, and every workgroup
+// more costs the pack barrier, the share count and the arrival count (CT
+// pair 52.8 -> 49.3 us with 512 instead of 768 on 256 CUs; the 512^3 cube,
+// with more near-pass entries than workgroups, 74.0 -> 74.4 us).  A requested
+// grid keeps the fused rule's bound.
+int near_grid_of(int occ, int cus, int requested, uint64_t ntiles) {
+  return fused_grid_of(requested > 0 ? occ : std::min(occ, 2), cus, requested, ntiles);
+}
 // Grid of the region launch: the fused launch's rule over tiles (the launch
 // may run the two-step body), at most four workgroups per region and at most
 // 2 x kWgThreads in all (its seed period publishes one count per workgroup
@@ -638,7 +647,7 @@ static int hausdorff_core(const void *gold, const void *segm, int dtype, int lab
     if (cfg.ev) e = hipEventRecord(cfg.ev_mode == 3 ? cfg.ev[2] : cfg.ev[0], s);
     if (e == hipSuccess) {
       if (near_path) {
-        e = tl->fused_near(dim3(fused_grid_of(c->occ_fused_near[tile], c->cus, opts.grid_ctas,
+        e = tl->fused_near(dim3(near_grid_of(c->occ_fused_near[tile], c->cus, opts.grid_ctas,
                                               g.ntiles)),
                            s, a);
         if (e == hipSuccess) c->marks_set ^= 1;

@@ -222,10 +222,9 @@ extern "C" int hdm_hausdorff_u8_noalloc(const uint8_t *gold, const uint8_t *segm
     a.marks_zero_words = (uint32_t)kMarkSetWords;
     a.desc.marks = a.marks;  // the pack marks through the descriptor
   }
-  const int grid = fused_grid_of(near_path ? D.occ_fused_near[tile]
-                                 : ds      ? D.occ_fused_ds[tile]
-                                           : D.occ_fused[tile],
-                                 D.cus, opts.grid_ctas, g.ntiles);
+  const int grid = near_path ? near_grid_of(D.occ_fused_near[tile], D.cus, opts.grid_ctas, g.ntiles)
+                             : fused_grid_of(ds ? D.occ_fused_ds[tile] : D.occ_fused[tile], D.cus,
+                                             opts.grid_ctas, g.ntiles);
   hipError_t e = (near_path ? tl->fused_near : ds ? tl->fused_ds : tl->fused)(dim3(grid), s, a);
   if (e == hipSuccess) {
     D.acc_set ^= 1;  // the launch zeroes the other set for the next one

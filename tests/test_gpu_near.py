@@ -155,3 +155,27 @@ def test_small_grids_take_every_pack_step(grid):
         bad = {k: (res[k], want[k]) for k in KEYS if res[k] != want[k]}
         assert not bad, (grid, g.shape, bad)
         assert np.array_equal(res["hist_fp"], hf) and np.array_equal(res["hist_fn"], hn)
+
+
+@pytest.mark.parametrize("far", [False, True])
+def test_empty_tiles_after_a_dense_pair_read_as_empty(far):
+    """The near launch's pack writes no rows of a mask that is empty in a
+    tile, so after a dense pair of the same size the packed buffers still
+    hold that pair's rows there; the near pass and the region launch must
+    take those tiles as empty from their flags.  A dense pair (most tiles
+    full) runs first, then a sparse pair of the same dims (a small ball and
+    a shifted copy, plus with far=True a voxel 30 steps away, which hands the
+    pair to the region launch); the sparse pair equals the oracle."""
+    shape = (48, 64, 256)
+    rng = np.random.default_rng(7)
+    dense_g = (rng.random(shape) < 0.6).astype(np.uint8)
+    dense_s = (rng.random(shape) < 0.6).astype(np.uint8)
+    check(dense_g, dense_s, what="dense pair")
+    g = blob(shape, (24, 32, 128), 9)
+    s = np.roll(g, 1, axis=2)
+    if far:
+        s[24, 32, 137 + 30] = 1  # the ball's +x pole is x = 137
+    res, lt = check(g, s, what=f"sparse pair after dense, far={far}")
+    assert lt["launches"] == (2 if far else 1), (res["hd"], lt)
+    if far:
+        assert res["hd"] == 30
