@@ -148,6 +148,14 @@ def roofline_obj(kernel, algo_bytes, ms, traffic_name, **extra):
     if os.path.exists(tf):
         with open(tf) as f:
             out["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    if out["traffic"]:
+        # the same launch time over the counted HBM bytes: below the
+        # algorithmic fraction where the launch moves fewer bytes than the
+        # algorithm's count (the near launch writes no rows of a mask that is
+        # empty in a tile), above it where it re-reads
+        cg = out["traffic"] / (ms * 1e-3) / 1e9
+        out["achieved_counted"] = round(cg, 1)
+        out["frac_counted"] = round(cg / HBM_PEAK_GBS, 4)
     out.update(extra)
     return out
 
