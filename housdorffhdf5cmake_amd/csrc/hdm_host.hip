@@ -220,6 +220,15 @@ static int tb_setting() {
 }
 bool tb_enabled() { return tb_setting() != 0; }
 int tb_mode() { return tb_setting(); }
+// HDM_SKIP_ROWS=0: a batch with a near pass writes every row in its pack (no
+// rows skipped, no zero fill afterwards)
+bool skip_rows() {
+  static const bool on = [] {
+    const char *e = std::getenv("HDM_SKIP_ROWS");
+    return !(e && std::atoi(e) == 0);
+  }();
+  return on;
+}
 int pack_pools() {
   static const int v = [] {
     const char *e = std::getenv("HDM_PACK_POOLS");
@@ -328,15 +337,22 @@ hipError_t enqueue_group(hipStream_t s, const TileLaunch *tl, const GroupPtrs &P
   if (e != hipSuccess || (e = mark(0)) != hipSuccess) return e;
   (void)total_rw;
   // several pairs: the last quarter of the steps from the engine's per-XCD
-  // counters, which the reset above zeroed
-  e = tl->pack(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs, total_tiles,
-               npairs > 1 && pack_pools() ? &P.engine->pack_grab[0] : nullptr);
+  // counters, which the reset above zeroed.  A batch whose near pass runs
+  // writes no rows of a mask that is empty in a tile (the near pass reads
+  // such rows as zero from the tile flags) and zero-fills them afterwards
+  // for the pairs the near pass leaves to the seed and the dilation
+  const bool skip = cfg.near_list && npairs > 1 && tl->pack_skip && tl->fill && skip_rows();
+  e = (skip ? tl->pack_skip : tl->pack)(dim3(cfg.pack_blocks), s, P.descs, P.idx, P.ctrs, npairs,
+                                        total_tiles,
+                                        npairs > 1 && pack_pools() ? &P.engine->pack_grab[0] : nullptr);
   if (e != hipSuccess) return e;
   if (npairs > 1 && (e = hdm_launch_totals(s, P.ctrs, 0, npairs)) != hipSuccess) return e;
   if (cfg.near_list && npairs > 1 &&
       (e = hdm_launch_near_batch(s, P.descs, P.idx, P.ctrs, npairs, cfg.near_words,
                                  &P.engine->near_count,
                                  cfg.near_list, cfg.near_cap, cfg.near_grid)) != hipSuccess)
+    return e;
+  if (skip && (e = tl->fill(dim3(cfg.pack_blocks), s, P.descs, P.ctrs, npairs)) != hipSuccess)
     return e;
   if ((e = mark(1)) != hipSuccess) return e;
   e = tl->seed(dim3(cfg.seed_blocks), s, P.descs, P.idx, P.ctrs, npairs, P.engine, P.q, qcap,

@@ -438,8 +438,12 @@ __device__ __forceinline__ void put(T *p, T v) {
 // on every CU.  LEAN (the near launch, whose only later reader is the region
 // launch): no x-face words, and of the tile flags only bits 0, 1, 14 and 15
 // per mask -- a quarter of the ballots per tile (512^3 pack about 66 -> 57 us
-// measured alone: the waves compute between their steps' loads).
-template <int TXB, int TY, int TZ, bool COH, bool MARK = false, bool LEAN = false>
+// measured alone: the waves compute between their steps' loads).  SKIP
+// (LEAN's default; a batch with a near pass): no rows of a mask that is
+// empty in a tile -- its readers take them as zero from the tile flags, or
+// fill_kernel writes them for the pairs a batch's near pass leaves.
+template <int TXB, int TY, int TZ, bool COH, bool MARK = false, bool LEAN = false,
+          bool SKIP = LEAN>
 __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            const PackIndex *__restrict__ idx, PairCtr *ctrs,
                                            int npairs, uint64_t total_tiles,
@@ -447,6 +451,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
                                            uint32_t *grab = nullptr, int pools = 1) {
   static_assert(!MARK || (TXB == 64 && TY == 8 && TZ == 8), "near marks: 64x8x8 tiles");
   static_assert(!LEAN || MARK, "the lean pack is the near launch's");
+  static_assert(!SKIP || MARK, "rows of empty masks are skipped on 64x8x8 tiles with marks");
   using Row = RowT<TXB>;
   constexpr int R = TY * TZ;
   constexpr int TPW = kWave / R;  // tiles per wave step
@@ -663,7 +668,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
       const uint32_t tile = tileU[u];
       const Row a = aU[u], b = bU[u];
       const uint64_t o = (uint64_t)tile * R + r;
-      if constexpr (LEAN) {
+      if constexpr (SKIP) {
         // a mask with no voxel in the tile writes no rows there: the readers
         // of the near launch's pack (near_window, the region launch) take
         // zero rows wherever the tile's flag says the mask is empty
@@ -780,7 +785,7 @@ __device__ __forceinline__ void pack_tiles(const PairDesc *__restrict__ descs,
   }
 }
 
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, bool SKIP = false>
 __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__restrict__ descs,
                                                           const PackIndex *__restrict__ idx,
                                                           PairCtr *ctrs, int npairs,
@@ -788,8 +793,42 @@ __global__ __launch_bounds__(kWgThreads) void pack_kernel(const PairDesc *__rest
   // per wave: the 16-bit compare pieces of 64 rows x 8 chunks, gold then segm
   __shared__ __align__(16) uint16_t s_pieces[kWavesPerWg][1024];
   constexpr bool kMark = TXB == 64 && TY == 8 && TZ == 8;  // pairs with a bitmap get marks
-  pack_tiles<TXB, TY, TZ, false, kMark>(descs, idx, ctrs, npairs, total_tiles, s_pieces, nullptr,
-                                        grab, 8);
+  pack_tiles<TXB, TY, TZ, false, kMark, false, SKIP>(descs, idx, ctrs, npairs, total_tiles,
+                                                     s_pieces, nullptr, grab, 8);
+}
+
+// Zero rows for the masks a skipping batch pack (pack_kernel<64, 8, 8, true>)
+// left unwritten -- a mask empty in a tile, flag bit 0 / 16 clear -- of the
+// pairs the batched near pass did not finish: their seed and dilation read
+// every row, while the near pass takes such rows as zero from the flags.  One
+// wave per tile, lane r row r; a finished pair (near_done) is skipped.
+__global__ __launch_bounds__(kWgThreads) void fill_kernel(const PairDesc *__restrict__ descs,
+                                                          const PairCtr *__restrict__ ctrs,
+                                                          int npairs) {
+  constexpr int R = 64;
+  const int lane = threadIdx.x & 63;
+  const uint32_t gw = blockIdx.x * kWavesPerWg +
+                      (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t nw = gridDim.x * kWavesPerWg;
+  // lane i checks pair p0 + i (the counters lie kilobytes apart: one load
+  // per pair in turn would cost a round trip each); the wave walks the pairs
+  // left open
+  for (int p0 = 0; p0 < npairs; p0 += kWave) {
+    const int pl = p0 + lane;
+    uint64_t left = __ballot(pl < npairs && ctrs[pl].near_done == 0);
+    while (left) {
+      const int p = p0 + __builtin_ctzll(left);
+      left &= left - 1;
+      const PairDesc &d = descs[p];
+      uint64_t *o0 = (uint64_t *)d.orig[0], *o1 = (uint64_t *)d.orig[1];
+      for (uint32_t tile = gw; tile < d.ntiles; tile += nw) {
+        const uint32_t f = (uint32_t)__builtin_amdgcn_readfirstlane((int)d.tflags[tile]);
+        const uint64_t o = (uint64_t)tile * R + lane;
+        if (!(f & 1u)) o0[o] = 0ull;
+        if (!(f & 0x10000u)) o1[o] = 0ull;
+      }
+    }
+  }
 }
 
 // The pack pass of a pair whose two masks were packed before, each on its own
@@ -4344,11 +4383,16 @@ __global__ __launch_bounds__(kWgThreads) void near_fold_kernel(const PairDesc *_
 }
 
 // ---------------------------------------------------------------- launchers
-template <int TXB, int TY, int TZ>
+template <int TXB, int TY, int TZ, bool SKIP = false>
 hipError_t launch_pack(dim3 grid, hipStream_t s, const PairDesc *d, const PackIndex *ix,
                        PairCtr *c, int np, uint64_t tw, uint32_t *grab) {
-  hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ>), grid, dim3(kWgThreads), 0, s, d, ix, c, np, tw,
-                     grab);
+  hipLaunchKernelGGL((pack_kernel<TXB, TY, TZ, SKIP>), grid, dim3(kWgThreads), 0, s, d, ix, c, np,
+                     tw, grab);
+  return hipGetLastError();
+}
+// the rows a skipping batch pack left for the pairs its near pass leaves (64x8x8)
+hipError_t launch_fill(dim3 grid, hipStream_t s, const PairDesc *d, const PairCtr *c, int np) {
+  hipLaunchKernelGGL(fill_kernel, grid, dim3(kWgThreads), 0, s, d, c, np);
   return hipGetLastError();
 }
 template <int TXB, int TY, int TZ>
@@ -4430,6 +4474,8 @@ TileLaunch make_launch() {
                     kDS ? (const void *)&dilate_near_kernel : nullptr,
                     (const void *)&pack_kernel<TXB, TY, TZ>,
                     &launch_combine<TXB, TY, TZ>,
+                    kDS ? &launch_pack<64, 8, 8, true> : nullptr,
+                    kDS ? &launch_fill : nullptr,
                     TXB,
                     TY,
                     TZ,

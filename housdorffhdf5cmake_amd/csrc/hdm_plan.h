@@ -100,6 +100,12 @@ struct TileLaunch {
   // and the pair's tile flags from the masks' own flags
   hipError_t (*combine)(dim3, hipStream_t, const PairDesc *, PairCtr *, const uint32_t *,
                         const uint32_t *, uint64_t);
+  // a batch whose near pass runs (64x8x8, 6 element; null otherwise): the
+  // pack that writes no rows of a mask empty in a tile, and after the near
+  // pass the zero rows of the pairs it did not finish
+  hipError_t (*pack_skip)(dim3, hipStream_t, const PairDesc *, const PackIndex *, PairCtr *, int,
+                          uint64_t, uint32_t *);
+  hipError_t (*fill)(dim3, hipStream_t, const PairDesc *, const PairCtr *, int);
   int txb, ty, tz;
   int conn;   // structuring element of the seed and dilation kernels
   int steps;  // dilation steps per engine iteration of the seed and dilation kernels

@@ -63,15 +63,18 @@ def run_here():
 
 @pytest.mark.parametrize("switch,value", [("HDM_DS", "0"), ("HDM_FUSED", "0"), ("HDM_COOP", "1"),
                                           ("HDM_TB", "0"), ("HDM_TB", "2"), ("HDM_NEAR", "0"),
-                                          ("HDM_PACK_POOLS", "0")])
+                                          ("HDM_PACK_POOLS", "0"), ("HDM_SKIP_ROWS", "0")])
 def test_switched_off_paths_give_the_same_results(switch, value):
     """HDM_COOP=1 (the fused launch as a cooperative launch), HDM_TB=0 (the
     two-step fused kernel only: no near pass, no region launch), HDM_TB=2 (the
     region body for every pair, right after the pack), HDM_NEAR=0 (the
     two-step fused kernel first, handing pairs whose HD bound reaches 16 steps
     to the region launch) and HDM_PACK_POOLS=0 (the fused pack in even static
-    shares, without the per-XCD counters for its last quarter) are compared
-    the same way."""
+    shares, without the per-XCD counters for its last quarter) and
+    HDM_SKIP_ROWS=0 (a batch writes every row in its pack, where by default it
+    skips the rows of masks empty in a tile and zero-fills them afterwards for
+    the pairs its near pass leaves -- here the long pair) are compared the
+    same way."""
     here = run_here()
     env = dict(os.environ, **{switch: value})
     r = subprocess.run([sys.executable, "-c", CHILD, ROOT, json.dumps(SPECS), json.dumps(KEYS)],
